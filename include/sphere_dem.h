@@ -1,0 +1,171 @@
+/*
+ * sphere_dem.h -- C-ABI of the MI355X (gfx950) DEM time step.
+ *
+ * The reference's boundary for this path is the C++ DEM class plus the .bin
+ * file (sphere, src/sphere.h:248-321; src/file_io.cpp:29-490).  Each entry
+ * point below replaces one piece of that class; the reference member it
+ * stands in for is named beside it.  Plain pointers and sizes only; every
+ * call returns SPH_OK (0) or a negative SPH_E_* code instead of exiting.
+ * A handle owns one HIP stream and its device buffers; it is not thread safe.
+ * Several handles may coexist: no global constant memory is used.
+ */
+#ifndef SPHERE_DEM_H_
+#define SPHERE_DEM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_ABI_VERSION 1
+/* contact slots per particle (src/constants.h:24) */
+#define SPH_NC 32
+
+typedef struct sph_dem sph_t;
+
+enum sph_status {
+    SPH_OK = 0,
+    SPH_E_IO = -1,               /* file missing, short read, short write      */
+    SPH_E_INVALID = -2,          /* checkValues failure or bad argument        */
+    SPH_E_OUT_OF_GRID = -3,      /* particle more than one cell off the grid   */
+    SPH_E_CONTACT_OVERFLOW = -4, /* more than SPH_NC contacts on a particle    */
+    SPH_E_NAN = -5,              /* non-finite position                        */
+    SPH_E_HIP = -6,              /* HIP runtime error                          */
+    SPH_E_STATE = -7,            /* call not valid in the handle's state       */
+    SPH_E_UNSUPPORTED = -8       /* input uses a feature outside this path     */
+};
+
+/* Field ids for sph_get / sph_set.  Shapes in entries of f64 or u32,
+ * original particle order, the reference's own layout. */
+enum sph_field {
+    SPH_X = 0,          /* np x 4 f64: x, y, z, r                              */
+    SPH_VEL = 1,        /* np x 4 f64: vx, vy, vz, fixvel                      */
+    SPH_ANGVEL = 2,     /* np x 3 f64                                          */
+    SPH_FORCE = 3,      /* np x 3 f64                                          */
+    SPH_TORQUE = 4,     /* np x 3 f64                                          */
+    SPH_ANGPOS = 5,     /* np x 3 f64                                          */
+    SPH_XYSUM = 6,      /* np x 2 f64                                          */
+    SPH_VEL0 = 7,       /* np x 3 f64, half-step velocity                      */
+    SPH_ANGVEL0 = 8,    /* np x 3 f64                                          */
+    SPH_ES_DOT = 9,     /* np f64                                              */
+    SPH_ES = 10,
+    SPH_EV_DOT = 11,
+    SPH_EV = 12,
+    SPH_P = 13,
+    SPH_CONTACTS = 14,  /* np x NC u32, empty slot = np  (get only)            */
+    SPH_DISTMOD = 15,   /* np x NC x 4 f64 (dx, dy, dz, r_b) (get only)        */
+    SPH_DELTA_T = 16,   /* np x NC x 3 f64 (get only)                          */
+    SPH_CELL_KEYS = 17, /* np u32, cell id per original particle (get only)    */
+    SPH_SORT_IDX = 18,  /* np u32, original index per sorted position (get)    */
+    SPH_CELL_START = 19,/* K u32, 0xffffffff for an empty cell (get only)      */
+    SPH_CELL_END = 20,  /* K u32, 0xffffffff for an empty cell (get only)      */
+    SPH_BONDS = 21,     /* nb0 x 2 u32                                         */
+    SPH_BONDS_DELTA = 22, /* nb0 x 4 f64: t.x, t.y, t.z, n                     */
+    SPH_BONDS_OMEGA = 23, /* nb0 x 4 f64                                       */
+    SPH_WALLS_NX = 24,  /* nw x 4 f64: nx, ny, nz, pos                         */
+    SPH_WALLS_MVFD = 25,/* nw x 4 f64: mass, vel, force, devs                  */
+    SPH_WALLS_VEL0 = 26,/* nw f64                                              */
+    SPH_WALL_PP = 27,   /* nw x np f64, per-particle wall force (get only)     */
+    SPH_TIME = 28,      /* 4 f64: dt, current, total, file_dt                  */
+    SPH_X_SORTED = 29,  /* np x 4 f64 (get only)                               */
+    SPH_NFIELDS = 30
+};
+
+/* Arithmetic of the pair forces. */
+enum sph_precision {
+    SPH_PREC_FP64 = 0,  /* everything in FP64 (the reference's Float=double)  */
+    SPH_PREC_MIXED = 1  /* pair forces in FP32; positions, velocities and the
+                           per-particle force sums in FP64                    */
+};
+
+/* Sizes of an open handle. */
+typedef struct sph_info {
+    uint64_t np, nb0, nw, nc, ncells;
+    uint32_t num[3];
+    int32_t periodic;
+    double dt, t_current, t_total, file_dt;
+    uint64_t steps_done;
+    int32_t precision;
+} sph_info_t;
+
+/* Contact parameters a caller may change between steps (the differentiable
+ * set of the adjoint plus the tangential pair). */
+typedef struct sph_params {
+    double k_n, k_t, gamma_n, gamma_t, mu_s, mu_d;
+} sph_params_t;
+
+/* Per-kernel timing of the last sph_step call (hipEvent, milliseconds,
+ * summed over the steps of the call; valid when profiling is enabled). */
+typedef struct sph_timing {
+    double bonds_ms, cellid_ms, sort_ms, reorder_ms, contact_ms, walls_ms;
+    double total_ms;
+    uint64_t steps;
+} sph_timing_t;
+
+/* DEM::DEM + readbin + checkValues + allocateGlobalDeviceMemory +
+ * transferToGlobalDeviceMemory (src/sphere.cpp:16-77, src/file_io.cpp:29-301,
+ * src/sphere.cpp:104-246, src/device.cu:233-324, :394-488).
+ * check_vals=1 applies checkValues; verbosity 0 is silent. */
+int sph_open(const char* bin_path, int verbosity, int check_vals, sph_t** out);
+
+/* The reference's dry run (DEM(..., dry=1), src/sphere.cpp:42-48): parse an
+ * in-memory .bin image and apply checkValues without touching the GPU.  When
+ * out is non-NULL the parsed state is serialised again into out (capacity
+ * out_cap bytes, size written to *out_len) for byte-level round trips. */
+int sph_dry_run(const void* bin, size_t bytes, int check_vals, void* out, size_t out_cap,
+                size_t* out_len, char* err, size_t err_len);
+
+/* Same as sph_open for an in-memory .bin image. */
+int sph_open_mem(const void* bin, size_t bytes, int verbosity, int check_vals, sph_t** out);
+
+/* ~DEM + freeGlobalDeviceMemory (src/sphere.cpp:80-100, src/device.cu:326-391). */
+void sph_close(sph_t* h);
+
+/* Message of the last failing call on this handle (or of the last failing
+ * sph_open on this thread when h is NULL). */
+const char* sph_last_error(const sph_t* h);
+
+/* n time steps of the loop body of DEM::startTime (src/device.cu:731-1111)
+ * without file output; time.current advances by dt per step. */
+int sph_step(sph_t* h, uint64_t n);
+
+/* DEM::startTime (src/device.cu:576-1299): steps while
+ * time.current <= time.total, writing <outdir>/<sid>.outputNNNNN.bin every
+ * file_dt and <outdir>/<sid>.status.dat. */
+int sph_run(sph_t* h, const char* outdir, const char* sid);
+
+/* DEM::writebin (src/file_io.cpp:304-490). */
+int sph_writebin(sph_t* h, const char* path);
+
+/* DEM::checkValues (src/sphere.cpp:104-246) on the current device state. */
+int sph_check_values(sph_t* h);
+
+/* transferFromGlobalDeviceMemory / transferToGlobalDeviceMemory for one
+ * field (src/device.cu:394-572); dst/src are host pointers owned by the
+ * caller and bytes must equal sph_field_bytes. */
+size_t sph_field_bytes(const sph_t* h, int field);
+int sph_get(sph_t* h, int field, void* dst, size_t bytes);
+int sph_set(sph_t* h, int field, const void* src, size_t bytes);
+
+int sph_info(const sph_t* h, sph_info_t* out);
+int sph_get_params(const sph_t* h, sph_params_t* out);
+/* Replaces the constant-memory upload of Params (src/device.cu:201-228). */
+int sph_set_params(sph_t* h, const sph_params_t* in);
+
+/* Chooses the pair-force arithmetic; a handle opens in SPH_PREC_FP64. */
+int sph_set_precision(sph_t* h, int precision);
+
+/* 1 = time each kernel with hipEvents in sph_step (the PROFILING flag of
+ * src/debug.h:7); 0 = off (default). */
+int sph_set_profiling(sph_t* h, int on);
+int sph_get_timing(const sph_t* h, sph_timing_t* out);
+
+/* 1 = replay the captured step as a HIP graph (default), 0 = plain launches. */
+int sph_set_graph(sph_t* h, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

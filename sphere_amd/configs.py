@@ -1,0 +1,235 @@
+"""Seeded synthetic inputs for the benchmark and parity workloads.
+
+Each builder returns a :class:`~sphere_amd.spherebin.Spherebin` that writes a
+reference-layout ``.bin``.  Seeds default to ``20261202 + index``.  Setup
+conventions follow the reference's Python front end where it has one:
+grid cells at least one maximum diameter (python/sphere.py:909-937), an upper
+wall placed on the highest particle with the default wall mass
+rho*np*pi*(cell/2)^3 (python/sphere.py:1199-1242), stress-controlled
+consolidation (python/sphere.py:1243-1258) and fixed-velocity shear layers
+(python/sphere.py:1297-1356).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .spherebin import Spherebin
+
+BASE_SEED = 20261202
+
+
+def _dt(sb: Spherebin, factor: float = 0.17) -> float:
+    r_min = float(np.min(sb.radius))
+    m_min = 4.0 / 3.0 * math.pi * r_min ** 3 * sb.rho
+    return factor * math.sqrt(m_min / max(sb.k_n, sb.k_t))
+
+
+def _contact_params(sb: Spherebin, k_n=1e6, k_t=1e6, gamma_n=100.0, gamma_t=100.0, mu=0.5, rho=2600.0):
+    sb.k_n, sb.k_t = float(k_n), float(k_t)
+    sb.gamma_n, sb.gamma_t = float(gamma_n), float(gamma_t)
+    sb.mu_s = sb.mu_d = float(mu)
+    sb.gamma_wn, sb.gamma_wt = float(gamma_n), float(gamma_t)
+    sb.mu_ws = sb.mu_wd = float(mu)
+    sb.rho = float(rho)
+    sb.contactmodel = 2
+
+
+def _grid(sb: Spherebin, L, cell_min: float):
+    sb.origo = np.zeros(3)
+    sb.L = np.asarray(L, dtype=np.float64)
+    sb.num = np.maximum(1, np.floor(sb.L / cell_min)).astype(np.uint32)
+
+
+def lognormal_radii(rng: np.random.Generator, n: int, median=1e-2, sigma=0.2, lo=0.5, hi=2.0):
+    r = median * np.exp(sigma * rng.standard_normal(n))
+    return np.clip(r, lo * median, hi * median)
+
+
+def lattice_settle(nside=(16, 16, 16), spacing=2.4e-2, jitter=1e-3, r_lo=0.9e-2, r_hi=1.1e-2,
+                   periodic=1, gravity=-9.81, steps=10000, seed=BASE_SEED, nw=0,
+                   sid="c0-settle", k_n=1e6, k_t=1e6, gamma_n=100.0, gamma_t=100.0,
+                   mu=0.5) -> Spherebin:
+    """Jittered cubic lattice settling under gravity onto the floor.
+
+    With the defaults this is the 4,096-sphere, x/y-periodic settling case:
+    radii U[0.9, 1.1]e-2 m, rho 2600, k_n = k_t = 1e6, gamma = 100, mu = 0.5,
+    dt = 0.17 sqrt(m_min/k_n), 10,000 steps.
+    """
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = nside
+    n = nx * ny * nz
+    sb = Spherebin(n, nw=0, sid=sid)
+    sb.set_walls(nw)
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float64)
+    # original order: x fastest, then y, then z
+    sites = sites[np.lexsort((sites[:, 0], sites[:, 1], sites[:, 2]))]
+    sb.x = (sites + 0.5) * spacing + rng.uniform(-jitter, jitter, size=(n, 3))
+    sb.radius = rng.uniform(r_lo, r_hi, size=n)
+    _contact_params(sb, k_n=k_n, k_t=k_t, gamma_n=gamma_n, gamma_t=gamma_t, mu=mu)
+    L = np.array([nx * spacing, ny * spacing, nz * spacing])
+    _grid(sb, L, 2.0 * r_hi)
+    sb.periodic = int(periodic)
+    sb.g = np.array([0.0, 0.0, gravity])
+    sb.time_dt = _dt(sb)
+    sb.time_current = 0.0
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    if nw:
+        sb.w_x[0] = L[2]
+        sb.w_m[0] = sb.rho * n * math.pi * (L[0] / sb.num[0] / 2.0) ** 3
+    return sb
+
+
+def random_compression(n=1048576, box=(2.07, 2.07, 4.13), periodic=1, stress=10e3, gravity=-9.81,
+                       steps=2000, seed=BASE_SEED + 1, sid="c1-compress", wall_mass_scale=1.0,
+                       height_margin=1.1) -> Spherebin:
+    """Random non-overlapping lognormal packing under a stress-controlled top wall.
+
+    Radii: lognormal, median 1e-2 m, sigma 0.2, clipped to [0.5, 2] x median.
+    The 0.8 x 0.8 x 1.6 m box of the nominal workload cannot hold 1,048,576
+    such spheres (5.26 m^3 of solid), so the default box is 2.07 x 2.07 x
+    4.13 m (packing fraction about 0.3); the grid height is 1.1 x the packing
+    height as adjustUpperWall leaves it.
+    """
+    from .dem import pack_rsa
+
+    rng = np.random.default_rng(seed)
+    sb = Spherebin(n, nw=1, sid=sid)
+    sb.radius = lognormal_radii(rng, n)
+    _contact_params(sb)
+    box = np.asarray(box, dtype=np.float64)
+    sb.x = pack_rsa(sb.radius, box, periodic, seed)
+    r_max = float(np.max(sb.radius))
+    L = np.array([box[0], box[1], box[2] * height_margin])
+    _grid(sb, L, 2.0 * r_max)
+    sb.periodic = int(periodic)
+    sb.g = np.array([0.0, 0.0, gravity])
+    sb.set_walls(1)
+    sb.wmode = np.array([1], dtype=np.int32)
+    sb.w_x[0] = float(np.max(sb.x[:, 2] + sb.radius))
+    cell = L[0] / sb.num[0]
+    sb.w_m[0] = wall_mass_scale * sb.rho * n * math.pi * (cell / 2.0) ** 3
+    sb.w_devs[0] = float(stress)
+    sb.time_dt = _dt(sb)
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    return sb
+
+
+def fcc_bonded(cells=(4, 4, 8), radius=1e-2, jitter_frac=0.01, bond_reach=1.05, k_bond=1e7,
+               lambda_bar=0.5, sigma_b=math.inf, tau_b=math.inf, pull_velocity=0.01,
+               gravity=0.0, steps=5000, seed=BASE_SEED + 2, sid="c2-bonded") -> Spherebin:
+    """Jittered FCC solid with parallel bonds between near neighbours.
+
+    Bonds join every pair with centre distance <= 1.05 (r_i + r_j).  The bond
+    stiffness enters through the contact k_n = k_t (the reference scales it by
+    the bond area, src/cohesion.cuh:124-177).  The top and bottom layers are
+    pulled apart with fixed velocities (vel.w = fixvel > 0 keeps vel_x, vel_y
+    and removes gravity; the vertical velocity is integrated).  No periodic
+    axes, no walls beyond the reference defaults.
+    """
+    rng = np.random.default_rng(seed)
+    a = 2.0 * math.sqrt(2.0) * radius
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cx, cy, cz = cells
+    ii, jj, kk = np.meshgrid(np.arange(cx), np.arange(cy), np.arange(cz), indexing="ij")
+    corners = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float64)
+    pts = (corners[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a
+    pts = pts[np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))]
+    n = pts.shape[0]
+    margin = 2.0 * radius
+    pts = pts + margin + rng.uniform(-jitter_frac * radius, jitter_frac * radius, size=pts.shape)
+    sb = Spherebin(n, nw=0, sid=sid)
+    sb.set_walls(0)
+    sb.x = pts
+    sb.radius = np.full(n, radius)
+    _contact_params(sb, k_n=k_bond, k_t=k_bond, gamma_n=0.0, gamma_t=0.0)
+    ext = np.array([cx * a, cy * a, cz * a]) + 2 * margin
+    L = ext + np.array([0.0, 0.0, 0.2 * ext[2]])
+    _grid(sb, L, 2.0 * radius * 1.0001)
+    sb.periodic = 0
+    sb.g = np.array([0.0, 0.0, gravity])
+    # bonds: neighbour pairs within reach (cell grid search)
+    from scipy.spatial import cKDTree
+
+    tree = cKDTree(sb.x)
+    pairs = np.array(sorted(tree.query_pairs(r=bond_reach * 2.0 * radius)), dtype=np.uint32).reshape(-1, 2)
+    sb.set_bonds(pairs)
+    sb.lambda_bar = float(lambda_bar)
+    sb.sigma_b = float(sigma_b)
+    sb.tau_b = float(tau_b)
+    z = sb.x[:, 2]
+    zlo, zhi = z.min(), z.max()
+    bottom = z < zlo + 0.5 * a
+    top = z > zhi - 0.5 * a
+    sb.fixvel[bottom | top] = 1.0
+    sb.vel[top, 2] = pull_velocity / 2.0
+    sb.vel[bottom, 2] = -pull_velocity / 2.0
+    sb.time_dt = _dt(sb, 0.17) * 0.5
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    return sb
+
+
+def shear_cell(n=65536, box=None, stress=10e3, shear_velocity=0.1, gravity=-9.81, steps=2000,
+               seed=BASE_SEED + 3, sid="c3-shear", packing_fraction=0.3) -> Spherebin:
+    """Polydisperse packing (config-1 radii), x/y periodic, top wall at 10 kPa,
+    the top layer moved at a fixed horizontal velocity and the bottom layer
+    held (python/sphere.py:1297-1356)."""
+    from .dem import pack_rsa
+
+    rng = np.random.default_rng(seed)
+    r = lognormal_radii(rng, n)
+    if box is None:
+        vol = float(np.sum(4.0 / 3.0 * math.pi * r ** 3)) / packing_fraction
+        side = (vol / 2.0) ** (1.0 / 3.0)
+        box = (side, side, 2.0 * side)
+    sb = random_compression(n=n, box=box, periodic=1, stress=stress, gravity=gravity, steps=steps,
+                            seed=seed, sid=sid)
+    sb.radius = r
+    sb.x = pack_rsa(r, np.asarray(box, float), 1, seed)
+    sb.w_x[0] = float(np.max(sb.x[:, 2] + sb.radius))
+    z = sb.x[:, 2]
+    zmin, zmax = float(np.min(z - r)), float(np.max(z + r))
+    d_bot = 2.0 * float(np.max(r[z < zmin + 0.3 * (zmax - zmin)]))
+    d_top = 2.0 * float(np.max(r[z > zmin + 0.7 * (zmax - zmin)]))
+    bot = z < zmin + d_bot
+    top = z > zmax - d_top
+    sb.fixvel[bot | top] = 1.0
+    sb.vel[bot, :2] = 0.0
+    sb.vel[top, 0] = shear_velocity
+    sb.vel[top, 1] = 0.0
+    sb.gamma_wn = sb.gamma_wt = 0.0
+    sb.mu_ws = sb.mu_wd = 0.0
+    sb.time_dt = _dt(sb)
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    return sb
+
+
+def two_spheres(distance=0.0, radius=0.5, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0), bonded=False,
+                gamma=0.0, k=1.16e9, mu=0.4, L=20.0, periodic=1, gravity=0.0, sid="pair") -> Spherebin:
+    """Two spheres on the x axis, the layout of tests/bond_tests.py:42-50."""
+    sb = Spherebin(2, nw=0, sid=sid)
+    sb.set_walls(0)
+    sb.x[0] = (10.0, 10.0, 10.0)
+    sb.x[1] = (10.0 + 2.0 * radius + distance, 10.0, 10.0)
+    sb.radius[:] = radius
+    sb.vel[1] = v
+    sb.angvel[1] = w
+    _contact_params(sb, k_n=k, k_t=k, gamma_n=gamma, gamma_t=gamma, mu=mu)
+    sb.gamma_wn = sb.gamma_wt = 0.0
+    _grid(sb, (L, L, L), 2.0 * radius * 1.0001)
+    sb.periodic = int(periodic)
+    sb.g = np.array([0.0, 0.0, gravity])
+    if bonded:
+        sb.bond(0, 1)
+        sb.lambda_bar = 1.0
+    sb.time_dt = _dt(sb, 0.075)
+    sb.time_total = 0.2
+    sb.time_file_dt = 0.01
+    return sb

@@ -1,0 +1,100 @@
+// dem_internal.h -- device layout and kernel entry points of the gfx950 DEM step.
+//
+// All per-particle state lives in HBM in the original particle order (the
+// order of the .bin file), as 32-byte double4 records so that the scattered
+// per-particle writes of the fused contact/integrate kernel fill whole
+// 32-byte sectors.  Sorted copies of position, velocity and spin (xs, vs, ws)
+// are rebuilt every step by the counting sort; neighbour reads go to them, so
+// a contact's partner is read from the spatially sorted, cache-resident copy.
+//
+// Contact history per particle: a 32-bit occupancy mask plus SPH_NC slots.
+// A slot word packs the partner's original index (low 28 bits) and the
+// periodic image code (high 4 bits: 2 bits per periodic axis, 0 = none,
+// 1 = +L, 2 = -L); the tangential spring delta_t of a slot is 3 doubles
+// (FP64) or 3 floats (mixed).  Only occupied slots are read or written.
+#ifndef SPHERE_DEM_INTERNAL_H_
+#define SPHERE_DEM_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sphere_dem.h"
+
+namespace sph {
+
+constexpr int NC = SPH_NC;
+constexpr uint32_t EMPTY_CELL = 0xffffffffu;
+constexpr uint32_t SLOT_IDX_MASK = 0x0fffffffu;
+constexpr uint32_t MAX_PARTICLES = SLOT_IDX_MASK;  // indices must fit 28 bits
+constexpr int BLOCK = 256;
+constexpr int SCAN_PER_THREAD = 8;
+constexpr int SCAN_TILE = BLOCK * SCAN_PER_THREAD;
+
+// device error flags (atomicOr'd by kernels, read after each sph_step chunk)
+enum : uint32_t {
+    FLAG_OUT_OF_GRID = 1u,
+    FLAG_CONTACT_OVERFLOW = 2u,
+    FLAG_NAN = 4u,
+};
+
+// Everything a step kernel needs, passed by value as one kernel argument.
+struct DemDev {
+    uint32_t np, nb0, nw, K;
+    uint32_t num[3];
+    int periodic;
+    uint32_t bond_broken;  // sentinel written into bonds[k].x on breakage
+    uint32_t nwall_blocks; // ceil(np/256), partials per wall
+    double origo[3], L[3], cell_len[3];
+    double dt;
+    double g[3];
+    double k_n, k_t, gamma_n, gamma_t, mu_s, mu_d;
+    double gamma_wn, gamma_wt, mu_ws, mu_wd, rho, kappa;
+    double lambda_bar, sigma_b, tau_b, devs_A, devs_f;
+
+    // particle state, original order
+    double4 *x, *vel, *angvel, *vel0, *angvel0, *angpos, *force, *torque;
+    double2* xysum;
+    double *es_dot, *es, *ev_dot, *ev, *p;
+
+    // contact history
+    uint32_t* cmask;    // np
+    uint32_t* cslot;    // np * NC
+    double* cdt;        // np * NC * 3 (FP64 mode)
+    float* cdt32;       // np * NC * 3 (mixed mode)
+
+    // cell list
+    uint32_t *key, *rank, *idx, *inv, *count, *cellStart, *cellEnd, *scan_bsum;
+    uint32_t scan_blocks;
+    double4 *xs, *vs, *ws;
+
+    // walls (6 entries; entries >= nw hold the reference's defaults)
+    double4 *wnx, *wmvfd;
+    double* wvel0;
+    int* wmode;
+    double* wall_pp;       // nw * np
+    double* wall_partial;  // nw * nwall_blocks
+    double* t_cur;         // device copy of time.current
+
+    // bonds
+    uint2* bonds;
+    double4 *bdelta, *bomega;
+    double4 *bond_f, *bond_t;
+    uint32_t *bcsr_off, *bcsr_ent;  // per-particle bond lists, entry = k<<1 | side
+
+    uint32_t* flags;
+};
+
+// kernel launchers (kernels.hip); all enqueue on `s`
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+void launch_bonds(const DemDev& d, hipStream_t s);
+void launch_cellid(const DemDev& d, hipStream_t s);
+void launch_scan(const DemDev& d, hipStream_t s);
+void launch_scatter(const DemDev& d, hipStream_t s);
+void launch_cellsort(const DemDev& d, hipStream_t s);
+void launch_reorder(const DemDev& d, hipStream_t s);
+void launch_contact(const DemDev& d, int precision, hipStream_t s);
+void launch_walls(const DemDev& d, hipStream_t s);
+
+}  // namespace sph
+
+#endif

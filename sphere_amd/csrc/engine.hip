@@ -1,0 +1,801 @@
+// engine.hip -- host engine behind the C-ABI of include/sphere_dem.h.
+//
+// Owns the device buffers of one simulation, enqueues the step kernels on
+// its own stream (captured once into a HIP graph and replayed), converts
+// between the device layout and the reference's array layout, and
+// reproduces DEM::startTime's output cadence (src/device.cu:576-1299).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "binio.h"
+#include "dem_internal.h"
+
+using sph::DemDev;
+using sph::HostState;
+
+struct sph_dem {
+    HostState hs;
+    DemDev d{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool use_graph = true;
+    int precision = SPH_PREC_FP64;
+    int profiling = 0;
+    int verbosity = 0;
+    sph_timing_t timing{};
+    hipEvent_t ev[8] = {};
+    bool events_made = false;
+    std::string err;
+    uint64_t steps_done = 0;
+    int device = 0;
+};
+
+namespace {
+
+thread_local std::string g_open_error;
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+            return SPH_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+template <class T> int dalloc(sph_dem* h, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        h->err = std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " bytes): " + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    h->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return SPH_OK;
+}
+
+#define ALLOC(ptr, count)                              \
+    do {                                               \
+        int rc_ = dalloc(h, &(ptr), (count));          \
+        if (rc_) return rc_;                           \
+    } while (0)
+
+std::vector<double> pad4(const std::vector<double>& a3, size_t n, const double* w = nullptr) {
+    std::vector<double> out(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+        out[4 * i + 0] = a3[3 * i + 0];
+        out[4 * i + 1] = a3[3 * i + 1];
+        out[4 * i + 2] = a3[3 * i + 2];
+        out[4 * i + 3] = w ? w[i] : 0.0;
+    }
+    return out;
+}
+
+void unpad4(const std::vector<double>& a4, size_t n, double* out3) {
+    for (size_t i = 0; i < n; ++i) {
+        out3[3 * i + 0] = a4[4 * i + 0];
+        out3[3 * i + 1] = a4[4 * i + 1];
+        out3[3 * i + 2] = a4[4 * i + 2];
+    }
+}
+
+void destroy_graph(sph_dem* h) {
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+}
+
+void fill_params(sph_dem* h) {
+    const HostState& s = h->hs;
+    DemDev& d = h->d;
+    d.np = s.np; d.nb0 = s.nb0; d.nw = s.nw;
+    for (int a = 0; a < 3; ++a) {
+        d.num[a] = s.num[a]; d.origo[a] = s.origo[a]; d.L[a] = s.L[a];
+        d.cell_len[a] = s.L[a] / (double)s.num[a];
+        d.g[a] = s.g[a];
+    }
+    d.K = s.num[0] * s.num[1] * s.num[2];
+    d.periodic = s.periodic;
+    d.bond_broken = s.nb0 >= s.np ? s.nb0 : s.np;
+    d.nwall_blocks = (s.np + 255) / 256;
+    d.dt = s.dt;
+    d.k_n = s.k_n; d.k_t = s.k_t; d.gamma_n = s.gamma_n; d.gamma_t = s.gamma_t;
+    d.mu_s = s.mu_s; d.mu_d = s.mu_d; d.gamma_wn = s.gamma_wn; d.gamma_wt = s.gamma_wt;
+    d.mu_ws = s.mu_ws; d.mu_wd = s.mu_wd; d.rho = s.rho; d.kappa = s.kappa;
+    d.lambda_bar = s.lambda_bar; d.sigma_b = s.sigma_b; d.tau_b = s.tau_b;
+    d.devs_A = s.devs_A; d.devs_f = s.devs_f;
+    d.scan_blocks = (d.K + sph::SCAN_TILE - 1) / sph::SCAN_TILE;
+}
+
+int validate_for_device(sph_dem* h) {
+    const HostState& s = h->hs;
+    if (s.contactmodel != 2) {
+        h->err = "contactmodel " + std::to_string(s.contactmodel) +
+                 " is not on this path (only the linear visco-elastic model 2)";
+        return SPH_E_UNSUPPORTED;
+    }
+    if (s.np == 0 || s.np > sph::MAX_PARTICLES) {
+        h->err = "np=" + std::to_string(s.np) + " outside [1, 2^28-1]";
+        return SPH_E_INVALID;
+    }
+    if (s.num[0] == 0 || s.num[1] == 0 || s.num[2] == 0 || !(s.L[0] > 0) || !(s.L[1] > 0) || !(s.L[2] > 0)) {
+        h->err = "grid has a zero-sized axis";
+        return SPH_E_INVALID;
+    }
+    const double K = (double)s.num[0] * s.num[1] * s.num[2];
+    if (K >= 4294967295.0) {
+        h->err = "grid has too many cells for 32-bit cell ids";
+        return SPH_E_INVALID;
+    }
+    if ((s.periodic == 1 || s.periodic == 2) && s.num[0] < 3) {
+        h->err = "periodic x needs at least 3 cells";
+        return SPH_E_INVALID;
+    }
+    if (s.periodic == 1 && s.num[1] < 3) {
+        h->err = "periodic y needs at least 3 cells";
+        return SPH_E_INVALID;
+    }
+    for (uint32_t k = 0; k < s.nb0; ++k) {
+        if (s.bonds[2 * k] < s.np && s.bonds[2 * k + 1] >= s.np) {
+            h->err = "bond " + std::to_string(k) + " names particle " + std::to_string(s.bonds[2 * k + 1]) +
+                     " >= np";
+            return SPH_E_INVALID;
+        }
+    }
+    return SPH_OK;
+}
+
+int upload_all(sph_dem* h) {
+    HostState& s = h->hs;
+    DemDev& d = h->d;
+    const size_t n = s.np;
+    std::vector<double> fix(n);
+    for (size_t i = 0; i < n; ++i) fix[i] = s.vel[4 * i + 3];
+    std::vector<double> v3(3 * n);
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) v3[3 * i + a] = s.vel[4 * i + a];
+    const std::vector<double> angvel4 = pad4(s.angvel, n), angpos4 = pad4(s.angpos, n);
+    const std::vector<double> force4 = pad4(s.force, n), torque4 = pad4(s.torque, n);
+    const std::vector<double> vel0_4 = pad4(v3, n, fix.data());
+    HIP_TRY(hipMemcpy(d.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.vel0, vel0_4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.angvel, angvel4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.angvel0, angvel4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.angpos, angpos4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.force, force4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.torque, torque4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.xysum, s.xysum.data(), n * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.es_dot, s.es_dot.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.es, s.es.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.ev_dot, s.ev_dot.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.ev, s.ev.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.p, s.p.data(), n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d.cmask, 0, n * 4));
+    HIP_TRY(hipMemset(d.cslot, 0, n * sph::NC * 4));
+    HIP_TRY(hipMemset(d.cdt, 0, n * sph::NC * 3 * 8));
+    HIP_TRY(hipMemset(d.flags, 0, 4));
+    HIP_TRY(hipMemset(d.key, 0, n * 4));
+    HIP_TRY(hipMemset(d.idx, 0, n * 4));
+    HIP_TRY(hipMemset(d.cellStart, 0xff, (size_t)d.K * 4));
+    HIP_TRY(hipMemset(d.cellEnd, 0xff, (size_t)d.K * 4));
+    HIP_TRY(hipMemset(d.xs, 0, n * 32));
+    // walls: entries past nw keep the defaults of src/contactsearch.cuh:416-427
+    double wnx[24] = {0.0, 0.0, -1.0, s.L[2], -1.0, 0.0, 0.0, s.L[0], 1.0, 0.0, 0.0, 0.0,
+                      0.0, -1.0, 0.0, s.L[1], 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double wmv[24] = {0};
+    double wv0[6] = {0};
+    int wmode[6] = {0};
+    for (uint32_t w = 0; w < s.nw; ++w) {
+        for (int a = 0; a < 4; ++a) { wnx[4 * w + a] = s.wnx[4 * w + a]; wmv[4 * w + a] = s.wmvfd[4 * w + a]; }
+        wv0[w] = s.wmvfd[4 * w + 1];
+        wmode[w] = s.wmode[w];
+    }
+    HIP_TRY(hipMemcpy(d.wnx, wnx, sizeof(wnx), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wmvfd, wmv, sizeof(wmv), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wvel0, wv0, sizeof(wv0), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wmode, wmode, sizeof(wmode), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.t_cur, &s.t_cur, 8, hipMemcpyHostToDevice));
+    if (s.nw) HIP_TRY(hipMemset(d.wall_pp, 0, (size_t)s.nw * n * 8));
+    // bonds
+    const size_t nb = s.nb0;
+    if (nb) {
+        std::vector<double> bd(4 * nb), bo(4 * nb);
+        for (size_t k = 0; k < nb; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                bd[4 * k + a] = s.bonds_delta_t[3 * k + a];
+                bo[4 * k + a] = s.bonds_omega_t[3 * k + a];
+            }
+            bd[4 * k + 3] = s.bonds_delta_n[k];
+            bo[4 * k + 3] = s.bonds_omega_n[k];
+        }
+        HIP_TRY(hipMemcpy(d.bonds, s.bonds.data(), nb * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.bdelta, bd.data(), nb * 32, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.bomega, bo.data(), nb * 32, hipMemcpyHostToDevice));
+        // per-particle bond lists in ascending bond index, first-index side first
+        std::vector<uint32_t> off(n + 1, 0), ent;
+        for (size_t k = 0; k < nb; ++k) {
+            const uint32_t bi = s.bonds[2 * k], bj = s.bonds[2 * k + 1];
+            if (!(bi < s.np)) continue;
+            ++off[bi + 1];
+            ++off[bj + 1];
+        }
+        for (size_t i = 0; i < n; ++i) off[i + 1] += off[i];
+        ent.assign(off[n] ? off[n] : 1, 0);
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (size_t k = 0; k < nb; ++k) {
+            const uint32_t bi = s.bonds[2 * k], bj = s.bonds[2 * k + 1];
+            if (!(bi < s.np)) continue;
+            ent[cur[bi]++] = (uint32_t)(k << 1);
+            ent[cur[bj]++] = (uint32_t)(k << 1) | 1u;
+        }
+        if (dalloc(h, &d.bcsr_ent, ent.size())) return SPH_E_HIP;
+        HIP_TRY(hipMemcpy(d.bcsr_off, off.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.bcsr_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SPH_OK;
+}
+
+int allocate(sph_dem* h) {
+    DemDev& d = h->d;
+    const size_t n = h->hs.np;
+    ALLOC(d.x, n); ALLOC(d.vel, n); ALLOC(d.angvel, n); ALLOC(d.vel0, n); ALLOC(d.angvel0, n);
+    ALLOC(d.angpos, n); ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.xysum, n);
+    ALLOC(d.es_dot, n); ALLOC(d.es, n); ALLOC(d.ev_dot, n); ALLOC(d.ev, n); ALLOC(d.p, n);
+    ALLOC(d.cmask, n); ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
+    d.cdt32 = nullptr;
+    ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.idx, n); ALLOC(d.inv, n);
+    ALLOC(d.count, d.K); ALLOC(d.cellStart, d.K); ALLOC(d.cellEnd, d.K); ALLOC(d.scan_bsum, d.scan_blocks);
+    ALLOC(d.xs, n); ALLOC(d.vs, n); ALLOC(d.ws, n);
+    ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
+    ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
+    ALLOC(d.t_cur, 1);
+    const size_t nb = h->hs.nb0;
+    ALLOC(d.bonds, nb); ALLOC(d.bdelta, nb); ALLOC(d.bomega, nb); ALLOC(d.bond_f, nb); ALLOC(d.bond_t, nb);
+    ALLOC(d.bcsr_off, n + 1);
+    d.bcsr_ent = nullptr;
+    ALLOC(d.flags, 1);
+    return SPH_OK;
+}
+
+void free_all(sph_dem* h) {
+    destroy_graph(h);
+    if (h->events_made)
+        for (auto& e : h->ev) (void)hipEventDestroy(e);
+    for (void* p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    h->stream = nullptr;
+}
+
+void enqueue_step(sph_dem* h, bool timed) {
+    const DemDev& d = h->d;
+    hipStream_t s = h->stream;
+    if (timed) (void)hipEventRecord(h->ev[0], s);
+    sph::launch_bonds(d, s);
+    if (timed) (void)hipEventRecord(h->ev[1], s);
+    sph::launch_fill_u32(d.count, d.K, 0u, s);
+    sph::launch_cellid(d, s);
+    if (timed) (void)hipEventRecord(h->ev[2], s);
+    sph::launch_scan(d, s);
+    sph::launch_scatter(d, s);
+    sph::launch_cellsort(d, s);
+    if (timed) (void)hipEventRecord(h->ev[3], s);
+    sph::launch_reorder(d, s);
+    if (timed) (void)hipEventRecord(h->ev[4], s);
+    sph::launch_contact(d, h->precision, s);
+    if (timed) (void)hipEventRecord(h->ev[5], s);
+    sph::launch_walls(d, s);
+    if (timed) (void)hipEventRecord(h->ev[6], s);
+}
+
+int ensure_graph(sph_dem* h) {
+    if (h->gexec) return SPH_OK;
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(h, false);
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture(h->stream, &g));
+    h->graph = g;
+    HIP_TRY(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    return SPH_OK;
+}
+
+int check_flags(sph_dem* h) {
+    uint32_t f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, h->d.flags, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (!f) return SPH_OK;
+    HIP_TRY(hipMemsetAsync(h->d.flags, 0, 4, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (f & sph::FLAG_NAN) { h->err = "non-finite particle position"; return SPH_E_NAN; }
+    if (f & sph::FLAG_OUT_OF_GRID) {
+        h->err = "one or more particles moved more than one cell outside the grid; is the time step too large?";
+        return SPH_E_OUT_OF_GRID;
+    }
+    h->err = "a particle has more than " + std::to_string(sph::NC) + " contacts";
+    return SPH_E_CONTACT_OVERFLOW;
+}
+
+int ensure_mixed_buffers(sph_dem* h) {
+    if (h->d.cdt32) return SPH_OK;
+    ALLOC(h->d.cdt32, (size_t)h->hs.np * sph::NC * 3);
+    HIP_TRY(hipMemset(h->d.cdt32, 0, (size_t)h->hs.np * sph::NC * 3 * 4));
+    return SPH_OK;
+}
+
+// Downloads every particle array into the host mirror (for writebin).
+int download_all(sph_dem* h) {
+    HostState& s = h->hs;
+    DemDev& d = h->d;
+    const size_t n = s.np;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<double> t4(4 * n);
+    HIP_TRY(hipMemcpy(s.x.data(), d.x, n * 32, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s.vel.data(), d.vel, n * 32, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s.xysum.data(), d.xysum, n * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(t4.data(), d.force, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.force.data());
+    HIP_TRY(hipMemcpy(t4.data(), d.angpos, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.angpos.data());
+    HIP_TRY(hipMemcpy(t4.data(), d.angvel, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.angvel.data());
+    HIP_TRY(hipMemcpy(t4.data(), d.torque, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.torque.data());
+    HIP_TRY(hipMemcpy(s.es_dot.data(), d.es_dot, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s.es.data(), d.es, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s.ev_dot.data(), d.ev_dot, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s.ev.data(), d.ev, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(s.p.data(), d.p, n * 8, hipMemcpyDeviceToHost));
+    if (s.nw) {
+        double wnx[24], wmv[24];
+        HIP_TRY(hipMemcpy(wnx, d.wnx, sizeof(wnx), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(wmv, d.wmvfd, sizeof(wmv), hipMemcpyDeviceToHost));
+        for (uint32_t w = 0; w < s.nw; ++w)
+            for (int a = 0; a < 4; ++a) { s.wnx[4 * w + a] = wnx[4 * w + a]; s.wmvfd[4 * w + a] = wmv[4 * w + a]; }
+    }
+    const size_t nb = s.nb0;
+    if (nb) {
+        std::vector<double> bd(4 * nb), bo(4 * nb);
+        HIP_TRY(hipMemcpy(s.bonds.data(), d.bonds, nb * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(bd.data(), d.bdelta, nb * 32, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(bo.data(), d.bomega, nb * 32, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nb; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                s.bonds_delta_t[3 * k + a] = bd[4 * k + a];
+                s.bonds_omega_t[3 * k + a] = bo[4 * k + a];
+            }
+            s.bonds_delta_n[k] = bd[4 * k + 3];
+            s.bonds_omega_n[k] = bo[4 * k + 3];
+        }
+    }
+    return SPH_OK;
+}
+
+int open_common(sph_dem* h, int verbosity, int check_vals) {
+    h->verbosity = verbosity;
+    int rc;
+    if (check_vals == 1 && (rc = sph::check_values(h->hs, h->err))) return rc;
+    if ((rc = validate_for_device(h))) return rc;
+    fill_params(h);
+    HIP_TRY(hipGetDevice(&h->device));
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if ((rc = allocate(h))) return rc;
+    if ((rc = upload_all(h))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    if (verbosity > 0) {
+        std::printf("sphere_dem: np=%u nb0=%u nw=%u grid=%ux%ux%u periodic=%d dt=%g\n", h->hs.np, h->hs.nb0,
+                    h->hs.nw, h->hs.num[0], h->hs.num[1], h->hs.num[2], h->hs.periodic, h->hs.dt);
+    }
+    return SPH_OK;
+}
+
+int finish_open(sph_dem* h, int rc, sph_t** out) {
+    if (rc) {
+        g_open_error = h->err;
+        free_all(h);
+        delete h;
+        *out = nullptr;
+        return rc;
+    }
+    *out = h;
+    return SPH_OK;
+}
+
+int make_events(sph_dem* h) {
+    if (h->events_made) return SPH_OK;
+    for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
+    h->events_made = true;
+    return SPH_OK;
+}
+
+int run_steps(sph_dem* h, uint64_t n) {
+    if (n == 0) return SPH_OK;
+    int rc;
+    const uint64_t chunk = 512;
+    if (h->profiling) {
+        if ((rc = make_events(h))) return rc;
+        sph_timing_t t{};
+        for (uint64_t i = 0; i < n; ++i) {
+            enqueue_step(h, true);
+            HIP_TRY(hipEventSynchronize(h->ev[6]));
+            float ms[6];
+            for (int k = 0; k < 6; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], h->ev[k], h->ev[k + 1]));
+            t.bonds_ms += ms[0]; t.cellid_ms += ms[1]; t.sort_ms += ms[2]; t.reorder_ms += ms[3];
+            t.contact_ms += ms[4]; t.walls_ms += ms[5];
+            h->hs.t_cur += h->hs.dt;
+            ++h->steps_done;
+        }
+        t.total_ms = t.bonds_ms + t.cellid_ms + t.sort_ms + t.reorder_ms + t.contact_ms + t.walls_ms;
+        t.steps = n;
+        h->timing = t;
+        return check_flags(h);
+    }
+    if (h->use_graph && (rc = ensure_graph(h))) return rc;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (h->use_graph) HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+        else enqueue_step(h, false);
+        h->hs.t_cur += h->hs.dt;
+        ++h->steps_done;
+        if ((i + 1) % chunk == 0 && i + 1 < n) {
+            if ((rc = check_flags(h))) return rc;
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return check_flags(h);
+}
+
+bool field_is_known(int f) { return f >= 0 && f < SPH_NFIELDS; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+extern "C" {
+
+int sph_open_mem(const void* bin, size_t bytes, int verbosity, int check_vals, sph_t** out) {
+    if (!out) return SPH_E_INVALID;
+    sph_dem* h = new sph_dem();
+    int rc = sph::parse_bin(bin, bytes, h->hs, h->err);
+    if (!rc) rc = open_common(h, verbosity, check_vals);
+    return finish_open(h, rc, out);
+}
+
+int sph_open(const char* path, int verbosity, int check_vals, sph_t** out) {
+    if (!out || !path) return SPH_E_INVALID;
+    sph_dem* h = new sph_dem();
+    int rc = sph::read_bin_file(path, h->hs, h->err);
+    if (!rc) rc = open_common(h, verbosity, check_vals);
+    return finish_open(h, rc, out);
+}
+
+void sph_close(sph_t* h) {
+    if (!h) return;
+    free_all(h);
+    delete h;
+}
+
+const char* sph_last_error(const sph_t* h) { return h ? h->err.c_str() : g_open_error.c_str(); }
+
+int sph_step(sph_t* h, uint64_t n) {
+    if (!h) return SPH_E_INVALID;
+    return run_steps(h, n);
+}
+
+int sph_writebin(sph_t* h, const char* path) {
+    if (!h || !path) return SPH_E_INVALID;
+    int rc = download_all(h);
+    if (rc) return rc;
+    return sph::write_bin_file(path, h->hs, h->err);
+}
+
+int sph_check_values(sph_t* h) {
+    if (!h) return SPH_E_INVALID;
+    int rc = download_all(h);
+    if (rc) return rc;
+    return sph::check_values(h->hs, h->err);
+}
+
+int sph_run(sph_t* h, const char* outdir, const char* sid) {
+    if (!h || !outdir || !sid) return SPH_E_INVALID;
+    HostState& s = h->hs;
+    const std::string base = std::string(outdir) + "/" + sid;
+    auto write_status = [&]() -> int {
+        const std::string p = base + ".status.dat";
+        FILE* f = std::fopen(p.c_str(), "w");
+        if (!f) { h->err = "cannot write " + p; return SPH_E_IO; }
+        std::fprintf(f, "%2.4e %2.4e %d\n", s.t_cur, 100.0 * s.t_cur / s.t_total, (int)s.step_count);
+        std::fclose(f);
+        return SPH_OK;
+    };
+    int rc = sph_writebin(h, (base + ".output00000.bin").c_str());
+    if (rc) return rc;
+    if ((rc = write_status())) return rc;
+    // src/device.cu:594,1109-1138: float output clock, double simulation time
+    float filetimeclock = 0.0f;
+    while (s.t_cur <= s.t_total) {
+        // count the steps until the next output or the end of the run
+        uint64_t k = 0;
+        double t = s.t_cur;
+        float ftc = filetimeclock;
+        bool out = false;
+        do {
+            t += s.dt;
+            ftc += s.dt;
+            ++k;
+            if (ftc >= s.file_dt) { out = true; break; }
+        } while (t <= s.t_total && k < 100000);
+        if ((rc = run_steps(h, k))) return rc;
+        filetimeclock = ftc;
+        if (out) {
+            s.step_count += 1;
+            char name[64];
+            std::snprintf(name, sizeof(name), ".output%05u.bin", s.step_count);
+            if ((rc = sph_writebin(h, (base + name).c_str()))) return rc;
+            if ((rc = write_status())) return rc;
+            filetimeclock = 0.0f;
+        }
+    }
+    return SPH_OK;
+}
+
+size_t sph_field_bytes(const sph_t* h, int f) {
+    if (!h || !field_is_known(f)) return 0;
+    const size_t n = h->hs.np, nb = h->hs.nb0, nw = h->hs.nw, nc = sph::NC;
+    switch (f) {
+        case SPH_X: case SPH_VEL: case SPH_X_SORTED: return n * 32;
+        case SPH_ANGVEL: case SPH_FORCE: case SPH_TORQUE: case SPH_ANGPOS: case SPH_VEL0:
+        case SPH_ANGVEL0: return n * 24;
+        case SPH_XYSUM: return n * 16;
+        case SPH_ES_DOT: case SPH_ES: case SPH_EV_DOT: case SPH_EV: case SPH_P: return n * 8;
+        case SPH_CONTACTS: return n * nc * 4;
+        case SPH_DISTMOD: return n * nc * 32;
+        case SPH_DELTA_T: return n * nc * 24;
+        case SPH_CELL_KEYS: case SPH_SORT_IDX: return n * 4;
+        case SPH_CELL_START: case SPH_CELL_END: return (size_t)h->d.K * 4;
+        case SPH_BONDS: return nb * 8;
+        case SPH_BONDS_DELTA: case SPH_BONDS_OMEGA: return nb * 32;
+        case SPH_WALLS_NX: case SPH_WALLS_MVFD: return nw * 32;
+        case SPH_WALLS_VEL0: return nw * 8;
+        case SPH_WALL_PP: return nw * n * 8;
+        case SPH_TIME: return 32;
+        default: return 0;
+    }
+}
+
+int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
+    if (!h || !field_is_known(f)) return SPH_E_INVALID;
+    const size_t need = sph_field_bytes(h, f);
+    if (bytes != need) {
+        h->err = "sph_get: field " + std::to_string(f) + " needs " + std::to_string(need) + " bytes";
+        return SPH_E_INVALID;
+    }
+    if (need == 0) return SPH_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const DemDev& d = h->d;
+    const size_t n = h->hs.np, nc = sph::NC;
+    auto copy = [&](const void* src) -> int {
+        HIP_TRY(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+        return SPH_OK;
+    };
+    auto copy_unpad = [&](const double4* src) -> int {
+        std::vector<double> t4(4 * n);
+        HIP_TRY(hipMemcpy(t4.data(), src, n * 32, hipMemcpyDeviceToHost));
+        unpad4(t4, n, static_cast<double*>(dst));
+        return SPH_OK;
+    };
+    switch (f) {
+        case SPH_X: return copy(d.x);
+        case SPH_VEL: return copy(d.vel);
+        case SPH_X_SORTED: return copy(d.xs);
+        case SPH_ANGVEL: return copy_unpad(d.angvel);
+        case SPH_FORCE: return copy_unpad(d.force);
+        case SPH_TORQUE: return copy_unpad(d.torque);
+        case SPH_ANGPOS: return copy_unpad(d.angpos);
+        case SPH_VEL0: return copy_unpad(d.vel0);
+        case SPH_ANGVEL0: return copy_unpad(d.angvel0);
+        case SPH_XYSUM: return copy(d.xysum);
+        case SPH_ES_DOT: return copy(d.es_dot);
+        case SPH_ES: return copy(d.es);
+        case SPH_EV_DOT: return copy(d.ev_dot);
+        case SPH_EV: return copy(d.ev);
+        case SPH_P: return copy(d.p);
+        case SPH_CELL_KEYS: return copy(d.key);
+        case SPH_SORT_IDX: return copy(d.idx);
+        case SPH_CELL_START: return copy(d.cellStart);
+        case SPH_CELL_END: return copy(d.cellEnd);
+        case SPH_BONDS: return copy(d.bonds);
+        case SPH_BONDS_DELTA: return copy(d.bdelta);
+        case SPH_BONDS_OMEGA: return copy(d.bomega);
+        case SPH_WALLS_NX: return copy(d.wnx);
+        case SPH_WALLS_MVFD: return copy(d.wmvfd);
+        case SPH_WALLS_VEL0: return copy(d.wvel0);
+        case SPH_WALL_PP: return copy(d.wall_pp);
+        case SPH_TIME: {
+            const double t[4] = {h->hs.dt, h->hs.t_cur, h->hs.t_total, h->hs.file_dt};
+            std::memcpy(dst, t, 32);
+            return SPH_OK;
+        }
+        case SPH_CONTACTS: case SPH_DISTMOD: case SPH_DELTA_T: {
+            std::vector<uint32_t> mask(n), slot(n * nc);
+            HIP_TRY(hipMemcpy(mask.data(), d.cmask, n * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(slot.data(), d.cslot, n * nc * 4, hipMemcpyDeviceToHost));
+            if (f == SPH_CONTACTS) {
+                uint32_t* out = static_cast<uint32_t*>(dst);
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t k = 0; k < nc; ++k)
+                        out[i * nc + k] = (mask[i] >> k) & 1u ? (slot[i * nc + k] & sph::SLOT_IDX_MASK) : (uint32_t)n;
+                return SPH_OK;
+            }
+            if (f == SPH_DISTMOD) {
+                std::vector<double> x(4 * n);
+                HIP_TRY(hipMemcpy(x.data(), d.x, n * 32, hipMemcpyDeviceToHost));
+                double* out = static_cast<double*>(dst);
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t k = 0; k < nc; ++k) {
+                        double* o4 = out + 4 * (i * nc + k);
+                        if (!((mask[i] >> k) & 1u)) { o4[0] = o4[1] = o4[2] = o4[3] = 0.0; continue; }
+                        const uint32_t w = slot[i * nc + k], code = w >> 28, b = w & sph::SLOT_IDX_MASK;
+                        const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
+                        o4[0] = cx == 1u ? h->hs.L[0] : (cx == 2u ? -h->hs.L[0] : 0.0);
+                        o4[1] = cy == 1u ? h->hs.L[1] : (cy == 2u ? -h->hs.L[1] : 0.0);
+                        o4[2] = 0.0;
+                        o4[3] = x[4 * (size_t)b + 3];
+                    }
+                return SPH_OK;
+            }
+            double* out = static_cast<double*>(dst);
+            if (h->precision == SPH_PREC_MIXED && d.cdt32) {
+                std::vector<float> t(n * nc * 3);
+                HIP_TRY(hipMemcpy(t.data(), d.cdt32, t.size() * 4, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < n * nc; ++i)
+                    for (int a = 0; a < 3; ++a)
+                        out[3 * i + a] = (mask[i / nc] >> (i % nc)) & 1u ? (double)t[3 * i + a] : 0.0;
+            } else {
+                HIP_TRY(hipMemcpy(out, d.cdt, n * nc * 24, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < n * nc; ++i)
+                    if (!((mask[i / nc] >> (i % nc)) & 1u)) out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0.0;
+            }
+            return SPH_OK;
+        }
+        default: break;
+    }
+    h->err = "sph_get: unknown field";
+    return SPH_E_INVALID;
+}
+
+int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
+    if (!h || !field_is_known(f)) return SPH_E_INVALID;
+    const size_t need = sph_field_bytes(h, f);
+    if (bytes != need) {
+        h->err = "sph_set: field " + std::to_string(f) + " needs " + std::to_string(need) + " bytes";
+        return SPH_E_INVALID;
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    DemDev& d = h->d;
+    const size_t n = h->hs.np;
+    auto copy = [&](void* dptr) -> int {
+        HIP_TRY(hipMemcpy(dptr, src, need, hipMemcpyHostToDevice));
+        return SPH_OK;
+    };
+    auto copy_pad = [&](double4* dptr, bool keep_w) -> int {
+        std::vector<double> t4(4 * n);
+        if (keep_w) HIP_TRY(hipMemcpy(t4.data(), dptr, n * 32, hipMemcpyDeviceToHost));
+        const double* s3 = static_cast<const double*>(src);
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) t4[4 * i + a] = s3[3 * i + a];
+        HIP_TRY(hipMemcpy(dptr, t4.data(), n * 32, hipMemcpyHostToDevice));
+        return SPH_OK;
+    };
+    switch (f) {
+        case SPH_X: return copy(d.x);
+        case SPH_VEL: return copy(d.vel);
+        case SPH_ANGVEL: return copy_pad(d.angvel, false);
+        case SPH_FORCE: return copy_pad(d.force, false);
+        case SPH_TORQUE: return copy_pad(d.torque, false);
+        case SPH_ANGPOS: return copy_pad(d.angpos, false);
+        case SPH_VEL0: return copy_pad(d.vel0, true);
+        case SPH_ANGVEL0: return copy_pad(d.angvel0, false);
+        case SPH_XYSUM: return copy(d.xysum);
+        case SPH_ES_DOT: return copy(d.es_dot);
+        case SPH_ES: return copy(d.es);
+        case SPH_EV_DOT: return copy(d.ev_dot);
+        case SPH_EV: return copy(d.ev);
+        case SPH_P: return copy(d.p);
+        case SPH_BONDS: return copy(d.bonds);
+        case SPH_BONDS_DELTA: return copy(d.bdelta);
+        case SPH_BONDS_OMEGA: return copy(d.bomega);
+        case SPH_WALLS_NX: return copy(d.wnx);
+        case SPH_WALLS_MVFD: return copy(d.wmvfd);
+        case SPH_WALLS_VEL0: return copy(d.wvel0);
+        case SPH_TIME: {
+            double t[4];
+            std::memcpy(t, src, 32);
+            h->hs.dt = t[0]; h->hs.t_cur = t[1]; h->hs.t_total = t[2]; h->hs.file_dt = t[3];
+            HIP_TRY(hipMemcpy(d.t_cur, &h->hs.t_cur, 8, hipMemcpyHostToDevice));
+            if (d.dt != h->hs.dt) { d.dt = h->hs.dt; destroy_graph(h); }
+            return SPH_OK;
+        }
+        default: break;
+    }
+    h->err = "sph_set: field " + std::to_string(f) + " is read-only";
+    return SPH_E_INVALID;
+}
+
+int sph_info(const sph_t* h, sph_info_t* out) {
+    if (!h || !out) return SPH_E_INVALID;
+    out->np = h->hs.np; out->nb0 = h->hs.nb0; out->nw = h->hs.nw; out->nc = sph::NC;
+    out->ncells = h->d.K;
+    for (int a = 0; a < 3; ++a) out->num[a] = h->hs.num[a];
+    out->periodic = h->hs.periodic;
+    out->dt = h->hs.dt; out->t_current = h->hs.t_cur; out->t_total = h->hs.t_total; out->file_dt = h->hs.file_dt;
+    out->steps_done = h->steps_done;
+    out->precision = h->precision;
+    return SPH_OK;
+}
+
+int sph_get_params(const sph_t* h, sph_params_t* out) {
+    if (!h || !out) return SPH_E_INVALID;
+    out->k_n = h->hs.k_n; out->k_t = h->hs.k_t; out->gamma_n = h->hs.gamma_n;
+    out->gamma_t = h->hs.gamma_t; out->mu_s = h->hs.mu_s; out->mu_d = h->hs.mu_d;
+    return SPH_OK;
+}
+
+int sph_set_params(sph_t* h, const sph_params_t* in) {
+    if (!h || !in) return SPH_E_INVALID;
+    h->hs.k_n = in->k_n; h->hs.k_t = in->k_t; h->hs.gamma_n = in->gamma_n;
+    h->hs.gamma_t = in->gamma_t; h->hs.mu_s = in->mu_s; h->hs.mu_d = in->mu_d;
+    fill_params(h);
+    destroy_graph(h);
+    return SPH_OK;
+}
+
+int sph_set_precision(sph_t* h, int precision) {
+    if (!h || (precision != SPH_PREC_FP64 && precision != SPH_PREC_MIXED)) return SPH_E_INVALID;
+    if (precision == h->precision) return SPH_OK;
+    int rc;
+    const size_t cnt = (size_t)h->hs.np * sph::NC * 3;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (precision == SPH_PREC_MIXED) {
+        if ((rc = ensure_mixed_buffers(h))) return rc;
+        std::vector<double> a(cnt);
+        std::vector<float> b(cnt);
+        HIP_TRY(hipMemcpy(a.data(), h->d.cdt, cnt * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cnt; ++i) b[i] = (float)a[i];
+        HIP_TRY(hipMemcpy(h->d.cdt32, b.data(), cnt * 4, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> a(cnt);
+        std::vector<float> b(cnt);
+        HIP_TRY(hipMemcpy(b.data(), h->d.cdt32, cnt * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cnt; ++i) a[i] = (double)b[i];
+        HIP_TRY(hipMemcpy(h->d.cdt, a.data(), cnt * 8, hipMemcpyHostToDevice));
+    }
+    h->precision = precision;
+    destroy_graph(h);
+    return SPH_OK;
+}
+
+int sph_set_profiling(sph_t* h, int on) {
+    if (!h) return SPH_E_INVALID;
+    h->profiling = on ? 1 : 0;
+    return SPH_OK;
+}
+
+int sph_get_timing(const sph_t* h, sph_timing_t* out) {
+    if (!h || !out) return SPH_E_INVALID;
+    *out = h->timing;
+    return SPH_OK;
+}
+
+int sph_set_graph(sph_t* h, int on) {
+    if (!h) return SPH_E_INVALID;
+    h->use_graph = on != 0;
+    if (!h->use_graph) destroy_graph(h);
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,678 @@
+// kernels.hip -- gfx950 kernels of one DEM time step.
+//
+// Step order (one HIP stream, optionally replayed as a HIP graph):
+//   bonds      one thread per bond: parallel-bond force, torque, breakage
+//              (src/cohesion.cuh:8-233); forces go to per-bond records, no
+//              scatter
+//   cellid     one thread per particle: periodic wrap, cell id, atomic
+//              count (src/sorting.cuh:7-43)
+//   scan       exclusive scan of the per-cell counts -> cellStart/cellEnd
+//   scatter    counting-sort placement of original indices
+//   cellsort   one thread per cell: order each cell's members by original
+//              index, which makes the counting sort stable (the reference's
+//              sort_by_key permutation, src/device.cu:763-765)
+//   reorder    gather x, vel, angvel into sorted copies + inverse
+//              permutation (src/sorting.cuh:49-121)
+//   contact    one thread per sorted particle: 27-cell search and slot
+//              update (src/contactsearch.cuh:181-355), contact and wall
+//              forces (:369-689, src/contactmodels.cuh:10-109,283-493),
+//              bond forces gathered in ascending bond index, leapfrog
+//              (src/integration.cuh:9-187)
+//   walls      per-wall tree sum of per-particle wall forces in the
+//              reference's block order, wall leapfrog
+//              (src/integration.cuh:191-295)
+//
+// Arithmetic keeps the reference's expression order and is built with
+// -ffp-contract=off, so the FP64 path reproduces the CPU oracle bit for bit.
+// Vector lengths use IEEE double sqrt (the reference narrows to float).
+
+#include <hip/hip_runtime.h>
+
+#include "dem_internal.h"
+
+namespace sph {
+
+namespace {
+
+constexpr double PI_REF = 3.14159265358979;   // src/constants.h:14
+constexpr double PI_WALL = 3.141596654;       // src/integration.cuh:258
+
+// ------------------------------------------------------------ small vectors
+template <class T> struct V3 { T x, y, z; };
+
+template <class T> __device__ __forceinline__ V3<T> mk(T x, T y, T z) { V3<T> r; r.x = x; r.y = y; r.z = z; return r; }
+template <class T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __device__ __forceinline__ T vlen(V3<T> a) { return sqrt(dot(a, a)); }
+template <class T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b) {
+    return mk<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return mk<T>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return mk<T>(a.x - b.x, a.y - b.y, a.z - b.z); }
+// scalar * vector and vector * scalar keep the operand order of
+// src/vector_arithmetic.h:150-157
+template <class T> __device__ __forceinline__ V3<T> operator*(T s, V3<T> a) { return mk<T>(s * a.x, s * a.y, s * a.z); }
+template <class T> __device__ __forceinline__ V3<T> operator*(V3<T> a, T s) { return mk<T>(a.x * s, a.y * s, a.z * s); }
+template <class T> __device__ __forceinline__ V3<T> operator/(V3<T> a, T s) { return mk<T>(a.x / s, a.y / s, a.z / s); }
+template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> a) { return mk<T>(-a.x, -a.y, -a.z); }
+
+__device__ __forceinline__ V3<double> xyz(double4 a) { return mk<double>(a.x, a.y, a.z); }
+template <class T> __device__ __forceinline__ V3<T> cvt(V3<double> a) { return mk<T>((T)a.x, (T)a.y, (T)a.z); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
+// walks one contiguous range of sorted particles and its L2 holds that
+// range's neighbourhood.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, xcd = b & 7u, k = b >> 3;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// ------------------------------------------------------------ cell hashing
+// Periodic wrap with the rule of src/integration.cuh:80-95.
+__device__ __forceinline__ bool wrap_xy(const DemDev& d, double& x, double& y) {
+    bool changed = false;
+    if (d.periodic == 1 || d.periodic == 2) {
+        if (x < d.origo[0]) { x += d.L[0]; changed = true; }
+        if (x > d.L[0]) { x -= d.L[0]; changed = true; }
+    }
+    if (d.periodic == 1) {
+        if (y < d.origo[1]) { y += d.L[1]; changed = true; }
+        if (y > d.L[1]) { y -= d.L[1]; changed = true; }
+    }
+    return changed;
+}
+
+// floor((v - origo)/(L/num)) (src/sorting.cuh:12-14), clamped to the grid.
+__device__ __forceinline__ int cell_coord(const DemDev& d, double v, int a, uint32_t& err) {
+    const double f = floor((v - d.origo[a]) / d.cell_len[a]);
+    if (f != f) { err |= FLAG_NAN; return 0; }
+    if (f < -1.0 || f > (double)d.num[a]) err |= FLAG_OUT_OF_GRID;
+    if (f < 0.0) return 0;
+    if (f > (double)(d.num[a] - 1)) return (int)(d.num[a] - 1);
+    return (int)f;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.np) return;
+    double4 x = d.x[i];
+    if (wrap_xy(d, x.x, x.y)) d.x[i] = x;
+    uint32_t err = 0;
+    const int ix = cell_coord(d, x.x, 0, err);
+    const int iy = cell_coord(d, x.y, 1, err);
+    const int iz = cell_coord(d, x.z, 2, err);
+    const uint32_t key = ((uint32_t)iz * d.num[1]) * d.num[0] + (uint32_t)iy * d.num[0] + (uint32_t)ix;
+    d.key[i] = key;
+    d.rank[i] = atomicAdd(&d.count[key], 1u);
+    if (err) atomicOr(d.flags, err);
+}
+
+// -------------------------------------------------------------------- scan
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+    // wave-level inclusive scan with shuffles, then across the 4 waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        const uint32_t ws = lds[w];
+        if (w < wave) woff += ws;
+        tot += ws;
+    }
+    __syncthreads();
+    total = tot;
+    return woff + inc - v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
+    __shared__ uint32_t lds[BLOCK / 64];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_PER_THREAD; ++j)
+        if (base + j < d.K) sum += d.count[base + j];
+    uint32_t total;
+    (void)block_exclusive_scan(sum, lds, total);
+    if (threadIdx.x == 0) d.scan_bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_top(DemDev d) {
+    __shared__ uint32_t lds[BLOCK / 64];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < d.scan_blocks; b0 += BLOCK) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < d.scan_blocks ? d.scan_bsum[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, lds, total);
+        if (i < d.scan_blocks) d.scan_bsum[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
+    __shared__ uint32_t lds[BLOCK / 64];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t c[SCAN_PER_THREAD];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        c[j] = base + j < d.K ? d.count[base + j] : 0u;
+        sum += c[j];
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive_scan(sum, lds, total) + d.scan_bsum[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        if (base + j < d.K) {
+            d.cellStart[base + j] = c[j] ? off : EMPTY_CELL;
+            d.cellEnd[base + j] = c[j] ? off + c[j] : EMPTY_CELL;
+        }
+        off += c[j];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(DemDev d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.np) return;
+    d.idx[d.cellStart[d.key[i]] + d.rank[i]] = i;
+}
+
+// Sort one cell's original indices ascending: odd-even transposition network
+// in registers for up to W members, insertion sort in memory beyond that.
+template <int W>
+__device__ __forceinline__ void sort_small(uint32_t* p, uint32_t n) {
+    uint32_t v[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = (uint32_t)j < n ? p[j] : 0xffffffffu;
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+#pragma unroll
+        for (int j = (r & 1); j + 1 < W; j += 2) {
+            const uint32_t a = v[j], b = v[j + 1];
+            v[j] = a < b ? a : b;
+            v[j + 1] = a < b ? b : a;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+        if ((uint32_t)j < n) p[j] = v[j];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cellsort(DemDev d) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.K) return;
+    const uint32_t st = d.cellStart[c];
+    if (st == EMPTY_CELL) return;
+    const uint32_t n = d.cellEnd[c] - st;
+    if (n < 2) return;
+    uint32_t* p = d.idx + st;
+    if (n <= 4) sort_small<4>(p, n);
+    else if (n <= 8) sort_small<8>(p, n);
+    else if (n <= 16) sort_small<16>(p, n);
+    else {
+        for (uint32_t i = 1; i < n; ++i) {
+            const uint32_t v = p[i];
+            uint32_t j = i;
+            while (j > 0 && p[j - 1] > v) { p[j] = p[j - 1]; --j; }
+            p[j] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= d.np) return;
+    const uint32_t o = d.idx[s];
+    d.xs[s] = d.x[o];
+    d.vs[s] = d.vel[o];
+    d.ws[s] = d.angvel[o];
+    d.inv[o] = s;
+}
+
+// ------------------------------------------------------------------- bonds
+// src/cohesion.cuh:8-233.  The force and torque of an intact bond go to
+// bond_f/bond_t; the contact kernel adds them per particle in ascending bond
+// index (the reference adds them with racing += at :220-226).
+__global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.nb0) return;
+    const uint2 bond = d.bonds[k];
+    if (!(bond.x < d.np)) return;
+    const double dt = d.dt;
+    const double4 d0 = d.bdelta[k], o0 = d.bomega[k];
+    V3<double> delta0_t = mk(d0.x, d0.y, d0.z);
+    const double delta0_n = d0.w;
+    V3<double> omega0_t = mk(o0.x, o0.y, o0.z);
+    const double omega0_n = o0.w;
+    const double4 x_i = d.x[bond.x], x_j = d.x[bond.y];
+    const double4 vel_i = d.vel[bond.x], vel_j = d.vel[bond.y];
+    const V3<double> angvel_i = xyz(d.angvel[bond.x]), angvel_j = xyz(d.angvel[bond.y]);
+    const double R_bar = d.lambda_bar * fmin(x_i.w, x_j.w);
+    const double A = PI_REF * R_bar * R_bar;
+    const double I = 0.25 * PI_REF * R_bar * R_bar * R_bar * R_bar;
+    const double J = I * 2.0;
+    const V3<double> x = mk(x_i.x - x_j.x, x_i.y - x_j.y, x_i.z - x_j.z);
+    const double x_length = vlen(x);
+    const double overlap = fmin(0.0, x_length - (x_i.w + x_j.w));
+    const V3<double> n = x / x_length;
+    delta0_t = delta0_t - (n * dot(n, delta0_t));
+    const V3<double> ddelta = (mk(vel_i.x - vel_j.x, vel_i.y - vel_j.y, vel_i.z - vel_j.z)
+                               + (x_i.w + overlap / 2.0) * cross(n, angvel_i)
+                               + (x_j.w + overlap / 2.0) * cross(n, angvel_j)) * dt;
+    const double ddelta_n = -dot(ddelta, n);
+    const double delta_n = delta0_n + ddelta_n;
+    const V3<double> ddelta_t = ddelta - n * dot(n, ddelta);
+    const V3<double> delta_t = delta0_t + ddelta_t;
+    const V3<double> f_n = (d.k_n * A * delta_n + d.gamma_n * ddelta_n / dt) * n;
+    const V3<double> f_t = -d.k_t * A * delta_t - d.gamma_t * ddelta_t / dt;
+    const V3<double> f = f_n + f_t;
+    const V3<double> mn = -n;
+    omega0_t = omega0_t - mn * dot(omega0_t, mn);
+    const V3<double> domega = mk(angvel_j.x - angvel_i.x, angvel_j.y - angvel_i.y, angvel_j.z - angvel_i.z) * dt;
+    const double domega_n = dot(domega, mn);
+    const double omega_n = omega0_n + domega_n;
+    const V3<double> domega_t = domega - domega_n * mn;
+    const V3<double> omega_t = omega0_t + domega_t;
+    const V3<double> t_n = -d.k_t * J * omega_n * mn;
+    const V3<double> t_t = -d.k_n * I * omega_t;
+    const V3<double> t_j = t_n + t_t;
+    const double sigma_max = vlen(f_n) / A + vlen(t_t) * R_bar / I;
+    const double tau_max = vlen(f_t) / A + vlen(t_n) * R_bar / J;
+    if (sigma_max >= d.sigma_b || tau_max >= d.tau_b) {
+        d.bonds[k].x = d.bond_broken;
+        return;
+    }
+    d.bdelta[k] = make_double4(delta_t.x, delta_t.y, delta_t.z, delta_n);
+    d.bomega[k] = make_double4(omega_t.x, omega_t.y, omega_t.z, omega_n);
+    d.bond_f[k] = make_double4(f.x, f.y, f.z, 0.0);
+    d.bond_t[k] = make_double4(t_j.x, t_j.y, t_j.z, 0.0);
+}
+
+// --------------------------------------------------------- contact models
+struct Acc {
+    V3<double> F, T;
+    double es_dot, ev_dot, p;
+};
+
+// src/contactmodels.cuh:283-493 in arithmetic T (double: the reference's
+// FP64 path; float: the mixed mode's pair force).  Sums go to FP64.
+template <class T>
+__device__ __forceinline__ void contact_linear(const DemDev& d, Acc& acc, V3<T> vel_a, V3<T> angvel_a,
+                                               V3<T> vel_b, V3<T> angvel_b, T radius_a, T radius_b,
+                                               V3<T> x, T x_length, T delta, V3<T>& delta_t_io) {
+    const T k_n = (T)d.k_n, k_t = (T)d.k_t, gamma_n = (T)d.gamma_n, gamma_t = (T)d.gamma_t;
+    const T mu_s = (T)d.mu_s, mu_d = (T)d.mu_d, kappa = (T)d.kappa, dt = (T)d.dt;
+    const V3<T> delta_t0_uncor = delta_t_io;
+    const V3<T> n = x / x_length;
+    const V3<T> vel_linear = vel_a - vel_b;
+    const V3<T> vel = vel_linear + (radius_a + delta / (T)2.0) * cross(n, angvel_a)
+                      + (radius_b + delta / (T)2.0) * cross(n, angvel_b);
+    const T vel_n = -dot(vel_linear, n);
+    const V3<T> vel_t = vel - n * dot(n, vel);
+    const T vel_t_length = vlen(vel_t);
+    const V3<T> delta_t0 = delta_t0_uncor - (n * dot(n, delta_t0_uncor));
+    const T delta_t0_length = vlen(delta_t0);
+    V3<T> delta_t = delta_t0;
+    V3<T> f_n = (-k_n * delta - gamma_n * vel_n) * n;
+    acc.ev_dot += (double)(gamma_n * vel_n * vel_n);
+    if (dot(f_n, n) < (T)0.0) f_n = mk<T>(0, 0, 0);
+    const T f_n_length = vlen(f_n);
+    const V3<T> f_c = (-kappa * (T)sqrt(radius_a * radius_b)) * n;
+    V3<T> f_t = mk<T>(0, 0, 0);
+    if (delta_t0_length > (T)0.0 || vel_t_length > (T)0.0) {
+        delta_t = delta_t0 + vel_t * dt;
+        const V3<T> f_t_elast = -k_t * delta_t;
+        const V3<T> f_t_visc = -gamma_t * vel_t;
+        f_t = f_t_elast + f_t_visc;
+        const T f_t_length = vlen(f_t);
+        const T f_t_limit = mu_s * vlen(f_n - f_c);
+        delta_t = delta_t0 + vel_t * dt;
+        if (f_t_length > f_t_limit) {
+            const V3<T> t = f_t / vlen(f_t);
+            f_t = f_t_limit * t;
+            delta_t = (T)-1.0 / k_t * (mu_d * vlen(f_n - f_c) * t + gamma_t * vel_t);
+            acc.es_dot += (double)(vlen(vlen(f_t) * vel_t * dt) / dt);
+        }
+    }
+    const V3<T> fsum = f_n + f_t + f_c;
+    acc.F = acc.F + mk<double>(fsum.x, fsum.y, fsum.z);
+    const V3<T> tq = cross(-(radius_a + delta * (T)0.5) * n, f_t);
+    acc.T = acc.T + mk<double>(tq.x, tq.y, tq.z);
+    acc.p += (double)(f_n_length / ((T)4.0 * (T)PI_REF * radius_a * radius_a));
+    delta_t_io = delta_t;
+}
+
+// src/contactmodels.cuh:10-109 contactLinear_wall (FP64 in both modes).
+__device__ __forceinline__ double contact_wall(const DemDev& d, Acc& acc, V3<double> linvel, V3<double> angvel,
+                                               double radius_a, V3<double> n, double delta, double wvel) {
+    const V3<double> v0 = linvel + radius_a * cross(n, angvel);
+    // operator+(Float3, Float) adds the scalar wall velocity to every component
+    const V3<double> vel = mk(v0.x + wvel, v0.y + wvel, v0.z + wvel);
+    const double vel_n = dot(vel, n);
+    const V3<double> vel_t = vel - n * dot(vel, n);
+    const double vel_t_length = vlen(vel_t);
+    V3<double> f_n = (-d.k_n * delta - d.gamma_wn * vel_n) * n;
+    if (dot(f_n, n) < 0.0) f_n = mk(0.0, 0.0, 0.0);
+    const double f_n_length = vlen(f_n);
+    V3<double> f_t = mk(0.0, 0.0, 0.0);
+    if (vel_t_length > 0.0) {
+        const double f_t_visc = d.gamma_wt * vel_t_length;
+        const double f_t_limit = vel_t_length > (double)0.001f ? d.mu_wd * f_n_length : d.mu_ws * f_n_length;
+        if (f_t_visc < f_t_limit) f_t = (-1.0 * f_t_visc) * vel_t / vel_t_length;
+        else f_t = (-1.0 * f_t_limit) * vel_t / vel_t_length;
+    }
+    acc.F = acc.F + (f_n + f_t);
+    const V3<double> tq = (-radius_a) * cross(n, f_t);
+    acc.T = acc.T + mk(tq.x + 0.0, tq.y + 0.0, tq.z + 0.0);
+    acc.p += f_n_length / (4.0 * PI_REF * radius_a * radius_a);
+    return dot(f_n, n);
+}
+
+__device__ __forceinline__ void decode_image(const DemDev& d, uint32_t code, double& dx, double& dy) {
+    const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
+    dx = cx == 1u ? d.L[0] : (cx == 2u ? -d.L[0] : 0.0);
+    dy = cy == 1u ? d.L[1] : (cy == 2u ? -d.L[1] : 0.0);
+}
+
+template <class T> struct SlotStore;
+template <> struct SlotStore<double> {
+    static __device__ __forceinline__ V3<double> load(const DemDev& d, size_t i) {
+        const double* p = d.cdt + 3 * i;
+        return mk(p[0], p[1], p[2]);
+    }
+    static __device__ __forceinline__ void store(const DemDev& d, size_t i, V3<double> v) {
+        double* p = d.cdt + 3 * i;
+        p[0] = v.x; p[1] = v.y; p[2] = v.z;
+    }
+};
+template <> struct SlotStore<float> {
+    static __device__ __forceinline__ V3<float> load(const DemDev& d, size_t i) {
+        const float* p = d.cdt32 + 3 * i;
+        return mk(p[0], p[1], p[2]);
+    }
+    static __device__ __forceinline__ void store(const DemDev& d, size_t i, V3<float> v) {
+        float* p = d.cdt32 + 3 * i;
+        p[0] = v.x; p[1] = v.y; p[2] = v.z;
+    }
+};
+
+// ----------------------------------------------------------- contact kernel
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= d.np) return;
+    const uint32_t o = d.idx[s];
+    const double4 xa = d.xs[s];
+    const double ra = xa.w;
+    uint32_t err = 0;
+
+    // ---- topology: 27 cells in z, y, x order (src/contactsearch.cuh:340-353)
+    uint32_t mask = d.cmask[o];
+    uint32_t fresh = 0;  // slots taken from empty this step: their delta_t starts at 0
+    uint32_t* slot = d.cslot + (size_t)o * NC;
+    const int gx = cell_coord(d, xa.x, 0, err), gy = cell_coord(d, xa.y, 1, err), gz = cell_coord(d, xa.z, 2, err);
+    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
+    uint32_t nc = 0;
+    bool stop = false;
+    for (int dz = -1; dz <= 1 && !stop; ++dz) {
+        const int tz = gz + dz;
+        if (tz < 0 || tz == nz) continue;
+        for (int dy = -1; dy <= 1 && !stop; ++dy) {
+            int ty = gy + dy;
+            uint32_t cy = 0;
+            if (ty < 0 || ty == ny) {
+                if (d.periodic != 1) continue;
+                cy = ty < 0 ? 1u : 2u;
+                ty = ty < 0 ? ny - 1 : 0;
+            }
+            for (int dx = -1; dx <= 1 && !stop; ++dx) {
+                int tx = gx + dx;
+                uint32_t cx = 0;
+                if (tx < 0 || tx == nx) {
+                    if (d.periodic == 0) continue;
+                    cx = tx < 0 ? 1u : 2u;
+                    tx = tx < 0 ? nx - 1 : 0;
+                }
+                const uint32_t code = cx | (cy << 2);
+                double dmx, dmy;
+                decode_image(d, code, dmx, dmy);
+                const uint32_t cid = (uint32_t)tx + (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+                const uint32_t st = d.cellStart[cid];
+                if (st == EMPTY_CELL) continue;
+                const uint32_t en = d.cellEnd[cid];
+                for (uint32_t sb = st; sb < en; ++sb) {
+                    if (sb == s) continue;
+                    const double4 xb = d.xs[sb];
+                    const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
+                    const double delta = vlen(xab) - (ra + xb.w);
+                    if (delta < 0.0) {
+                        const uint32_t bo = d.idx[sb];
+                        // src/contactsearch.cuh:252-265: the slot already holding b
+                        // (lowest), else the highest empty slot
+                        int cpos = -1;
+                        for (uint32_t m = mask; m; m &= m - 1) {
+                            const int i = __ffs(m) - 1;
+                            if ((slot[i] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+                        }
+                        if (cpos < 0) {
+                            const uint32_t empty = ~mask;
+                            if (empty == 0u) { err |= FLAG_CONTACT_OVERFLOW; stop = true; break; }
+                            cpos = 31 - __clz(empty);
+                            fresh |= 1u << cpos;
+                        }
+                        slot[cpos] = bo | (code << 28);
+                        mask |= 1u << cpos;
+                        if (++nc > (uint32_t)NC) { err |= FLAG_CONTACT_OVERFLOW; stop = true; break; }
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
+    const double4 va4 = d.vs[s];
+    const V3<double> va = xyz(va4), wa = xyz(d.ws[s]);
+    Acc acc;
+    acc.F = mk(0.0, 0.0, 0.0);
+    acc.T = mk(0.0, 0.0, 0.0);
+    acc.es_dot = 0.0; acc.ev_dot = 0.0; acc.p = 0.0;
+    for (uint32_t m = mask; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        const uint32_t w = slot[i];
+        const uint32_t bo = w & SLOT_IDX_MASK;
+        const uint32_t sb = d.inv[bo];
+        const double4 xb = d.xs[sb];
+        double dmx, dmy;
+        decode_image(d, w >> 28, dmx, dmy);
+        const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
+        const double len = vlen(xab);
+        const double delta_n = len - (ra + xb.w);
+        if (delta_n < 0.0) {
+            const size_t mp = (size_t)o * NC + i;
+            V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
+            contact_linear<T>(d, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.vs[sb])), cvt<T>(xyz(d.ws[sb])),
+                              (T)ra, (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
+            SlotStore<T>::store(d, mp, dt0);
+        } else {
+            mask &= ~(1u << i);  // contact ended: the slot is empty again
+        }
+    }
+    d.cmask[o] = mask;
+
+    // ---- walls (src/contactsearch.cuh:587-663)
+    double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
+    {
+        const double4 w0 = d.wnx[0];
+        const double dw = w0.w - (xa.z + ra);
+        if (dw < 0.0) wf0 = contact_wall(d, acc, va, wa, ra, mk(w0.x, w0.y, w0.z), dw, d.wmvfd[0].y);
+    }
+    {
+        const double dw = xa.z - ra - d.origo[2];
+        if (dw < 0.0) (void)contact_wall(d, acc, va, wa, ra, mk(0.0, 0.0, 1.0), dw, 0.0);
+    }
+    if (d.periodic == 0) {
+        const double4 w1 = d.wnx[1], w2 = d.wnx[2];
+        double dw = w1.w - (xa.x + ra);
+        if (dw < 0.0) wf1 = contact_wall(d, acc, va, wa, ra, mk(w1.x, w1.y, w1.z), dw, d.wmvfd[1].y);
+        dw = xa.x - ra - w2.w;
+        if (dw < 0.0) wf2 = contact_wall(d, acc, va, wa, ra, mk(w2.x, w2.y, w2.z), dw, d.wmvfd[2].y);
+    }
+    if (d.periodic == 0 || d.periodic == 2) {
+        const double4 w3 = d.wnx[3], w4 = d.wnx[4];
+        double dw = w3.w - (xa.y + ra);
+        if (dw < 0.0) wf3 = contact_wall(d, acc, va, wa, ra, mk(w3.x, w3.y, w3.z), dw, d.wmvfd[3].y);
+        dw = xa.y - ra - w4.w;
+        if (dw < 0.0) wf4 = contact_wall(d, acc, va, wa, ra, mk(w4.x, w4.y, w4.z), dw, d.wmvfd[4].y);
+    }
+
+    // ---- per-particle outputs (src/contactsearch.cuh:669-687)
+    const double dt = d.dt;
+    d.es_dot[o] = acc.es_dot;
+    d.ev_dot[o] = acc.ev_dot;
+    d.es[o] += acc.es_dot * dt;
+    d.ev[o] += acc.ev_dot * dt;
+    d.p[o] = acc.p;
+    if (d.nw > 0) d.wall_pp[o] = wf0;
+    if (d.nw > 1) d.wall_pp[(size_t)d.np + o] = wf1;
+    if (d.nw > 2) d.wall_pp[2 * (size_t)d.np + o] = wf2;
+    if (d.nw > 3) d.wall_pp[3 * (size_t)d.np + o] = wf3;
+    if (d.nw > 4) d.wall_pp[4 * (size_t)d.np + o] = wf4;
+
+    // ---- bond forces in ascending bond index (src/cohesion.cuh:220-226)
+    V3<double> force = acc.F, torque = acc.T;
+    if (d.nb0) {
+        const uint32_t e0 = d.bcsr_off[o], e1 = d.bcsr_off[o + 1];
+        for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t ent = d.bcsr_ent[e];
+            const uint32_t k = ent >> 1;
+            if (!(d.bonds[k].x < d.np)) continue;
+            const V3<double> f = xyz(d.bond_f[k]), t = xyz(d.bond_t[k]);
+            if ((ent & 1u) == 0u) { force = force + f; torque = torque - t; }
+            else { force = force - f; torque = torque + t; }
+        }
+    }
+    d.force[o] = make_double4(force.x, force.y, force.z, 0.0);
+    d.torque[o] = make_double4(torque.x, torque.y, torque.z, 0.0);
+
+    // ---- leapfrog (src/integration.cuh:20-185)
+    const double radius = ra;
+    const double m = 4.0 / 3.0 * PI_REF * radius * radius * radius * d.rho;
+    double ax = force.x / m, ay = force.y / m, az = force.z / m;
+    double bx = torque.x * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    double by = torque.y * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    double bz = torque.z * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    ax += d.g[0]; ay += d.g[1]; az += d.g[2];
+    if (va4.w > 0.0) {
+        ax = 0.0; ay = 0.0; az -= d.g[2];
+        bx = 0.0; by = 0.0; bz = 0.0;
+    }
+    double px = xa.x, py = xa.y, pz = xa.z;
+    (void)wrap_xy(d, px, py);
+    double4 v0 = d.vel0[o], w0 = d.angvel0[o], ap = d.angpos[o];
+    v0.x += ax * dt; v0.y += ay * dt; v0.z += az * dt;
+    w0.x += bx * dt; w0.y += by * dt; w0.z += bz * dt;
+    px += v0.x * dt; py += v0.y * dt; pz += v0.z * dt;
+    ap.x += w0.x * dt; ap.y += w0.y * dt; ap.z += w0.z * dt;
+    const double vx = v0.x + 0.5 * ax * dt, vy = v0.y + 0.5 * ay * dt, vz = v0.z + 0.5 * az * dt;
+    const double wx = w0.x + 0.5 * bx * dt, wy = w0.y + 0.5 * by * dt, wz = w0.z + 0.5 * bz * dt;
+    double sx = 0.0, sy = 0.0;
+    sx += vx * dt;
+    sy += vy * dt;
+    double2 xy = d.xysum[o];
+    xy.x += sx; xy.y += sy;
+    d.xysum[o] = xy;
+    d.angvel[o] = make_double4(wx, wy, wz, 0.0);
+    d.angvel0[o] = w0;
+    d.vel[o] = make_double4(vx, vy, vz, va4.w);
+    d.vel0[o] = v0;
+    d.angpos[o] = ap;
+    d.x[o] = make_double4(px, py, pz, radius);
+    if (err) atomicOr(d.flags, err);
+}
+
+// ------------------------------------------------------------------- walls
+// src/integration.cuh:191-221: blocks of 256 original indices, tree halving.
+__global__ __launch_bounds__(BLOCK) void k_wallsum(DemDev d) {
+    __shared__ double cache[BLOCK];
+    const uint32_t w = blockIdx.y;
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    double temp = 0.0;
+    if (i < d.np) temp += d.wall_pp[(size_t)w * d.np + i];
+    cache[threadIdx.x] = temp;
+    __syncthreads();
+    for (uint32_t h = BLOCK / 2; h != 0; h >>= 1) {
+        if (threadIdx.x < h) cache[threadIdx.x] += cache[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.wall_partial[(size_t)w * d.nwall_blocks + blockIdx.x] = cache[0];
+}
+
+// src/integration.cuh:224-295, one thread per wall; then time.current += dt.
+__global__ __launch_bounds__(64) void k_walls(DemDev d) {
+    const uint32_t w = threadIdx.x;
+    const double t_current = *d.t_cur;
+    if (w < d.nw && d.wmode[w] != 0) {
+        const int wmode = d.wmode[w];
+        double4 w_nx = d.wnx[w], w_mvfd = d.wmvfd[w];
+        double vel0 = d.wvel0[w];
+        const double* part = d.wall_partial + (size_t)w * d.nwall_blocks;
+        double fz = 0.0;
+        for (uint32_t i = 0; i < d.nwall_blocks; ++i) fz += part[i];
+        w_mvfd.z = fz;
+        const double dt = d.dt;
+        const double sigma_0 = w_mvfd.w + d.devs_A * sin(2.0 * PI_WALL * d.devs_f * t_current);
+        const double N = -sigma_0 * d.L[0] * d.L[1];
+        double acc = (w_mvfd.z + N) / w_mvfd.x;
+        if (wmode == 2) acc = 0.0;
+        vel0 += acc * dt;
+        w_nx.w += vel0 * dt;
+        w_mvfd.y = vel0 + 0.5 * acc * dt;
+        d.wnx[w] = w_nx;
+        d.wmvfd[w] = w_mvfd;
+        d.wvel0[w] = vel0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *d.t_cur = t_current + d.dt;
+}
+
+inline uint32_t nblk(uint64_t n, uint32_t b = BLOCK) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n) k_fill_u32<<<nblk(n), BLOCK, 0, s>>>(p, n, v);
+}
+void launch_bonds(const DemDev& d, hipStream_t s) {
+    if (d.nb0) k_bonds<<<nblk(d.nb0), BLOCK, 0, s>>>(d);
+}
+void launch_cellid(const DemDev& d, hipStream_t s) { k_cellid<<<nblk(d.np), BLOCK, 0, s>>>(d); }
+void launch_scan(const DemDev& d, hipStream_t s) {
+    k_scan_reduce<<<d.scan_blocks, BLOCK, 0, s>>>(d);
+    k_scan_top<<<1, BLOCK, 0, s>>>(d);
+    k_scan_down<<<d.scan_blocks, BLOCK, 0, s>>>(d);
+}
+void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BLOCK, 0, s>>>(d); }
+void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
+void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
+void launch_contact(const DemDev& d, int precision, hipStream_t s) {
+    if (precision == SPH_PREC_MIXED) k_contact<float><<<nblk(d.np), BLOCK, 0, s>>>(d);
+    else k_contact<double><<<nblk(d.np), BLOCK, 0, s>>>(d);
+}
+void launch_walls(const DemDev& d, hipStream_t s) {
+    if (d.nw) k_wallsum<<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);
+    k_walls<<<1, 64, 0, s>>>(d);
+}
+
+}  // namespace sph

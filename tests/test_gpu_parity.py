@@ -1,0 +1,105 @@
+"""GPU vs CPU-oracle parity of the FP64 step, bit for bit, through the C-ABI."""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from parity import run_pair
+from sphere_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_fp64_sqrt_div_correctly_rounded(tmp_path):
+    """The parity of the whole path rests on IEEE-correct f64 sqrt and divide on
+    gfx950: a two-sphere contact's overlap goes through both; compare one step
+    over many random separations and radii against the oracle."""
+    rng = np.random.default_rng(7)
+    for trial in range(8):
+        d = float(rng.uniform(-0.2, -1e-6))
+        sb = configs.two_spheres(distance=d, radius=float(rng.uniform(0.1, 0.5)),
+                                 v=tuple(rng.uniform(-1e-3, 1e-3, 3)), w=tuple(rng.uniform(-1, 1, 3)),
+                                 gamma=float(rng.uniform(0, 10)), k=1e5, sid=f"pair{trial}")
+        errs = run_pair(sb, tmp_path, [1, 3])
+        assert not errs, errs
+
+
+@pytest.mark.parametrize("periodic", [0, 1, 2])
+def test_small_settle_parity(tmp_path, periodic):
+    _say(f"small settle periodic={periodic}")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=periodic,
+                                gamma_n=0.0, gamma_t=10.0, sid=f"settle-p{periodic}")
+    errs = run_pair(sb, tmp_path, [1, 10, 200, 800])
+    assert not errs, errs
+
+
+def test_c0_settle_parity(tmp_path):
+    """4,096-sphere x/y-periodic settling case, 2,000 steps, checked at intervals."""
+    _say("c0 settle 4096")
+    sb = configs.lattice_settle()
+    errs = run_pair(sb, tmp_path, [1, 99, 400, 1500])
+    assert not errs, errs
+
+
+def test_c0_blowup_same_step(tmp_path):
+    """The reference's model-2 normal dashpot adds energy (vel_n = -dot(v, n),
+    src/contactmodels.cuh:341,366), so the 4,096-sphere settling case leaves
+    the grid once the lowest layer collides.  The GPU must fail at the same
+    step as the oracle, with the same state up to that step."""
+    from oracle_lib import Oracle
+    from sphere_amd.dem import DEM, OutOfGrid
+
+    _say("c0 blow-up step")
+    sb = configs.lattice_settle()
+    orc = Oracle.from_spherebin(sb, tmp_path)
+    dem = DEM(sb, verbosity=0)
+    dem.step(2000)
+    assert orc.step(2000) == 0
+    gpu_fail = orc_fail = None
+    for k in range(2001, 3001):
+        if orc_fail is None and orc.step(1):
+            orc_fail = k
+        if gpu_fail is None:
+            try:
+                dem.step(1)
+            except OutOfGrid:
+                gpu_fail = k
+        if gpu_fail and orc_fail:
+            break
+    dem.close()
+    assert orc_fail is not None
+    assert gpu_fail == orc_fail
+
+
+def test_top_wall_stress_parity(tmp_path):
+    _say("stress wall")
+    sb = configs.lattice_settle(nside=(6, 6, 6), spacing=2.25e-2, jitter=5e-4, periodic=1, nw=1,
+                                gamma_n=0.0, gamma_t=10.0, sid="wall")
+    sb.wmode = np.array([1], dtype=np.int32)
+    sb.w_devs[0] = 5e3
+    sb.w_x[0] = float(np.max(sb.x[:, 2] + sb.radius))
+    sb.w_m[0] = 10.0
+    errs = run_pair(sb, tmp_path, [1, 50, 500])
+    assert not errs, errs
+
+
+def test_bond_pair_parity(tmp_path):
+    _say("bond pair")
+    sb = configs.two_spheres(distance=0.0, radius=0.5, v=(1e-4, 2e-5, -3e-5), w=(1e-4, -2e-4, 5e-5),
+                             bonded=True, k=1.16e9, sid="bondpair")
+    errs = run_pair(sb, tmp_path, [1, 100, 1000])
+    assert not errs, errs
+
+
+def test_bonded_lattice_parity(tmp_path):
+    _say("bonded fcc")
+    sb = configs.fcc_bonded(cells=(3, 3, 4), sigma_b=2e7, tau_b=2e7)
+    errs = run_pair(sb, tmp_path, [1, 50, 300])
+    assert not errs, errs
