@@ -165,6 +165,14 @@ int sph_get_timing(const sph_t* h, sph_timing_t* out);
 /* 1 = replay the captured step as a HIP graph (default), 0 = plain launches. */
 int sph_set_graph(sph_t* h, int on);
 
+/* Setup helper (host only): seeded non-overlapping random sequential packing
+ * of n spheres with the given radii in box[3] (periodic: 0 none, 1 x+y, 2 x),
+ * largest first; xyz[3n] receives the centres.  Stands in for the
+ * reference's unseeded initRandomPos (python/sphere.py:856-908).  Returns 0
+ * or -1 when a sphere finds no free spot in max_tries draws. */
+int sph_pack_rsa(uint64_t n, const double* radii, const double* box, int periodic, uint64_t seed,
+                 uint64_t max_tries, double* xyz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -456,7 +456,12 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
                     if (sb == s) continue;
                     const double4 xb = d.xs[sb];
                     const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
-                    const double delta = vlen(xab) - (ra + xb.w);
+                    const double rsum = ra + xb.w;
+                    const double d2 = dot(xab, xab);
+                    // d2 above (ra+rb)^2 by more than rounding: sqrt(d2) >= ra+rb,
+                    // so the exact test below would fail; skip the sqrt
+                    if (d2 > rsum * rsum * 1.0000001) continue;
+                    const double delta = sqrt(d2) - rsum;
                     if (delta < 0.0) {
                         const uint32_t bo = d.idx[sb];
                         // src/contactsearch.cuh:252-265: the slot already holding b
@@ -620,17 +625,43 @@ __global__ __launch_bounds__(BLOCK) void k_wallsum(DemDev d) {
 }
 
 // src/integration.cuh:224-295, one thread per wall; then time.current += dt.
-__global__ __launch_bounds__(64) void k_walls(DemDev d) {
+// The wall force is the serial sum of the block partials in block order; the
+// block stages the partials through LDS in chunks so the summing thread reads
+// LDS instead of waiting on HBM for every term.
+constexpr uint32_t WALL_CHUNK = 4096;
+__global__ __launch_bounds__(BLOCK) void k_walls(DemDev d) {
+    __shared__ double part[WALL_CHUNK];
+    __shared__ double fz[6];
     const uint32_t w = threadIdx.x;
     const double t_current = *d.t_cur;
+    for (uint32_t wall = 0; wall < d.nw; ++wall) {
+        if (d.wmode[wall] == 0) continue;
+        const double* src = d.wall_partial + (size_t)wall * d.nwall_blocks;
+        double acc = 0.0;
+        for (uint32_t c0 = 0; c0 < d.nwall_blocks; c0 += WALL_CHUNK) {
+            const uint32_t cn = d.nwall_blocks - c0 < WALL_CHUNK ? d.nwall_blocks - c0 : WALL_CHUNK;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < cn; i += BLOCK) part[i] = src[c0 + i];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t i = 0;
+                for (; i + 8 <= cn; i += 8) {
+                    const double p0 = part[i], p1 = part[i + 1], p2 = part[i + 2], p3 = part[i + 3];
+                    const double p4 = part[i + 4], p5 = part[i + 5], p6 = part[i + 6], p7 = part[i + 7];
+                    acc += p0; acc += p1; acc += p2; acc += p3;
+                    acc += p4; acc += p5; acc += p6; acc += p7;
+                }
+                for (; i < cn; ++i) acc += part[i];
+            }
+        }
+        if (threadIdx.x == 0) fz[wall] = acc;
+    }
+    __syncthreads();
     if (w < d.nw && d.wmode[w] != 0) {
         const int wmode = d.wmode[w];
         double4 w_nx = d.wnx[w], w_mvfd = d.wmvfd[w];
         double vel0 = d.wvel0[w];
-        const double* part = d.wall_partial + (size_t)w * d.nwall_blocks;
-        double fz = 0.0;
-        for (uint32_t i = 0; i < d.nwall_blocks; ++i) fz += part[i];
-        w_mvfd.z = fz;
+        w_mvfd.z = fz[w];
         const double dt = d.dt;
         const double sigma_0 = w_mvfd.w + d.devs_A * sin(2.0 * PI_WALL * d.devs_f * t_current);
         const double N = -sigma_0 * d.L[0] * d.L[1];
@@ -672,7 +703,7 @@ void launch_contact(const DemDev& d, int precision, hipStream_t s) {
 }
 void launch_walls(const DemDev& d, hipStream_t s) {
     if (d.nw) k_wallsum<<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);
-    k_walls<<<1, 64, 0, s>>>(d);
+    k_walls<<<1, BLOCK, 0, s>>>(d);
 }
 
 }  // namespace sph

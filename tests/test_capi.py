@@ -1,0 +1,46 @@
+"""The C-ABI library loads and exports every entry point its header declares
+(no compute calls: they need a GPU)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+from sphere_amd import dem
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            text = open(os.path.join(inc, f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names.update(re.findall(r"\b(sph_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = dem.load_library()
+    declared = _declared()
+    assert len(declared) >= 20, sorted(declared)
+    for name in sorted(declared):
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+
+
+def test_exports_have_c_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", dem.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\b(sph_[a-z0-9_]+)$", out, flags=re.M))
+    assert _declared() <= exported
+
+
+def test_wrapper_binds_every_export():
+    assert _declared() <= set(dem.EXPORTS)
+
+
+def test_library_is_gfx950_only():
+    data = open(dem.LIB_PATH, "rb").read()
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa-+(gfx[0-9a-z]+)", data))
+    assert targets == {b"gfx950"}, targets
