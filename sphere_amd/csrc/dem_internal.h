@@ -1,17 +1,27 @@
 // dem_internal.h -- device layout and kernel entry points of the gfx950 DEM step.
 //
-// All per-particle state lives in HBM in the original particle order (the
-// order of the .bin file), as 32-byte double4 records so that the scattered
-// per-particle writes of the fused contact/integrate kernel fill whole
-// 32-byte sectors.  Sorted copies of position, velocity and spin (xs, vs, ws)
-// are rebuilt every step by the counting sort; neighbour reads go to them, so
-// a contact's partner is read from the spatially sorted, cache-resident copy.
+// Per-particle state is kept in HBM in the cell-sorted order of the last
+// step, not in the .bin file's order: every per-particle kernel then reads
+// and writes consecutive records, whatever order the input numbered the
+// particles in.  Each particle carries its original index (`id`), which is
+// what the reference's semantics are stated in (the stable sort's tie-break,
+// contact partners, bond endpoints, the wall-force reduction order and the
+// output file order).  Two buffer sets hold the state:
 //
-// Contact history per particle: a 32-bit occupancy mask plus SPH_NC slots.
-// A slot word packs the partner's original index (low 28 bits) and the
-// periodic image code (high 4 bits: 2 bits per periodic axis, 0 = none,
-// 1 = +L, 2 = -L); the tangential spring delta_t of a slot is 3 doubles
-// (FP64) or 3 floats (mixed).  Only occupied slots are read or written.
+//   b  state at the start of a step, in the previous step's sorted order
+//   a  the same state gathered into this step's sorted order (reorder)
+//
+// The contact kernel reads a (own particle and neighbours) and writes the
+// integrated state back to b at the same sorted position, so neighbour reads
+// never race with updates.  Records are 32-byte double4 so each access fills
+// whole sectors.
+//
+// Contact history stays keyed by original index: a 32-bit occupancy mask
+// (moved with the particle) plus SPH_NC slots.  A slot word packs the
+// partner's original index (low 28 bits) and the periodic image code (high 4
+// bits: 2 bits per periodic axis, 0 = none, 1 = +L, 2 = -L); the tangential
+// spring delta_t of a slot is 3 doubles (FP64) or 3 floats (mixed).  Only
+// occupied slots are read or written.
 #ifndef SPHERE_DEM_INTERNAL_H_
 #define SPHERE_DEM_INTERNAL_H_
 
@@ -37,6 +47,19 @@ enum : uint32_t {
     FLAG_NAN = 4u,
 };
 
+// One buffer set of per-particle state in storage order.
+struct PState {
+    double4 *x;       // x, y, z, r
+    double4 *vel;     // vx, vy, vz, fixvel
+    double4 *angvel;  // wx, wy, wz, 0
+    double4 *vel0;    // half-step velocity
+    double4 *angvel0;
+    double4 *angpos;
+    double4 *acc;     // xysum.x, xysum.y, es, ev
+    uint32_t *id;     // original index
+    uint32_t *cmask;  // occupied contact slots
+};
+
 // Everything a step kernel needs, passed by value as one kernel argument.
 struct DemDev {
     uint32_t np, nb0, nw, K;
@@ -51,27 +74,27 @@ struct DemDev {
     double gamma_wn, gamma_wt, mu_ws, mu_wd, rho, kappa;
     double lambda_bar, sigma_b, tau_b, devs_A, devs_f;
 
-    // particle state, original order
-    double4 *x, *vel, *angvel, *vel0, *angvel0, *angpos, *force, *torque;
-    double2* xysum;
-    double *es_dot, *es, *ev_dot, *ev, *p;
+    PState a, b;
+    uint32_t* pos_of;   // original index -> sorted position of this step
 
-    // contact history
-    uint32_t* cmask;    // np
+    // per-step outputs, sorted order of the step
+    double4 *force, *torque;
+    double4 *rates;     // es_dot, ev_dot, p, 0
+
+    // contact history, keyed by original index
     uint32_t* cslot;    // np * NC
     double* cdt;        // np * NC * 3 (FP64 mode)
     float* cdt32;       // np * NC * 3 (mixed mode)
 
     // cell list
-    uint32_t *key, *rank, *idx, *inv, *count, *cellStart, *cellEnd, *scan_bsum;
+    uint32_t *key, *rank, *perm, *count, *off, *scan_bsum;  // off: K+1 prefix
     uint32_t scan_blocks;
-    double4 *xs, *vs, *ws;
 
     // walls (6 entries; entries >= nw hold the reference's defaults)
     double4 *wnx, *wmvfd;
     double* wvel0;
     int* wmode;
-    double* wall_pp;       // nw * np
+    double* wall_pp;       // nw * np, original order
     double* wall_partial;  // nw * nwall_blocks
     double* t_cur;         // device copy of time.current
 

@@ -155,41 +155,61 @@ int validate_for_device(sph_dem* h) {
     return SPH_OK;
 }
 
+// Host-side view of the device state in original particle order.
+struct Snapshot {
+    std::vector<uint32_t> id;       // storage position -> original index
+    std::vector<double> x, vel, angvel, vel0, angvel0, angpos, acc;  // 4 per particle, original order
+};
+
 int upload_all(sph_dem* h) {
     HostState& s = h->hs;
     DemDev& d = h->d;
     const size_t n = s.np;
-    std::vector<double> fix(n);
-    for (size_t i = 0; i < n; ++i) fix[i] = s.vel[4 * i + 3];
-    std::vector<double> v3(3 * n);
-    for (size_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) v3[3 * i + a] = s.vel[4 * i + a];
-    const std::vector<double> angvel4 = pad4(s.angvel, n), angpos4 = pad4(s.angpos, n);
-    const std::vector<double> force4 = pad4(s.force, n), torque4 = pad4(s.torque, n);
-    const std::vector<double> vel0_4 = pad4(v3, n, fix.data());
-    HIP_TRY(hipMemcpy(d.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.vel0, vel0_4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.angvel, angvel4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.angvel0, angvel4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.angpos, angpos4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.force, force4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.torque, torque4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.xysum, s.xysum.data(), n * 16, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.es_dot, s.es_dot.data(), n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.es, s.es.data(), n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.ev_dot, s.ev_dot.data(), n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.ev, s.ev.data(), n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.p, s.p.data(), n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(d.cmask, 0, n * 4));
+    // storage starts in original order: b holds the state, id and pos_of are the identity
+    std::vector<double> v4(4 * n), w4(4 * n), p4(4 * n), acc(4 * n), f4(4 * n), t4(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            w4[4 * i + a] = s.angvel[3 * i + a];
+            p4[4 * i + a] = s.angpos[3 * i + a];
+            f4[4 * i + a] = s.force[3 * i + a];
+            t4[4 * i + a] = s.torque[3 * i + a];
+        }
+        acc[4 * i + 0] = s.xysum[2 * i + 0];
+        acc[4 * i + 1] = s.xysum[2 * i + 1];
+        acc[4 * i + 2] = s.es[i];
+        acc[4 * i + 3] = s.ev[i];
+    }
+    std::vector<uint32_t> iota(n);
+    for (size_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
+    std::vector<double> rates(4 * n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        rates[4 * i + 0] = s.es_dot[i];
+        rates[4 * i + 1] = s.ev_dot[i];
+        rates[4 * i + 2] = s.p[i];
+    }
+    HIP_TRY(hipMemcpy(d.b.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.b.vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
+    // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel
+    HIP_TRY(hipMemcpy(d.b.vel0, s.vel.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.b.angvel, w4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.b.angvel0, w4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.b.angpos, p4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.b.acc, acc.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.b.id, iota.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.a.id, iota.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.pos_of, iota.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d.b.cmask, 0, n * 4));
+    HIP_TRY(hipMemset(d.a.cmask, 0, n * 4));
+    HIP_TRY(hipMemcpy(d.force, f4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.torque, t4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.rates, rates.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.cslot, 0, n * sph::NC * 4));
     HIP_TRY(hipMemset(d.cdt, 0, n * sph::NC * 3 * 8));
     HIP_TRY(hipMemset(d.flags, 0, 4));
     HIP_TRY(hipMemset(d.key, 0, n * 4));
-    HIP_TRY(hipMemset(d.idx, 0, n * 4));
-    HIP_TRY(hipMemset(d.cellStart, 0xff, (size_t)d.K * 4));
-    HIP_TRY(hipMemset(d.cellEnd, 0xff, (size_t)d.K * 4));
-    HIP_TRY(hipMemset(d.xs, 0, n * 32));
+    HIP_TRY(hipMemset(d.perm, 0, n * 4));
+    HIP_TRY(hipMemset(d.off, 0, ((size_t)d.K + 1) * 4));
+    HIP_TRY(hipMemcpy(d.a.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     // walls: entries past nw keep the defaults of src/contactsearch.cuh:416-427
     double wnx[24] = {0.0, 0.0, -1.0, s.L[2], -1.0, 0.0, 0.0, s.L[0], 1.0, 0.0, 0.0, 0.0,
                       0.0, -1.0, 0.0, s.L[1], 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -246,17 +266,40 @@ int upload_all(sph_dem* h) {
     return SPH_OK;
 }
 
+// Copies a storage-order double4 array into original order (w kept).
+int fetch4(sph_dem* h, const double4* src, const std::vector<uint32_t>& id, std::vector<double>& out) {
+    const size_t n = h->hs.np;
+    std::vector<double> t(4 * n);
+    HIP_TRY(hipMemcpy(t.data(), src, n * 32, hipMemcpyDeviceToHost));
+    out.resize(4 * n);
+    for (size_t q = 0; q < n; ++q)
+        for (int a = 0; a < 4; ++a) out[4 * (size_t)id[q] + a] = t[4 * q + a];
+    return SPH_OK;
+}
+
+int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) {
+    id.resize(h->hs.np);
+    HIP_TRY(hipMemcpy(id.data(), h->d.b.id, h->hs.np * 4, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int alloc_pstate(sph_dem* h, sph::PState& p, size_t n) {
+    ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.vel0, n); ALLOC(p.angvel0, n);
+    ALLOC(p.angpos, n); ALLOC(p.acc, n); ALLOC(p.id, n); ALLOC(p.cmask, n);
+    return SPH_OK;
+}
+
 int allocate(sph_dem* h) {
     DemDev& d = h->d;
     const size_t n = h->hs.np;
-    ALLOC(d.x, n); ALLOC(d.vel, n); ALLOC(d.angvel, n); ALLOC(d.vel0, n); ALLOC(d.angvel0, n);
-    ALLOC(d.angpos, n); ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.xysum, n);
-    ALLOC(d.es_dot, n); ALLOC(d.es, n); ALLOC(d.ev_dot, n); ALLOC(d.ev, n); ALLOC(d.p, n);
-    ALLOC(d.cmask, n); ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
+    int rc;
+    if ((rc = alloc_pstate(h, d.a, n)) || (rc = alloc_pstate(h, d.b, n))) return rc;
+    ALLOC(d.pos_of, n);
+    ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.rates, n);
+    ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
     d.cdt32 = nullptr;
-    ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.idx, n); ALLOC(d.inv, n);
-    ALLOC(d.count, d.K); ALLOC(d.cellStart, d.K); ALLOC(d.cellEnd, d.K); ALLOC(d.scan_bsum, d.scan_blocks);
-    ALLOC(d.xs, n); ALLOC(d.vs, n); ALLOC(d.ws, n);
+    ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
+    ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
     ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
     ALLOC(d.t_cur, 1);
@@ -339,19 +382,33 @@ int download_all(sph_dem* h) {
     DemDev& d = h->d;
     const size_t n = s.np;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    std::vector<double> t4(4 * n);
-    HIP_TRY(hipMemcpy(s.x.data(), d.x, n * 32, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(s.vel.data(), d.vel, n * 32, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(s.xysum.data(), d.xysum, n * 16, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(t4.data(), d.force, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.force.data());
-    HIP_TRY(hipMemcpy(t4.data(), d.angpos, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.angpos.data());
-    HIP_TRY(hipMemcpy(t4.data(), d.angvel, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.angvel.data());
-    HIP_TRY(hipMemcpy(t4.data(), d.torque, n * 32, hipMemcpyDeviceToHost)); unpad4(t4, n, s.torque.data());
-    HIP_TRY(hipMemcpy(s.es_dot.data(), d.es_dot, n * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(s.es.data(), d.es, n * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(s.ev_dot.data(), d.ev_dot, n * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(s.ev.data(), d.ev, n * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(s.p.data(), d.p, n * 8, hipMemcpyDeviceToHost));
+    int rc;
+    std::vector<uint32_t> id;
+    if ((rc = fetch_ids(h, id))) return rc;
+    std::vector<double> t;
+    if ((rc = fetch4(h, d.b.x, id, t))) return rc;
+    s.x = t;
+    if ((rc = fetch4(h, d.b.vel, id, t))) return rc;
+    s.vel = t;
+    auto unpad = [&](std::vector<double>& dst) {
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) dst[3 * i + a] = t[4 * i + a];
+    };
+    if ((rc = fetch4(h, d.force, id, t))) return rc;
+    unpad(s.force);
+    if ((rc = fetch4(h, d.b.angpos, id, t))) return rc;
+    unpad(s.angpos);
+    if ((rc = fetch4(h, d.b.angvel, id, t))) return rc;
+    unpad(s.angvel);
+    if ((rc = fetch4(h, d.torque, id, t))) return rc;
+    unpad(s.torque);
+    if ((rc = fetch4(h, d.b.acc, id, t))) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        s.xysum[2 * i] = t[4 * i]; s.xysum[2 * i + 1] = t[4 * i + 1];
+        s.es[i] = t[4 * i + 2]; s.ev[i] = t[4 * i + 3];
+    }
+    if ((rc = fetch4(h, d.rates, id, t))) return rc;
+    for (size_t i = 0; i < n; ++i) { s.es_dot[i] = t[4 * i]; s.ev_dot[i] = t[4 * i + 1]; s.p[i] = t[4 * i + 2]; }
     if (s.nw) {
         double wnx[24], wmv[24];
         HIP_TRY(hipMemcpy(wnx, d.wnx, sizeof(wnx), hipMemcpyDeviceToHost));
@@ -577,37 +634,57 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
     if (need == 0) return SPH_OK;
     HIP_TRY(hipStreamSynchronize(h->stream));
     const DemDev& d = h->d;
-    const size_t n = h->hs.np, nc = sph::NC;
+    const size_t n = h->hs.np, nc = sph::NC, K = d.K;
+    double* dd = static_cast<double*>(dst);
+    uint32_t* du = static_cast<uint32_t*>(dst);
     auto copy = [&](const void* src) -> int {
         HIP_TRY(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
         return SPH_OK;
     };
-    auto copy_unpad = [&](const double4* src) -> int {
-        std::vector<double> t4(4 * n);
-        HIP_TRY(hipMemcpy(t4.data(), src, n * 32, hipMemcpyDeviceToHost));
-        unpad4(t4, n, static_cast<double*>(dst));
+    int rc;
+    std::vector<uint32_t> id;
+    std::vector<double> t;
+    // per-particle double4 record in storage order -> comps [c0, c0+nc_) in original order
+    auto gather = [&](const double4* src, int c0, int ncomp) -> int {
+        int r;
+        if ((r = fetch_ids(h, id)) || (r = fetch4(h, src, id, t))) return r;
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < ncomp; ++a) dd[(size_t)ncomp * i + a] = t[4 * i + c0 + a];
         return SPH_OK;
     };
     switch (f) {
-        case SPH_X: return copy(d.x);
-        case SPH_VEL: return copy(d.vel);
-        case SPH_X_SORTED: return copy(d.xs);
-        case SPH_ANGVEL: return copy_unpad(d.angvel);
-        case SPH_FORCE: return copy_unpad(d.force);
-        case SPH_TORQUE: return copy_unpad(d.torque);
-        case SPH_ANGPOS: return copy_unpad(d.angpos);
-        case SPH_VEL0: return copy_unpad(d.vel0);
-        case SPH_ANGVEL0: return copy_unpad(d.angvel0);
-        case SPH_XYSUM: return copy(d.xysum);
-        case SPH_ES_DOT: return copy(d.es_dot);
-        case SPH_ES: return copy(d.es);
-        case SPH_EV_DOT: return copy(d.ev_dot);
-        case SPH_EV: return copy(d.ev);
-        case SPH_P: return copy(d.p);
-        case SPH_CELL_KEYS: return copy(d.key);
-        case SPH_SORT_IDX: return copy(d.idx);
-        case SPH_CELL_START: return copy(d.cellStart);
-        case SPH_CELL_END: return copy(d.cellEnd);
+        case SPH_X: return gather(d.b.x, 0, 4);
+        case SPH_VEL: return gather(d.b.vel, 0, 4);
+        case SPH_X_SORTED: return copy(d.a.x);
+        case SPH_ANGVEL: return gather(d.b.angvel, 0, 3);
+        case SPH_FORCE: return gather(d.force, 0, 3);
+        case SPH_TORQUE: return gather(d.torque, 0, 3);
+        case SPH_ANGPOS: return gather(d.b.angpos, 0, 3);
+        case SPH_VEL0: return gather(d.b.vel0, 0, 3);
+        case SPH_ANGVEL0: return gather(d.b.angvel0, 0, 3);
+        case SPH_XYSUM: return gather(d.b.acc, 0, 2);
+        case SPH_ES: return gather(d.b.acc, 2, 1);
+        case SPH_EV: return gather(d.b.acc, 3, 1);
+        case SPH_ES_DOT: return gather(d.rates, 0, 1);
+        case SPH_EV_DOT: return gather(d.rates, 1, 1);
+        case SPH_P: return gather(d.rates, 2, 1);
+        case SPH_SORT_IDX: return copy(d.a.id);
+        case SPH_CELL_KEYS: case SPH_CELL_START: case SPH_CELL_END: {
+            std::vector<uint32_t> off(K + 1);
+            HIP_TRY(hipMemcpy(off.data(), d.off, (K + 1) * 4, hipMemcpyDeviceToHost));
+            if (f == SPH_CELL_KEYS) {
+                std::vector<uint32_t> sid(n);
+                HIP_TRY(hipMemcpy(sid.data(), d.a.id, n * 4, hipMemcpyDeviceToHost));
+                for (size_t c = 0; c < K; ++c)
+                    for (uint32_t q = off[c]; q < off[c + 1]; ++q) du[sid[q]] = (uint32_t)c;
+                return SPH_OK;
+            }
+            for (size_t c = 0; c < K; ++c) {
+                const bool occ = off[c + 1] > off[c];
+                du[c] = occ ? (f == SPH_CELL_START ? off[c] : off[c + 1]) : sph::EMPTY_CELL;
+            }
+            return SPH_OK;
+        }
         case SPH_BONDS: return copy(d.bonds);
         case SPH_BONDS_DELTA: return copy(d.bdelta);
         case SPH_BONDS_OMEGA: return copy(d.bomega);
@@ -616,29 +693,30 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_WALLS_VEL0: return copy(d.wvel0);
         case SPH_WALL_PP: return copy(d.wall_pp);
         case SPH_TIME: {
-            const double t[4] = {h->hs.dt, h->hs.t_cur, h->hs.t_total, h->hs.file_dt};
-            std::memcpy(dst, t, 32);
+            const double tm[4] = {h->hs.dt, h->hs.t_cur, h->hs.t_total, h->hs.file_dt};
+            std::memcpy(dst, tm, 32);
             return SPH_OK;
         }
         case SPH_CONTACTS: case SPH_DISTMOD: case SPH_DELTA_T: {
-            std::vector<uint32_t> mask(n), slot(n * nc);
-            HIP_TRY(hipMemcpy(mask.data(), d.cmask, n * 4, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> smask(n), mask(n), slot(n * nc);
+            if ((rc = fetch_ids(h, id))) return rc;
+            HIP_TRY(hipMemcpy(smask.data(), d.b.cmask, n * 4, hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < n; ++q) mask[id[q]] = smask[q];
             HIP_TRY(hipMemcpy(slot.data(), d.cslot, n * nc * 4, hipMemcpyDeviceToHost));
+            auto occ = [&](size_t i, size_t k) { return ((mask[i] >> k) & 1u) != 0; };
             if (f == SPH_CONTACTS) {
-                uint32_t* out = static_cast<uint32_t*>(dst);
                 for (size_t i = 0; i < n; ++i)
                     for (size_t k = 0; k < nc; ++k)
-                        out[i * nc + k] = (mask[i] >> k) & 1u ? (slot[i * nc + k] & sph::SLOT_IDX_MASK) : (uint32_t)n;
+                        du[i * nc + k] = occ(i, k) ? (slot[i * nc + k] & sph::SLOT_IDX_MASK) : (uint32_t)n;
                 return SPH_OK;
             }
             if (f == SPH_DISTMOD) {
-                std::vector<double> x(4 * n);
-                HIP_TRY(hipMemcpy(x.data(), d.x, n * 32, hipMemcpyDeviceToHost));
-                double* out = static_cast<double*>(dst);
+                std::vector<double> x;
+                if ((rc = fetch4(h, d.b.x, id, x))) return rc;
                 for (size_t i = 0; i < n; ++i)
                     for (size_t k = 0; k < nc; ++k) {
-                        double* o4 = out + 4 * (i * nc + k);
-                        if (!((mask[i] >> k) & 1u)) { o4[0] = o4[1] = o4[2] = o4[3] = 0.0; continue; }
+                        double* o4 = dd + 4 * (i * nc + k);
+                        if (!occ(i, k)) { o4[0] = o4[1] = o4[2] = o4[3] = 0.0; continue; }
                         const uint32_t w = slot[i * nc + k], code = w >> 28, b = w & sph::SLOT_IDX_MASK;
                         const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
                         o4[0] = cx == 1u ? h->hs.L[0] : (cx == 2u ? -h->hs.L[0] : 0.0);
@@ -648,17 +726,15 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
                     }
                 return SPH_OK;
             }
-            double* out = static_cast<double*>(dst);
             if (h->precision == SPH_PREC_MIXED && d.cdt32) {
-                std::vector<float> t(n * nc * 3);
-                HIP_TRY(hipMemcpy(t.data(), d.cdt32, t.size() * 4, hipMemcpyDeviceToHost));
+                std::vector<float> tf(n * nc * 3);
+                HIP_TRY(hipMemcpy(tf.data(), d.cdt32, tf.size() * 4, hipMemcpyDeviceToHost));
                 for (size_t i = 0; i < n * nc; ++i)
-                    for (int a = 0; a < 3; ++a)
-                        out[3 * i + a] = (mask[i / nc] >> (i % nc)) & 1u ? (double)t[3 * i + a] : 0.0;
+                    for (int a = 0; a < 3; ++a) dd[3 * i + a] = occ(i / nc, i % nc) ? (double)tf[3 * i + a] : 0.0;
             } else {
-                HIP_TRY(hipMemcpy(out, d.cdt, n * nc * 24, hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(dd, d.cdt, n * nc * 24, hipMemcpyDeviceToHost));
                 for (size_t i = 0; i < n * nc; ++i)
-                    if (!((mask[i / nc] >> (i % nc)) & 1u)) out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = 0.0;
+                    if (!occ(i / nc, i % nc)) dd[3 * i] = dd[3 * i + 1] = dd[3 * i + 2] = 0.0;
             }
             return SPH_OK;
         }
@@ -678,34 +754,38 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     DemDev& d = h->d;
     const size_t n = h->hs.np;
+    const double* sd = static_cast<const double*>(src);
     auto copy = [&](void* dptr) -> int {
         HIP_TRY(hipMemcpy(dptr, src, need, hipMemcpyHostToDevice));
         return SPH_OK;
     };
-    auto copy_pad = [&](double4* dptr, bool keep_w) -> int {
-        std::vector<double> t4(4 * n);
-        if (keep_w) HIP_TRY(hipMemcpy(t4.data(), dptr, n * 32, hipMemcpyDeviceToHost));
-        const double* s3 = static_cast<const double*>(src);
-        for (size_t i = 0; i < n; ++i)
-            for (int a = 0; a < 3; ++a) t4[4 * i + a] = s3[3 * i + a];
-        HIP_TRY(hipMemcpy(dptr, t4.data(), n * 32, hipMemcpyHostToDevice));
+    // original-order input -> comps [c0, c0+ncomp) of a storage-order double4 record
+    auto scatter = [&](double4* dptr, int c0, int ncomp) -> int {
+        std::vector<uint32_t> id;
+        int r;
+        if ((r = fetch_ids(h, id))) return r;
+        std::vector<double> t(4 * n);
+        HIP_TRY(hipMemcpy(t.data(), dptr, n * 32, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q)
+            for (int a = 0; a < ncomp; ++a) t[4 * q + c0 + a] = sd[(size_t)ncomp * id[q] + a];
+        HIP_TRY(hipMemcpy(dptr, t.data(), n * 32, hipMemcpyHostToDevice));
         return SPH_OK;
     };
     switch (f) {
-        case SPH_X: return copy(d.x);
-        case SPH_VEL: return copy(d.vel);
-        case SPH_ANGVEL: return copy_pad(d.angvel, false);
-        case SPH_FORCE: return copy_pad(d.force, false);
-        case SPH_TORQUE: return copy_pad(d.torque, false);
-        case SPH_ANGPOS: return copy_pad(d.angpos, false);
-        case SPH_VEL0: return copy_pad(d.vel0, true);
-        case SPH_ANGVEL0: return copy_pad(d.angvel0, false);
-        case SPH_XYSUM: return copy(d.xysum);
-        case SPH_ES_DOT: return copy(d.es_dot);
-        case SPH_ES: return copy(d.es);
-        case SPH_EV_DOT: return copy(d.ev_dot);
-        case SPH_EV: return copy(d.ev);
-        case SPH_P: return copy(d.p);
+        case SPH_X: return scatter(d.b.x, 0, 4);
+        case SPH_VEL: return scatter(d.b.vel, 0, 4);
+        case SPH_ANGVEL: return scatter(d.b.angvel, 0, 3);
+        case SPH_FORCE: return scatter(d.force, 0, 3);
+        case SPH_TORQUE: return scatter(d.torque, 0, 3);
+        case SPH_ANGPOS: return scatter(d.b.angpos, 0, 3);
+        case SPH_VEL0: return scatter(d.b.vel0, 0, 3);
+        case SPH_ANGVEL0: return scatter(d.b.angvel0, 0, 3);
+        case SPH_XYSUM: return scatter(d.b.acc, 0, 2);
+        case SPH_ES: return scatter(d.b.acc, 2, 1);
+        case SPH_EV: return scatter(d.b.acc, 3, 1);
+        case SPH_ES_DOT: return scatter(d.rates, 0, 1);
+        case SPH_EV_DOT: return scatter(d.rates, 1, 1);
+        case SPH_P: return scatter(d.rates, 2, 1);
         case SPH_BONDS: return copy(d.bonds);
         case SPH_BONDS_DELTA: return copy(d.bdelta);
         case SPH_BONDS_OMEGA: return copy(d.bomega);
@@ -713,9 +793,9 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
         case SPH_WALLS_MVFD: return copy(d.wmvfd);
         case SPH_WALLS_VEL0: return copy(d.wvel0);
         case SPH_TIME: {
-            double t[4];
-            std::memcpy(t, src, 32);
-            h->hs.dt = t[0]; h->hs.t_cur = t[1]; h->hs.t_total = t[2]; h->hs.file_dt = t[3];
+            double tm[4];
+            std::memcpy(tm, src, 32);
+            h->hs.dt = tm[0]; h->hs.t_cur = tm[1]; h->hs.t_total = tm[2]; h->hs.file_dt = tm[3];
             HIP_TRY(hipMemcpy(d.t_cur, &h->hs.t_cur, 8, hipMemcpyHostToDevice));
             if (d.dt != h->hs.dt) { d.dt = h->hs.dt; destroy_graph(h); }
             return SPH_OK;

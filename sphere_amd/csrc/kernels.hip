@@ -1,18 +1,18 @@
 // kernels.hip -- gfx950 kernels of one DEM time step.
 //
-// Step order (one HIP stream, optionally replayed as a HIP graph):
+// Step order (one HIP stream, replayed as a HIP graph):
 //   bonds      one thread per bond: parallel-bond force, torque, breakage
 //              (src/cohesion.cuh:8-233); forces go to per-bond records, no
 //              scatter
 //   cellid     one thread per particle: periodic wrap, cell id, atomic
 //              count (src/sorting.cuh:7-43)
-//   scan       exclusive scan of the per-cell counts -> cellStart/cellEnd
-//   scatter    counting-sort placement of original indices
+//   scan       exclusive scan of the per-cell counts -> cell offsets
+//   scatter    counting-sort placement of the particles
 //   cellsort   one thread per cell: order each cell's members by original
-//              index, which makes the counting sort stable (the reference's
-//              sort_by_key permutation, src/device.cu:763-765)
-//   reorder    gather x, vel, angvel into sorted copies + inverse
-//              permutation (src/sorting.cuh:49-121)
+//              index, which makes the counting sort the stable sort of
+//              src/device.cu:763-765
+//   reorder    gather every per-particle record into the new sorted order
+//              (src/sorting.cuh:49-121 gathers x, vel, angvel only)
 //   contact    one thread per sorted particle: 27-cell search and slot
 //              update (src/contactsearch.cuh:181-355), contact and wall
 //              forces (:369-689, src/contactmodels.cuh:10-109,283-493),
@@ -100,8 +100,8 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint64_t n, uin
 __global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
-    double4 x = d.x[i];
-    if (wrap_xy(d, x.x, x.y)) d.x[i] = x;
+    double4 x = d.b.x[i];
+    if (wrap_xy(d, x.x, x.y)) d.b.x[i] = x;
     uint32_t err = 0;
     const int ix = cell_coord(d, x.x, 0, err);
     const int iy = cell_coord(d, x.y, 1, err);
@@ -161,6 +161,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(DemDev d) {
     }
 }
 
+// off[c] = first sorted position of cell c, off[K] = np
 __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
     __shared__ uint32_t lds[BLOCK / 64];
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
@@ -175,70 +176,77 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
     uint32_t off = block_exclusive_scan(sum, lds, total) + d.scan_bsum[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < SCAN_PER_THREAD; ++j) {
-        if (base + j < d.K) {
-            d.cellStart[base + j] = c[j] ? off : EMPTY_CELL;
-            d.cellEnd[base + j] = c[j] ? off + c[j] : EMPTY_CELL;
-        }
+        if (base + j < d.K) d.off[base + j] = off;
         off += c[j];
+        if (base + j + 1 == d.K) d.off[d.K] = off;
     }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scatter(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
-    d.idx[d.cellStart[d.key[i]] + d.rank[i]] = i;
+    d.perm[d.off[d.key[i]] + d.rank[i]] = i;
 }
 
-// Sort one cell's original indices ascending: odd-even transposition network
-// in registers for up to W members, insertion sort in memory beyond that.
+// Order one cell's members by original index: odd-even transposition network
+// on (id, position) pairs in registers for up to W members, insertion sort in
+// memory beyond that.
 template <int W>
-__device__ __forceinline__ void sort_small(uint32_t* p, uint32_t n) {
-    uint32_t v[W];
+__device__ __forceinline__ void sort_small(const DemDev& d, uint32_t* p, uint32_t n) {
+    uint64_t v[W];
 #pragma unroll
-    for (int j = 0; j < W; ++j) v[j] = (uint32_t)j < n ? p[j] : 0xffffffffu;
+    for (int j = 0; j < W; ++j)
+        v[j] = (uint32_t)j < n ? ((uint64_t)d.b.id[p[j]] << 32) | p[j] : ~0ull;
 #pragma unroll
     for (int r = 0; r < W; ++r) {
 #pragma unroll
         for (int j = (r & 1); j + 1 < W; j += 2) {
-            const uint32_t a = v[j], b = v[j + 1];
+            const uint64_t a = v[j], b = v[j + 1];
             v[j] = a < b ? a : b;
             v[j + 1] = a < b ? b : a;
         }
     }
 #pragma unroll
     for (int j = 0; j < W; ++j)
-        if ((uint32_t)j < n) p[j] = v[j];
+        if ((uint32_t)j < n) p[j] = (uint32_t)v[j];
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cellsort(DemDev d) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.K) return;
-    const uint32_t st = d.cellStart[c];
-    if (st == EMPTY_CELL) return;
-    const uint32_t n = d.cellEnd[c] - st;
+    const uint32_t st = d.off[c];
+    const uint32_t n = d.off[c + 1] - st;
     if (n < 2) return;
-    uint32_t* p = d.idx + st;
-    if (n <= 4) sort_small<4>(p, n);
-    else if (n <= 8) sort_small<8>(p, n);
-    else if (n <= 16) sort_small<16>(p, n);
+    uint32_t* p = d.perm + st;
+    if (n <= 4) sort_small<4>(d, p, n);
+    else if (n <= 8) sort_small<8>(d, p, n);
+    else if (n <= 16) sort_small<16>(d, p, n);
     else {
         for (uint32_t i = 1; i < n; ++i) {
             const uint32_t v = p[i];
+            const uint32_t key = d.b.id[v];
             uint32_t j = i;
-            while (j > 0 && p[j - 1] > v) { p[j] = p[j - 1]; --j; }
+            while (j > 0 && d.b.id[p[j - 1]] > key) { p[j] = p[j - 1]; --j; }
             p[j] = v;
         }
     }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
-    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= d.np) return;
-    const uint32_t o = d.idx[s];
-    d.xs[s] = d.x[o];
-    d.vs[s] = d.vel[o];
-    d.ws[s] = d.angvel[o];
-    d.inv[o] = s;
+    const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
+    if (q >= d.np) return;
+    const uint32_t p = d.perm[q];
+    d.a.x[q] = d.b.x[p];
+    d.a.vel[q] = d.b.vel[p];
+    d.a.angvel[q] = d.b.angvel[p];
+    d.a.vel0[q] = d.b.vel0[p];
+    d.a.angvel0[q] = d.b.angvel0[p];
+    d.a.angpos[q] = d.b.angpos[p];
+    d.a.acc[q] = d.b.acc[p];
+    d.a.cmask[q] = d.b.cmask[p];
+    const uint32_t id = d.b.id[p];
+    d.a.id[q] = id;
+    d.pos_of[id] = q;
 }
 
 // ------------------------------------------------------------------- bonds
@@ -256,9 +264,10 @@ __global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
     const double delta0_n = d0.w;
     V3<double> omega0_t = mk(o0.x, o0.y, o0.z);
     const double omega0_n = o0.w;
-    const double4 x_i = d.x[bond.x], x_j = d.x[bond.y];
-    const double4 vel_i = d.vel[bond.x], vel_j = d.vel[bond.y];
-    const V3<double> angvel_i = xyz(d.angvel[bond.x]), angvel_j = xyz(d.angvel[bond.y]);
+    const uint32_t pi = d.pos_of[bond.x], pj = d.pos_of[bond.y];
+    const double4 x_i = d.b.x[pi], x_j = d.b.x[pj];
+    const double4 vel_i = d.b.vel[pi], vel_j = d.b.vel[pj];
+    const V3<double> angvel_i = xyz(d.b.angvel[pi]), angvel_j = xyz(d.b.angvel[pj]);
     const double R_bar = d.lambda_bar * fmin(x_i.w, x_j.w);
     const double A = PI_REF * R_bar * R_bar;
     const double I = 0.25 * PI_REF * R_bar * R_bar * R_bar * R_bar;
@@ -413,13 +422,13 @@ template <class T>
 __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
-    const uint32_t o = d.idx[s];
-    const double4 xa = d.xs[s];
+    const uint32_t o = d.a.id[s];
+    const double4 xa = d.a.x[s];
     const double ra = xa.w;
     uint32_t err = 0;
 
     // ---- topology: 27 cells in z, y, x order (src/contactsearch.cuh:340-353)
-    uint32_t mask = d.cmask[o];
+    uint32_t mask = d.a.cmask[s];
     uint32_t fresh = 0;  // slots taken from empty this step: their delta_t starts at 0
     uint32_t* slot = d.cslot + (size_t)o * NC;
     const int gx = cell_coord(d, xa.x, 0, err), gy = cell_coord(d, xa.y, 1, err), gz = cell_coord(d, xa.z, 2, err);
@@ -449,12 +458,10 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
                 double dmx, dmy;
                 decode_image(d, code, dmx, dmy);
                 const uint32_t cid = (uint32_t)tx + (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
-                const uint32_t st = d.cellStart[cid];
-                if (st == EMPTY_CELL) continue;
-                const uint32_t en = d.cellEnd[cid];
+                const uint32_t st = d.off[cid], en = d.off[cid + 1];
                 for (uint32_t sb = st; sb < en; ++sb) {
                     if (sb == s) continue;
-                    const double4 xb = d.xs[sb];
+                    const double4 xb = d.a.x[sb];
                     const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
                     const double rsum = ra + xb.w;
                     const double d2 = dot(xab, xab);
@@ -463,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
                     if (d2 > rsum * rsum * 1.0000001) continue;
                     const double delta = sqrt(d2) - rsum;
                     if (delta < 0.0) {
-                        const uint32_t bo = d.idx[sb];
+                        const uint32_t bo = d.a.id[sb];
                         // src/contactsearch.cuh:252-265: the slot already holding b
                         // (lowest), else the highest empty slot
                         int cpos = -1;
@@ -487,8 +494,8 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
     }
 
     // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
-    const double4 va4 = d.vs[s];
-    const V3<double> va = xyz(va4), wa = xyz(d.ws[s]);
+    const double4 va4 = d.a.vel[s];
+    const V3<double> va = xyz(va4), wa = xyz(d.a.angvel[s]);
     Acc acc;
     acc.F = mk(0.0, 0.0, 0.0);
     acc.T = mk(0.0, 0.0, 0.0);
@@ -497,8 +504,8 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
         const int i = __ffs(m) - 1;
         const uint32_t w = slot[i];
         const uint32_t bo = w & SLOT_IDX_MASK;
-        const uint32_t sb = d.inv[bo];
-        const double4 xb = d.xs[sb];
+        const uint32_t sb = d.pos_of[bo];
+        const double4 xb = d.a.x[sb];
         double dmx, dmy;
         decode_image(d, w >> 28, dmx, dmy);
         const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
@@ -507,14 +514,13 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
         if (delta_n < 0.0) {
             const size_t mp = (size_t)o * NC + i;
             V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
-            contact_linear<T>(d, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.vs[sb])), cvt<T>(xyz(d.ws[sb])),
+            contact_linear<T>(d, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.a.vel[sb])), cvt<T>(xyz(d.a.angvel[sb])),
                               (T)ra, (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
             SlotStore<T>::store(d, mp, dt0);
         } else {
             mask &= ~(1u << i);  // contact ended: the slot is empty again
         }
     }
-    d.cmask[o] = mask;
 
     // ---- walls (src/contactsearch.cuh:587-663)
     double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
@@ -544,11 +550,10 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
 
     // ---- per-particle outputs (src/contactsearch.cuh:669-687)
     const double dt = d.dt;
-    d.es_dot[o] = acc.es_dot;
-    d.ev_dot[o] = acc.ev_dot;
-    d.es[o] += acc.es_dot * dt;
-    d.ev[o] += acc.ev_dot * dt;
-    d.p[o] = acc.p;
+    double4 ea = d.a.acc[s];  // xysum.x, xysum.y, es, ev
+    ea.z += acc.es_dot * dt;
+    ea.w += acc.ev_dot * dt;
+    d.rates[s] = make_double4(acc.es_dot, acc.ev_dot, acc.p, 0.0);
     if (d.nw > 0) d.wall_pp[o] = wf0;
     if (d.nw > 1) d.wall_pp[(size_t)d.np + o] = wf1;
     if (d.nw > 2) d.wall_pp[2 * (size_t)d.np + o] = wf2;
@@ -568,8 +573,8 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
             else { force = force - f; torque = torque + t; }
         }
     }
-    d.force[o] = make_double4(force.x, force.y, force.z, 0.0);
-    d.torque[o] = make_double4(torque.x, torque.y, torque.z, 0.0);
+    d.force[s] = make_double4(force.x, force.y, force.z, 0.0);
+    d.torque[s] = make_double4(torque.x, torque.y, torque.z, 0.0);
 
     // ---- leapfrog (src/integration.cuh:20-185)
     const double radius = ra;
@@ -585,7 +590,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
     }
     double px = xa.x, py = xa.y, pz = xa.z;
     (void)wrap_xy(d, px, py);
-    double4 v0 = d.vel0[o], w0 = d.angvel0[o], ap = d.angpos[o];
+    double4 v0 = d.a.vel0[s], w0 = d.a.angvel0[s], ap = d.a.angpos[s];
     v0.x += ax * dt; v0.y += ay * dt; v0.z += az * dt;
     w0.x += bx * dt; w0.y += by * dt; w0.z += bz * dt;
     px += v0.x * dt; py += v0.y * dt; pz += v0.z * dt;
@@ -595,15 +600,17 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
     double sx = 0.0, sy = 0.0;
     sx += vx * dt;
     sy += vy * dt;
-    double2 xy = d.xysum[o];
-    xy.x += sx; xy.y += sy;
-    d.xysum[o] = xy;
-    d.angvel[o] = make_double4(wx, wy, wz, 0.0);
-    d.angvel0[o] = w0;
-    d.vel[o] = make_double4(vx, vy, vz, va4.w);
-    d.vel0[o] = v0;
-    d.angpos[o] = ap;
-    d.x[o] = make_double4(px, py, pz, radius);
+    ea.x += sx;
+    ea.y += sy;
+    d.b.acc[s] = ea;
+    d.b.angvel[s] = make_double4(wx, wy, wz, 0.0);
+    d.b.angvel0[s] = w0;
+    d.b.vel[s] = make_double4(vx, vy, vz, va4.w);
+    d.b.vel0[s] = v0;
+    d.b.angpos[s] = ap;
+    d.b.x[s] = make_double4(px, py, pz, radius);
+    d.b.id[s] = o;
+    d.b.cmask[s] = mask;
     if (err) atomicOr(d.flags, err);
 }
 
