@@ -102,6 +102,8 @@ typedef struct sph_timing {
     double bonds_ms, cellid_ms, sort_ms, reorder_ms, contact_ms, walls_ms;
     double total_ms;
     uint64_t steps;
+    double search_ms, force_ms;  /* contact_ms = search_ms + force_ms */
+    uint32_t tile_x, tile_y;     /* cells per contact-search block */
 } sph_timing_t;
 
 /* DEM::DEM + readbin + checkValues + allocateGlobalDeviceMemory +
@@ -164,6 +166,15 @@ int sph_get_timing(const sph_t* h, sph_timing_t* out);
 
 /* 1 = replay the captured step as a HIP graph (default), 0 = plain launches. */
 int sph_set_graph(sph_t* h, int on);
+
+/* Cells per contact-search block (tile_x along x, tile_y along y); 0, 0
+ * restores the automatic choice made at open from the cell occupancy. */
+int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y);
+
+/* Contact-search kernel: 1 = one block per tile with the neighbourhood staged
+ * in LDS, 2 = one thread per sorted particle reading through L1/L2, 0 = the
+ * default (2). */
+int sph_set_search(sph_t* h, int mode);
 
 /* Setup helper (host only): seeded non-overlapping random sequential packing
  * of n spheres with the given radii in box[3] (periodic: 0 none, 1 x+y, 2 x),

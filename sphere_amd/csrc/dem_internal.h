@@ -39,6 +39,8 @@ constexpr uint32_t MAX_PARTICLES = SLOT_IDX_MASK;  // indices must fit 28 bits
 constexpr int BLOCK = 256;
 constexpr int SCAN_PER_THREAD = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_PER_THREAD;
+constexpr int TOPO_CAP = 1024;          // staged neighbour particles per search block
+constexpr int TOPO_MAX_CELLS = 384;     // staged cells per search block
 
 // device error flags (atomicOr'd by kernels, read after each sph_step chunk)
 enum : uint32_t {
@@ -90,6 +92,13 @@ struct DemDev {
     uint32_t *key, *rank, *perm, *count, *off, *scan_bsum;  // off: K+1 prefix
     uint32_t scan_blocks;
 
+    // contact search tiles: TX x TY cells of one z plane per block
+    uint32_t tile_x, tile_y, tiles_x, tiles_y;
+    int search_mode;    // 1: LDS tiles, 2: one thread per sorted particle, HBM/L2 reads
+    uint32_t* skey;     // cell id per sorted position
+    uint32_t* tmask;    // slot occupancy after the search, sorted order
+    uint32_t* tfresh;   // slots taken from empty by the search, sorted order
+
     // walls (6 entries; entries >= nw hold the reference's defaults)
     double4 *wnx, *wmvfd;
     double* wvel0;
@@ -115,7 +124,9 @@ void launch_scan(const DemDev& d, hipStream_t s);
 void launch_scatter(const DemDev& d, hipStream_t s);
 void launch_cellsort(const DemDev& d, hipStream_t s);
 void launch_reorder(const DemDev& d, hipStream_t s);
-void launch_contact(const DemDev& d, int precision, hipStream_t s);
+void launch_contact(const DemDev& d, int precision, hipStream_t s);  // search + forces
+void launch_topology(const DemDev& d, hipStream_t s);
+void launch_forces(const DemDev& d, int precision, hipStream_t s);
 void launch_walls(const DemDev& d, hipStream_t s);
 
 }  // namespace sph

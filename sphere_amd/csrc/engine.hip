@@ -117,6 +117,58 @@ void fill_params(sph_dem* h) {
     d.scan_blocks = (d.K + sph::SCAN_TILE - 1) / sph::SCAN_TILE;
 }
 
+// Contact-search tile: tile_x x tile_y cells of one z plane per block.  Aim
+// for about one block of home particles with the staged neighbourhood
+// (3 (tile_y+2)(tile_x+2) cells) inside the LDS capacity, using the mean
+// occupancy of the non-empty cells.
+void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
+    const HostState& s = h->hs;
+    DemDev& d = h->d;
+    const uint32_t nx = s.num[0], ny = s.num[1];
+    uint32_t bx = 1, by = 1;
+    if (want_x && want_y) {
+        bx = want_x; by = want_y;
+    } else {
+        std::vector<uint32_t> cnt(d.K, 0);
+        uint64_t nonempty = 0;
+        for (uint32_t i = 0; i < s.np; ++i) {
+            uint32_t c[3];
+            for (int a = 0; a < 3; ++a) {
+                double f = std::floor((s.x[4 * (size_t)i + a] - s.origo[a]) / d.cell_len[a]);
+                if (!(f >= 0.0)) f = 0.0;
+                if (f > (double)(s.num[a] - 1)) f = (double)(s.num[a] - 1);
+                c[a] = (uint32_t)f;
+            }
+            uint32_t& k = cnt[(c[2] * s.num[1] + c[1]) * s.num[0] + c[0]];
+            nonempty += k == 0;
+            ++k;
+        }
+        const double lam = nonempty ? (double)s.np / (double)nonempty : 1.0;
+        double best = -1.0;
+        for (uint32_t ty = 1; ty <= ny && ty <= 16; ++ty)
+            for (uint32_t tx = 1; tx <= nx && tx <= 64; ++tx) {
+                const double cells = 3.0 * (tx + 2) * (ty + 2);
+                if (cells > sph::TOPO_MAX_CELLS) continue;
+                const double staged = cells * lam, home = tx * ty * lam;
+                if (staged > 0.8 * sph::TOPO_CAP) continue;
+                const double passes = std::ceil(home / sph::BLOCK);
+                const double util = home / (passes * sph::BLOCK);
+                const double score = util * home / staged;
+                if (score > best) { best = score; bx = tx; by = ty; }
+            }
+    }
+    bx = bx < 1 ? 1 : (bx > nx ? nx : bx);
+    by = by < 1 ? 1 : (by > ny ? ny : by);
+    while (3u * (bx + 2) * (by + 2) > (uint32_t)sph::TOPO_MAX_CELLS) {
+        if (bx > by) --bx; else --by;
+    }
+    d.tile_x = bx;
+    d.tile_y = by;
+    if (d.search_mode == 0) d.search_mode = 2;
+    d.tiles_x = (nx + bx - 1) / bx;
+    d.tiles_y = (ny + by - 1) / by;
+}
+
 int validate_for_device(sph_dem* h) {
     const HostState& s = h->hs;
     if (s.contactmodel != 2) {
@@ -299,6 +351,7 @@ int allocate(sph_dem* h) {
     ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
     d.cdt32 = nullptr;
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
+    ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
     ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
@@ -336,10 +389,12 @@ void enqueue_step(sph_dem* h, bool timed) {
     if (timed) (void)hipEventRecord(h->ev[3], s);
     sph::launch_reorder(d, s);
     if (timed) (void)hipEventRecord(h->ev[4], s);
-    sph::launch_contact(d, h->precision, s);
+    sph::launch_topology(d, s);
     if (timed) (void)hipEventRecord(h->ev[5], s);
-    sph::launch_walls(d, s);
+    sph::launch_forces(d, h->precision, s);
     if (timed) (void)hipEventRecord(h->ev[6], s);
+    sph::launch_walls(d, s);
+    if (timed) (void)hipEventRecord(h->ev[7], s);
 }
 
 int ensure_graph(sph_dem* h) {
@@ -440,6 +495,7 @@ int open_common(sph_dem* h, int verbosity, int check_vals) {
     if (check_vals == 1 && (rc = sph::check_values(h->hs, h->err))) return rc;
     if ((rc = validate_for_device(h))) return rc;
     fill_params(h);
+    choose_tile(h);
     HIP_TRY(hipGetDevice(&h->device));
     HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     if ((rc = allocate(h))) return rc;
@@ -480,16 +536,19 @@ int run_steps(sph_dem* h, uint64_t n) {
         sph_timing_t t{};
         for (uint64_t i = 0; i < n; ++i) {
             enqueue_step(h, true);
-            HIP_TRY(hipEventSynchronize(h->ev[6]));
-            float ms[6];
-            for (int k = 0; k < 6; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], h->ev[k], h->ev[k + 1]));
+            HIP_TRY(hipEventSynchronize(h->ev[7]));
+            float ms[7];
+            for (int k = 0; k < 7; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], h->ev[k], h->ev[k + 1]));
             t.bonds_ms += ms[0]; t.cellid_ms += ms[1]; t.sort_ms += ms[2]; t.reorder_ms += ms[3];
-            t.contact_ms += ms[4]; t.walls_ms += ms[5];
+            t.search_ms += ms[4]; t.force_ms += ms[5]; t.walls_ms += ms[6];
             h->hs.t_cur += h->hs.dt;
             ++h->steps_done;
         }
+        t.contact_ms = t.search_ms + t.force_ms;
         t.total_ms = t.bonds_ms + t.cellid_ms + t.sort_ms + t.reorder_ms + t.contact_ms + t.walls_ms;
         t.steps = n;
+        t.tile_x = h->d.tile_x;
+        t.tile_y = h->d.tile_y;
         h->timing = t;
         return check_flags(h);
     }
@@ -868,6 +927,20 @@ int sph_set_profiling(sph_t* h, int on) {
 int sph_get_timing(const sph_t* h, sph_timing_t* out) {
     if (!h || !out) return SPH_E_INVALID;
     *out = h->timing;
+    return SPH_OK;
+}
+
+int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y) {
+    if (!h) return SPH_E_INVALID;
+    choose_tile(h, tile_x, tile_y);
+    destroy_graph(h);
+    return SPH_OK;
+}
+
+int sph_set_search(sph_t* h, int mode) {
+    if (!h || mode < 0 || mode > 2) return SPH_E_INVALID;
+    h->d.search_mode = mode == 0 ? 2 : mode;
+    destroy_graph(h);
     return SPH_OK;
 }
 

@@ -247,6 +247,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     const uint32_t id = d.b.id[p];
     d.a.id[q] = id;
     d.pos_of[id] = q;
+    d.skey[q] = d.key[p];
 }
 
 // ------------------------------------------------------------------- bonds
@@ -417,28 +418,211 @@ template <> struct SlotStore<float> {
     }
 };
 
-// ----------------------------------------------------------- contact kernel
-template <class T>
-__global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
+// ----------------------------------------------------------- contact search
+// One block per tile of tile_x x tile_y cells in one z plane.  The positions
+// of every particle in the 3 x (tile_y+2) x (tile_x+2) cells around the tile
+// are staged once into LDS (each staged cell is a contiguous run of the
+// sorted arrays); each thread then walks its particle's 27 cells from LDS in
+// the reference's z, y, x order and, within a cell, in sorted order
+// (src/contactsearch.cuh:181-355).  A tile whose neighbourhood exceeds the
+// LDS capacity reads the same candidates from HBM instead.
+struct TopoShared {
+    double4 pos[TOPO_CAP];
+    uint32_t cstart[TOPO_MAX_CELLS + 1];  // LDS offset of each staged cell
+    uint32_t cglob[TOPO_MAX_CELLS];       // first sorted index of each staged cell
+    uint8_t ccode[TOPO_MAX_CELLS];        // periodic image code of each staged cell
+    uint32_t scan[BLOCK / 64];
+};
+
+// Slot bookkeeping of one confirmed overlap (src/contactsearch.cuh:252-284):
+// the slot already holding b (lowest), else the highest empty slot.
+struct SlotState {
+    uint32_t mask, fresh, nc, err;
+    bool stop;
+};
+
+__device__ __noinline__ void slot_insert(uint32_t* slot, SlotState& st, uint32_t bo, uint32_t code) {
+    int cpos = -1;
+    for (uint32_t m = st.mask; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        if ((slot[i] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+    }
+    if (cpos < 0) {
+        const uint32_t empty = ~st.mask;
+        if (empty == 0u) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; return; }
+        cpos = 31 - __clz(empty);
+        st.fresh |= 1u << cpos;
+    }
+    slot[cpos] = bo | (code << 28);
+    st.mask |= 1u << cpos;
+    if (++st.nc > (uint32_t)NC) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; }
+}
+
+template <bool FROM_LDS>
+__device__ __forceinline__ double4 staged_pos(const DemDev& d, const TopoShared& sh, uint32_t lb, uint32_t sb) {
+    if constexpr (FROM_LDS) return sh.pos[lb];
+    else return d.a.x[sb];
+}
+
+// Walks the 27 cells of every home particle of the tile: 9 rows (z, y order),
+// each row the contiguous run of cells x-1, x, x+1, within a cell in sorted
+// order -- the candidate order of src/contactsearch.cuh:340-353.  Candidates
+// come from LDS (FROM_LDS) or from HBM.  The hot loop only rejects pairs whose
+// squared distance exceeds (ra+rb)^2 by more than rounding; the exact test
+// sqrt(d2) - (ra+rb) < 0 runs on the few that remain.
+template <bool FROM_LDS>
+__device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& sh, int W, int H, int nxh,
+                                            int nyh) {
+    uint32_t home_total = 0;
+    for (int hr = 0; hr < nyh; ++hr) {
+        const int e0 = (1 * H + hr + 1) * W + 1;
+        home_total += sh.cstart[e0 + nxh] - sh.cstart[e0];
+    }
+    for (uint32_t h = threadIdx.x; h < home_total; h += BLOCK) {
+        int hr = 0, e0 = 0;
+        uint32_t li = 0, rem = h;
+        for (; hr < nyh; ++hr) {
+            e0 = (1 * H + hr + 1) * W + 1;
+            const uint32_t n_row = sh.cstart[e0 + nxh] - sh.cstart[e0];
+            if (rem < n_row) { li = sh.cstart[e0] + rem; break; }
+            rem -= n_row;
+        }
+        int eh = e0;
+        while (sh.cstart[eh + 1] <= li) ++eh;
+        const int colh = eh - (1 * H + hr + 1) * W;  // 1..nxh
+        const uint32_t s = sh.cglob[eh] + (li - sh.cstart[eh]);
+        const double4 xa = staged_pos<FROM_LDS>(d, sh, li, s);
+        const double ra = xa.w;
+        const uint32_t o = d.a.id[s];
+        uint32_t* slot = d.cslot + (size_t)o * NC;
+        SlotState st;
+        st.mask = d.a.cmask[s];
+        st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false;
+        for (int lay = 0; lay < 3 && !st.stop; ++lay) {
+            for (int dy = 0; dy < 3 && !st.stop; ++dy) {
+                const int e = (lay * H + hr + dy) * W + colh - 1;
+                const uint32_t b0 = sh.cstart[e], b3 = sh.cstart[e + 3];
+                if (b0 == b3) continue;
+                const uint32_t codes = (uint32_t)sh.ccode[e] | ((uint32_t)sh.ccode[e + 1] << 8) |
+                                       ((uint32_t)sh.ccode[e + 2] << 16);
+                for (int c = 0; c < 3 && !st.stop; ++c) {
+                    const uint32_t lo = c == 0 ? b0 : sh.cstart[e + c];
+                    const uint32_t hi = c == 2 ? b3 : sh.cstart[e + c + 1];
+                    if (lo == hi) continue;
+                    const uint32_t code = (codes >> (8 * c)) & 0xffu;
+                    double dmx, dmy;
+                    decode_image(d, code, dmx, dmy);
+                    const uint32_t gb = sh.cglob[e + c] - lo;
+                    for (uint32_t lb = lo; lb < hi; ++lb) {
+                        const double4 xb = staged_pos<FROM_LDS>(d, sh, lb, gb + lb);
+                        const double ex = (xa.x - xb.x) + dmx;
+                        const double ey = (xa.y - xb.y) + dmy;
+                        const double ez = xa.z - xb.z;
+                        // (ez + 0.0)^2 == ez^2, so d2 equals dot(xab, xab) of the exact test
+                        const double d2 = ex * ex + ey * ey + ez * ez;
+                        const double rsum = ra + xb.w;
+                        if (__builtin_expect(d2 <= rsum * rsum * 1.0000001, 0) && lb != li) {
+                            // exact test of src/contactsearch.cuh:242-248 (double sqrt)
+                            if (sqrt(d2) - rsum < 0.0) {
+                                slot_insert(slot, st, d.a.id[gb + lb], code);
+                                if (st.stop) break;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        d.tmask[s] = st.mask;
+        d.tfresh[s] = st.fresh;
+        if (st.err) atomicOr(d.flags, st.err);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
+    __shared__ TopoShared sh;
+    const uint32_t bt = xcd_block();
+    const uint32_t txi = bt % d.tiles_x, rest = bt / d.tiles_x;
+    const uint32_t tyi = rest % d.tiles_y, tz = rest / d.tiles_y;
+    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
+    const int x0 = (int)(txi * d.tile_x), y0 = (int)(tyi * d.tile_y);
+    const int nxh = min((int)d.tile_x, nx - x0), nyh = min((int)d.tile_y, ny - y0);
+    const int W = nxh + 2, H = nyh + 2;
+    const int ncell = 3 * H * W;  // <= TOPO_MAX_CELLS (host-checked)
+
+    // 1. staged cells: global run, image code, count -> LDS offsets
+    uint32_t cnt2[2] = {0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int e = 2 * (int)threadIdx.x + k;
+        if (e >= ncell) continue;
+        const int col = e % W, rr = e / W, row = rr % H, lay = rr / H;
+        int cx = x0 - 1 + col, cy = y0 - 1 + row;
+        const int cz = (int)tz - 1 + lay;
+        uint32_t code = 0, g0 = 0, cnt = 0;
+        bool ok = cz >= 0 && cz < nz;
+        if (ok && (cy < 0 || cy >= ny)) {
+            if (d.periodic == 1) { code |= (cy < 0 ? 1u : 2u) << 2; cy = cy < 0 ? ny - 1 : 0; }
+            else ok = false;
+        }
+        if (ok && (cx < 0 || cx >= nx)) {
+            if (d.periodic != 0) { code |= cx < 0 ? 1u : 2u; cx = cx < 0 ? nx - 1 : 0; }
+            else ok = false;
+        }
+        if (ok) {
+            const uint32_t cid = (uint32_t)cx + (uint32_t)cy * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)cz;
+            g0 = d.off[cid];
+            cnt = d.off[cid + 1] - g0;
+        }
+        sh.cglob[e] = g0;
+        sh.ccode[e] = (uint8_t)code;
+        cnt2[k] = cnt;
+    }
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(cnt2[0] + cnt2[1], sh.scan, total);
+    if (2 * (int)threadIdx.x < ncell) sh.cstart[2 * threadIdx.x] = ex;
+    if (2 * (int)threadIdx.x + 1 < ncell) sh.cstart[2 * threadIdx.x + 1] = ex + cnt2[0];
+    if (threadIdx.x == 0) sh.cstart[ncell] = total;
+    __syncthreads();
+    const bool use_lds = total <= (uint32_t)TOPO_CAP;
+
+    // 2. stage positions
+    if (use_lds) {
+        for (uint32_t i = threadIdx.x; i < total; i += BLOCK) {
+            int lo = 0, hi = ncell - 1;  // last cell with cstart <= i
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sh.cstart[mid] <= i) lo = mid; else hi = mid - 1;
+            }
+            sh.pos[i] = d.a.x[sh.cglob[lo] + (i - sh.cstart[lo])];
+        }
+    }
+    __syncthreads();
+
+    // 3. home particles: cells (lay 1, rows 1..nyh, cols 1..nxh), row by row
+    if (use_lds) topo_search<true>(d, sh, W, H, nxh, nyh);
+    else topo_search<false>(d, sh, W, H, nxh, nyh);
+}
+
+// One thread per sorted particle; the 27 cells are walked as 9 runs of the
+// sorted arrays (cells x-1, x, x+1 of a row have consecutive ids), the same
+// candidate order as k_topo, candidates read through L1/L2.
+__global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
-    const uint32_t o = d.a.id[s];
+    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
+    const uint32_t key = d.skey[s];
+    const int gx = (int)(key % d.num[0]), gy = (int)((key / d.num[0]) % d.num[1]), gz = (int)(key / (d.num[0] * d.num[1]));
     const double4 xa = d.a.x[s];
     const double ra = xa.w;
-    uint32_t err = 0;
-
-    // ---- topology: 27 cells in z, y, x order (src/contactsearch.cuh:340-353)
-    uint32_t mask = d.a.cmask[s];
-    uint32_t fresh = 0;  // slots taken from empty this step: their delta_t starts at 0
+    const uint32_t o = d.a.id[s];
     uint32_t* slot = d.cslot + (size_t)o * NC;
-    const int gx = cell_coord(d, xa.x, 0, err), gy = cell_coord(d, xa.y, 1, err), gz = cell_coord(d, xa.z, 2, err);
-    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
-    uint32_t nc = 0;
-    bool stop = false;
-    for (int dz = -1; dz <= 1 && !stop; ++dz) {
+    SlotState st;
+    st.mask = d.a.cmask[s];
+    st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false;
+    for (int dz = -1; dz <= 1 && !st.stop; ++dz) {
         const int tz = gz + dz;
         if (tz < 0 || tz == nz) continue;
-        for (int dy = -1; dy <= 1 && !stop; ++dy) {
+        for (int dy = -1; dy <= 1 && !st.stop; ++dy) {
             int ty = gy + dy;
             uint32_t cy = 0;
             if (ty < 0 || ty == ny) {
@@ -446,7 +630,26 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
                 cy = ty < 0 ? 1u : 2u;
                 ty = ty < 0 ? ny - 1 : 0;
             }
-            for (int dx = -1; dx <= 1 && !stop; ++dx) {
+            const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+            if (gx >= 1 && gx + 1 < nx && cy == 0) {
+                // no periodic image anywhere in the run: one contiguous range
+                const uint32_t lo = d.off[row + gx - 1], hi = d.off[row + gx + 2];
+                for (uint32_t sb = lo; sb < hi; ++sb) {
+                    const double4 xb = d.a.x[sb];
+                    const double ex = xa.x - xb.x, ey = xa.y - xb.y, ez = xa.z - xb.z;
+                    // ((e + 0.0)^2 == e^2: d2 equals dot(xab, xab) of the exact test)
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const double rsum = ra + xb.w;
+                    if (__builtin_expect(d2 <= rsum * rsum * 1.0000001, 0) && sb != s) {
+                        if (sqrt(d2) - rsum < 0.0) {
+                            slot_insert(slot, st, d.a.id[sb], 0u);
+                            if (st.stop) break;
+                        }
+                    }
+                }
+                continue;
+            }
+            for (int dx = -1; dx <= 1 && !st.stop; ++dx) {
                 int tx = gx + dx;
                 uint32_t cx = 0;
                 if (tx < 0 || tx == nx) {
@@ -457,41 +660,42 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
                 const uint32_t code = cx | (cy << 2);
                 double dmx, dmy;
                 decode_image(d, code, dmx, dmy);
-                const uint32_t cid = (uint32_t)tx + (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
-                const uint32_t st = d.off[cid], en = d.off[cid + 1];
-                for (uint32_t sb = st; sb < en; ++sb) {
-                    if (sb == s) continue;
+                const uint32_t lo = d.off[row + tx], hi = d.off[row + tx + 1];
+                for (uint32_t sb = lo; sb < hi; ++sb) {
                     const double4 xb = d.a.x[sb];
-                    const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
+                    const double ex = (xa.x - xb.x) + dmx, ey = (xa.y - xb.y) + dmy, ez = xa.z - xb.z;
+                    const double d2 = ex * ex + ey * ey + ez * ez;
                     const double rsum = ra + xb.w;
-                    const double d2 = dot(xab, xab);
-                    // d2 above (ra+rb)^2 by more than rounding: sqrt(d2) >= ra+rb,
-                    // so the exact test below would fail; skip the sqrt
-                    if (d2 > rsum * rsum * 1.0000001) continue;
-                    const double delta = sqrt(d2) - rsum;
-                    if (delta < 0.0) {
-                        const uint32_t bo = d.a.id[sb];
-                        // src/contactsearch.cuh:252-265: the slot already holding b
-                        // (lowest), else the highest empty slot
-                        int cpos = -1;
-                        for (uint32_t m = mask; m; m &= m - 1) {
-                            const int i = __ffs(m) - 1;
-                            if ((slot[i] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+                    if (__builtin_expect(d2 <= rsum * rsum * 1.0000001, 0) && sb != s) {
+                        if (sqrt(d2) - rsum < 0.0) {
+                            slot_insert(slot, st, d.a.id[sb], code);
+                            if (st.stop) break;
                         }
-                        if (cpos < 0) {
-                            const uint32_t empty = ~mask;
-                            if (empty == 0u) { err |= FLAG_CONTACT_OVERFLOW; stop = true; break; }
-                            cpos = 31 - __clz(empty);
-                            fresh |= 1u << cpos;
-                        }
-                        slot[cpos] = bo | (code << 28);
-                        mask |= 1u << cpos;
-                        if (++nc > (uint32_t)NC) { err |= FLAG_CONTACT_OVERFLOW; stop = true; break; }
                     }
                 }
             }
         }
     }
+    d.tmask[s] = st.mask;
+    d.tfresh[s] = st.fresh;
+    if (st.err) atomicOr(d.flags, st.err);
+}
+
+// ------------------------------------------------------------ force kernel
+// One thread per sorted particle: contact forces from the occupied slots in
+// slot order, wall contacts, bond forces, leapfrog.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= d.np) return;
+    const uint32_t o = d.a.id[s];
+    const double4 xa = d.a.x[s];
+    const double ra = xa.w;
+
+    // contact slots after the search kernel
+    uint32_t mask = d.tmask[s];
+    const uint32_t fresh = d.tfresh[s];  // slots taken from empty this step: delta_t starts at 0
+    const uint32_t* slot = d.cslot + (size_t)o * NC;
 
     // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
     const double4 va4 = d.a.vel[s];
@@ -611,7 +815,6 @@ __global__ __launch_bounds__(BLOCK) void k_contact(DemDev d) {
     d.b.x[s] = make_double4(px, py, pz, radius);
     d.b.id[s] = o;
     d.b.cmask[s] = mask;
-    if (err) atomicOr(d.flags, err);
 }
 
 // ------------------------------------------------------------------- walls
@@ -704,9 +907,17 @@ void launch_scan(const DemDev& d, hipStream_t s) {
 void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
 void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
+void launch_topology(const DemDev& d, hipStream_t s) {
+    if (d.search_mode == 2) k_topo_rows<<<nblk(d.np), BLOCK, 0, s>>>(d);
+    else k_topo<<<d.tiles_x * d.tiles_y * d.num[2], BLOCK, 0, s>>>(d);
+}
+void launch_forces(const DemDev& d, int precision, hipStream_t s) {
+    if (precision == SPH_PREC_MIXED) k_force<float><<<nblk(d.np), BLOCK, 0, s>>>(d);
+    else k_force<double><<<nblk(d.np), BLOCK, 0, s>>>(d);
+}
 void launch_contact(const DemDev& d, int precision, hipStream_t s) {
-    if (precision == SPH_PREC_MIXED) k_contact<float><<<nblk(d.np), BLOCK, 0, s>>>(d);
-    else k_contact<double><<<nblk(d.np), BLOCK, 0, s>>>(d);
+    launch_topology(d, s);
+    launch_forces(d, precision, s);
 }
 void launch_walls(const DemDev& d, hipStream_t s) {
     if (d.nw) k_wallsum<<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);

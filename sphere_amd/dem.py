@@ -105,7 +105,8 @@ class SphParams(C.Structure):
 class SphTiming(C.Structure):
     _fields_ = [("bonds_ms", C.c_double), ("cellid_ms", C.c_double), ("sort_ms", C.c_double),
                 ("reorder_ms", C.c_double), ("contact_ms", C.c_double), ("walls_ms", C.c_double),
-                ("total_ms", C.c_double), ("steps", C.c_uint64)]
+                ("total_ms", C.c_double), ("steps", C.c_uint64), ("search_ms", C.c_double),
+                ("force_ms", C.c_double), ("tile_x", C.c_uint32), ("tile_y", C.c_uint32)]
 
 
 EXPORTS = {
@@ -127,6 +128,8 @@ EXPORTS = {
     "sph_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_get_timing": (C.c_int, [C.c_void_p, C.POINTER(SphTiming)]),
     "sph_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_set_tile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "sph_set_search": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_dry_run": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t,
                               C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
     "sph_pack_rsa": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
@@ -290,6 +293,12 @@ class DEM:
 
     def set_graph(self, on: bool = True):
         self._check(self._lib.sph_set_graph(self._h, 1 if on else 0))
+
+    def set_tile(self, tile_x: int = 0, tile_y: int = 0):
+        self._check(self._lib.sph_set_tile(self._h, int(tile_x), int(tile_y)))
+
+    def set_search(self, mode: int = 0):
+        self._check(self._lib.sph_set_search(self._h, int(mode)))
 
     def timing(self) -> Dict[str, float]:
         t = SphTiming()

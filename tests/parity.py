@@ -56,13 +56,16 @@ def all_fields(orc):
     return fields
 
 
-def run_pair(sb, tmpdir, steps_list, precision="fp64"):
-    """Steps GPU and oracle side by side; returns a list of mismatch messages."""
+def run_pair(sb, tmpdir, steps_list, precision="fp64", setup=None):
+    """Steps GPU and oracle side by side; returns a list of mismatch messages.
+    `setup(dem)` may choose kernel variants before the first step."""
     from oracle_lib import Oracle
     from sphere_amd.dem import DEM
 
     orc = Oracle.from_spherebin(sb, tmpdir)
     dem = DEM(sb, verbosity=0)
+    if setup is not None:
+        setup(dem)
     if precision != "fp64":
         dem.set_precision(precision)
     errors = []

@@ -103,3 +103,24 @@ def test_bonded_lattice_parity(tmp_path):
     sb = configs.fcc_bonded(cells=(3, 3, 4), sigma_b=2e7, tau_b=2e7)
     errs = run_pair(sb, tmp_path, [1, 50, 300])
     assert not errs, errs
+
+
+SEARCH_VARIANTS = [
+    ("rows", lambda d: d.set_search(2)),
+    ("tiles-auto", lambda d: d.set_search(1)),
+    ("tiles-1x1", lambda d: (d.set_search(1), d.set_tile(1, 1))),
+    ("tiles-wide", lambda d: (d.set_search(1), d.set_tile(6, 3))),
+]
+
+
+@pytest.mark.parametrize("name,setup", SEARCH_VARIANTS, ids=[v[0] for v in SEARCH_VARIANTS])
+@pytest.mark.parametrize("periodic", [0, 1, 2])
+def test_search_variants_parity(tmp_path, name, setup, periodic):
+    """Both contact-search kernels (LDS tiles of any shape, one thread per
+    sorted particle) give the oracle's slots bit for bit, including tiles
+    wider than the grid where periodic images repeat staged cells."""
+    _say(f"search {name} periodic={periodic}")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=periodic,
+                                gamma_n=0.0, gamma_t=10.0, sid=f"search-{name}-{periodic}")
+    errs = run_pair(sb, tmp_path, [1, 50, 300], setup=setup)
+    assert not errs, errs
