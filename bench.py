@@ -3,12 +3,16 @@
 Default workload: 1,048,576 spheres, lognormal radii (median 1e-2 m, sigma
 0.2, clipped to [0.5, 2] x median), seeded random non-overlapping packing in
 a 2.07 x 2.07 x 4.13 m x/y-periodic box (the nominal 0.8 x 0.8 x 1.6 m box
-cannot hold that much solid), gravity, top wall at 10 kPa, the contact
+cannot hold that much solid), gravity, top wall at 10 kPa, contact
 parameters k_n = k_t = 1e6 N/m, gamma_n = gamma_t = 100 N s/m, mu = 0.5.
 
-A step is one full DEM time step (bonds, cell ids, sort, reorder,
-contact search + forces + integration, wall reduction + wall update) with
-all state resident in HBM.  Prints ONE JSON line.
+A step is one full DEM time step (bonds, cell ids, sort, reorder, contact
+search, contact forces + integration, wall reduction + wall update) with all
+state resident in HBM.  Prints ONE JSON line on stdout.
+
+Per-kernel time comes from hipEvents recorded on the simulation's own stream
+around each stage, over a separate run of the same steps; the roofline entry
+is for the stage with the largest share of the step.
 """
 
 from __future__ import annotations
@@ -17,6 +21,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import subprocess
 import sys
 import tempfile
@@ -26,22 +31,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 1M spheres (FP64 vs mixed); adjoint/forward step time ratio"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def contact_bytes(n, c, w):
-    """Algorithmic HBM bytes of one contact-kernel launch (topology + interact +
-    integrate fused): 32N + 32C (topology), 196N + 76C + 8WN (interact),
-    388N (integrate)."""
-    return 616 * n + 108 * c + 8 * w * n
-
-
-def step_bytes(n, c, b, w, k):
-    """Algorithmic bytes of one FP64 step: 848N + 16WN + 108C + 136B + 8K."""
-    return 848 * n + 16 * w * n + 108 * c + 136 * b + 8 * k
+def kernel_bytes(n, c, b, w, k):
+    """Algorithmic HBM bytes per launch of each stage: FP64 compact records,
+    8 B per double, 4 B per index, every required array read or written once,
+    neighbour and bond-endpoint gathers counted as cache resident."""
+    return {
+        "bonds": 136 * b,                          # pair 8, delta 32, omega 32 read; delta, omega 64 write
+        "cellid": 32 * n,                          # xyz 24 read, key + index 8 write
+        "sort": 16 * n + 8 * k,                    # key + index read and write, cell bounds
+        "reorder": 184 * n,                        # key, index, 11 doubles gathered and written
+        "search": 32 * n + 32 * c,                 # own x,y,z,r; old + new slot, image per active slot
+        "force": 584 * n + 76 * c + 8 * w * n,     # interact (196N + 76C + 8WN) + integrate (388N)
+        "walls": 8 * w * n,
+    }
 
 
 def cpu_model():
@@ -55,16 +64,29 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "c1_1M_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*%s\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$" % re.escape(kernel), line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
 def cpu_baseline(bin_path, n, sample_steps):
-    """Serial CPU oracle (one core) on the first `sample_steps` steps of the
-    same input; returns particle-steps/s."""
+    """Serial CPU oracle (one core) on `sample_steps` steps of the same input
+    after one untimed step; returns particle-steps/s and seconds."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
 
     t0 = time.perf_counter()
     orc = Oracle(bin_path)
     log(f"oracle loaded in {time.perf_counter() - t0:.1f}s")
-    orc.step(1)  # first touch of the slot arrays, untimed
+    orc.step(1)
     t0 = time.perf_counter()
     flags = orc.step(sample_steps)
     dt = time.perf_counter() - t0
@@ -72,6 +94,12 @@ def cpu_baseline(bin_path, n, sample_steps):
     if flags:
         log(f"oracle flags {flags}")
     return n * sample_steps / dt, dt
+
+
+STAGE_KERNELS = {
+    "bonds": "k_bonds", "cellid": "k_cellid", "sort": "k_scan_*/k_scatter/k_cellsort",
+    "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
+}
 
 
 def main():
@@ -84,19 +112,26 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="run only the copy-kernel bandwidth probe (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is a design note only",
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design note",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
     from sphere_amd import configs
-    from sphere_amd.dem import DEM
+    from sphere_amd.dem import DEM, bandwidth_probe
+
+    if args.probe_only:
+        gbs = bandwidth_probe(1 << 30, 20)
+        print(json.dumps({"probe": "k_copy_probe", "bytes_per_launch": 2 * (1 << 30), "GBs": gbs}))
+        return
 
     t0 = time.perf_counter()
     sb = configs.random_compression(n=args.n)
-    log(f"generated {args.n} spheres in {time.perf_counter() - t0:.1f}s, grid {list(sb.num)}")
+    log(f"generated {args.n} spheres in {time.perf_counter() - t0:.1f}s, grid {[int(v) for v in sb.num]}")
     tmp = tempfile.mkdtemp(prefix="sphere-bench-")
     bin_path = os.path.join(tmp, "c1.bin")
     sb.writebin(bin_path)
@@ -107,34 +142,34 @@ def main():
         dem = DEM(bin_path, verbosity=0)
         dem.set_precision(precision)
         dem.step(args.warmup)
-        import ctypes as C
-        from sphere_amd.dem import load_library
-        lib = load_library()
-        # barrier + sync on both sides of the timed region: sph_step returns
-        # only after the device finished (it reads back the error flags)
+        # sph_step returns after the device finished the steps (it reads back
+        # the error flags), so the timed region is bracketed by device syncs
         t0 = time.perf_counter()
         dem.step(args.steps)
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1e3 / args.steps
-        contacts = dem.get("contacts")
-        c = int((contacts != n).sum())
-        # per-kernel hipEvent timing on the handle's stream over separate steps
+        c = int((dem.get("contacts") != n).sum())
         dem.set_profiling(True)
         dem.step(min(args.steps, 50))
         tim = dem.timing()
         dem.set_profiling(False)
-        wall = dem.get("walls_mvfd")
         dem.close()
-        return ms, c, tim, wall
+        return ms, c, tim
 
-    ms, c, tim, wall = timed_run(args.precision)
-    log(f"{args.precision}: {ms:.3f} ms/step, C/N={c / n:.3f}, per-kernel {tim}")
-    value = n / (ms * 1e-3)
+    ms, c, tim = timed_run(args.precision)
     steps_prof = tim["steps"] or 1
-    contact_ms = tim["contact_ms"] / steps_prof
-    achieved = contact_bytes(n, c, 1) / (contact_ms * 1e-3) / 1e9
+    stage_ms = {k: tim[f"{k}_ms"] / steps_prof for k in ("bonds", "cellid", "sort", "reorder", "search", "force", "walls")}
+    log(f"{args.precision}: {ms:.4f} ms/step, C/N={c / n:.3f}, stages {stage_ms}")
+    value = n / (ms * 1e-3)
+    kb = kernel_bytes(n, c, sb.nb0, sb.nw, K)
+    per_kernel = {k: {"ms": stage_ms[k], "kernels": STAGE_KERNELS[k], "algorithmic_bytes": kb[k],
+                      "GBs": (kb[k] / (stage_ms[k] * 1e-3) / 1e9) if stage_ms[k] > 0 else None}
+                  for k in stage_ms}
+    dom = max(stage_ms, key=lambda k: stage_ms[k])
+    achieved = kb[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    step_b = sum(kb.values())
     result = {
-        "metric": "particle-steps/s at 1M spheres (FP64 vs mixed); adjoint/forward step time ratio",
+        "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -144,27 +179,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 pair forces",
         "data": "synthetic (seeded lognormal random packing)",
-        "config": {"workload": "1,048,576-sphere lognormal packing under a 10 kPa top wall (2.07x2.07x4.13 m, x/y periodic)",
-                   "n": n, "cells": K, "contacts_per_particle": c / n, "precision": args.precision},
-        "per_kernel_ms": {k: (v / steps_prof if k.endswith("_ms") else v) for k, v in tim.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_contact", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": contact_bytes(n, c, 1)},
-        "step_roofline": {"algorithmic_bytes": step_bytes(n, c, 0, 1, K),
-                          "achieved_GBs": step_bytes(n, c, 0, 1, K) / (ms * 1e-3) / 1e9},
+        "config": {"workload": "1,048,576 lognormal spheres, random packing, 10 kPa top wall, 2.07x2.07x4.13 m x/y-periodic box",
+                   "n": n, "cells": K, "tile": [tim["tile_x"], tim["tile_y"]],
+                   "contacts_per_particle": c / n, "precision": args.precision},
+        "per_kernel": per_kernel,
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(STAGE_KERNELS[dom]),
+                     "algorithmic_bytes_per_launch": kb[dom]},
+        "step_roofline": {"algorithmic_bytes": step_b, "achieved_GBs": step_b / (ms * 1e-3) / 1e9,
+                          "frac": step_b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_mixed and args.precision == "fp64":
-        ms_m, c_m, tim_m, _ = timed_run("mixed")
-        log(f"mixed: {ms_m:.3f} ms/step")
+        ms_m, c_m, tim_m = timed_run("mixed")
+        log(f"mixed: {ms_m:.4f} ms/step")
         result["mixed"] = {"value": n / (ms_m * 1e-3), "ms_per_step": ms_m, "speedup_vs_fp64": ms / ms_m,
-                           "contact_ms": tim_m["contact_ms"] / (tim_m["steps"] or 1)}
+                           "contacts_per_particle": c_m / n}
     result["adjoint_forward_ratio"] = None
     if not args.no_cpu:
         rate, secs = cpu_baseline(bin_path, n, args.cpu_steps)
         result["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                                  "sample": f"same 1,048,576-sphere input, {args.cpu_steps} steps after one untimed step ({secs:.1f}s)",
+                                  "sample": f"same 1,048,576-sphere input, {args.cpu_steps} steps after one untimed step ({secs:.1f} s)",
                                   "cpu": cpu_model()}
     print(json.dumps(result), flush=True)
 

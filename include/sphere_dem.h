@@ -176,6 +176,11 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y);
  * default (2). */
 int sph_set_search(sph_t* h, int mode);
 
+/* Measurement helper with no counterpart in the reference: copies `bytes`
+ * between two fresh device buffers `reps` times with a plain streaming kernel
+ * (k_copy_probe) and returns the achieved read+write rate in GB/s. */
+int sph_bandwidth_probe(uint64_t bytes, int reps, double* gbs);
+
 /* Setup helper (host only): seeded non-overlapping random sequential packing
  * of n spheres with the given radii in box[3] (periodic: 0 none, 1 x+y, 2 x),
  * largest first; xyz[3n] receives the centres.  Stands in for the

@@ -128,6 +128,7 @@ void launch_contact(const DemDev& d, int precision, hipStream_t s);  // search +
 void launch_topology(const DemDev& d, hipStream_t s);
 void launch_forces(const DemDev& d, int precision, hipStream_t s);
 void launch_walls(const DemDev& d, hipStream_t s);
+void launch_copy_probe(const double4* src, double4* dst, uint64_t n, hipStream_t s);
 
 }  // namespace sph
 

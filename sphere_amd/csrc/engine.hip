@@ -951,4 +951,32 @@ int sph_set_graph(sph_t* h, int on) {
     return SPH_OK;
 }
 
+int sph_bandwidth_probe(uint64_t bytes, int reps, double* gbs) {
+    if (bytes < 64 || reps < 1 || !gbs) return SPH_E_INVALID;
+    const uint64_t n = bytes / 32;
+    double4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPH_OK;
+    if (hipMalloc(&a, n * 32) != hipSuccess || hipMalloc(&b, n * 32) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemset(a, 0, n * 32) != hipSuccess) {
+        rc = SPH_E_HIP;
+    } else {
+        sph::launch_copy_probe(a, b, n, st);  // warm up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) sph::launch_copy_probe(a, b, n, st);
+        (void)hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SPH_E_HIP;
+        else *gbs = 2.0 * (double)(n * 32) * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 }  // extern "C"

@@ -888,6 +888,13 @@ __global__ __launch_bounds__(BLOCK) void k_walls(DemDev d) {
     if (threadIdx.x == 0) *d.t_cur = t_current + d.dt;
 }
 
+// Plain streaming copy used to calibrate bandwidth counters.
+__global__ __launch_bounds__(BLOCK) void k_copy_probe(const double4* __restrict__ src, double4* __restrict__ dst,
+                                                      uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 inline uint32_t nblk(uint64_t n, uint32_t b = BLOCK) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
@@ -918,6 +925,9 @@ void launch_forces(const DemDev& d, int precision, hipStream_t s) {
 void launch_contact(const DemDev& d, int precision, hipStream_t s) {
     launch_topology(d, s);
     launch_forces(d, precision, s);
+}
+void launch_copy_probe(const double4* src, double4* dst, uint64_t n, hipStream_t s) {
+    k_copy_probe<<<2048, BLOCK, 0, s>>>(src, dst, n);
 }
 void launch_walls(const DemDev& d, hipStream_t s) {
     if (d.nw) k_wallsum<<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);

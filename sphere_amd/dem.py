@@ -130,6 +130,7 @@ EXPORTS = {
     "sph_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_set_tile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "sph_set_search": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "sph_dry_run": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t,
                               C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
     "sph_pack_rsa": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
@@ -347,3 +348,13 @@ def dry_run(data: bytes, check_vals: int = 1, roundtrip: bool = False):
     if rc != SPH_OK:
         _raise(rc, err.value.decode())
     return out.raw[: n.value]
+
+
+def bandwidth_probe(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Read+write GB/s of a plain device-to-device copy (k_copy_probe)."""
+    lib = load_library()
+    out = C.c_double(0.0)
+    rc = lib.sph_bandwidth_probe(int(nbytes), int(reps), C.byref(out))
+    if rc != SPH_OK:
+        _raise(rc, "bandwidth probe failed")
+    return out.value
