@@ -172,8 +172,8 @@ int sph_set_graph(sph_t* h, int on);
 int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y);
 
 /* Contact-search kernel: 1 = one block per tile with the neighbourhood staged
- * in LDS, 2 = one thread per sorted particle reading through L1/L2, 0 = the
- * default (2). */
+ * in LDS, 2 = one thread per sorted particle reading through L1/L2, 3 = one
+ * wavefront per cell, one lane per candidate, 0 = the default (2). */
 int sph_set_search(sph_t* h, int mode);
 
 /* Measurement helper with no counterpart in the reference: copies `bytes`

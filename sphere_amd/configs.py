@@ -211,6 +211,36 @@ def shear_cell(n=65536, box=None, stress=10e3, shear_velocity=0.1, gravity=-9.81
     return sb
 
 
+def dense_lattice(nside=(128, 128, 64), spacing=1.98e-2, radius=1e-2, jitter=1e-5, steps=2000,
+                  seed=BASE_SEED + 5, sid="dense-lattice") -> Spherebin:
+    """Confined simple-cubic crystal with every nearest-neighbour pair in
+    contact: spacing below 2 r, x/y periodic at the lattice period, the floor
+    and a fixed top wall each overlapping their layer by the same amount.
+    About six active contacts per sphere for the whole run, the contact load
+    the loose random packing does not reach within its time window."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = nside
+    n = nx * ny * nz
+    sb = Spherebin(n, nw=1, sid=sid)
+    kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float64)
+    sb.x = (sites + 0.5) * spacing + rng.uniform(-jitter, jitter, size=(n, 3))
+    sb.radius = np.full(n, radius)
+    _contact_params(sb)
+    L = np.array([nx * spacing, ny * spacing, nz * spacing * 1.05])
+    _grid(sb, L, 2.0 * radius * 1.0001)
+    sb.periodic = 1
+    sb.g = np.array([0.0, 0.0, -9.81])
+    sb.set_walls(1)
+    sb.wmode = np.array([0], dtype=np.int32)
+    sb.w_x[0] = nz * spacing
+    sb.w_m[0] = sb.rho * n * math.pi * (L[0] / sb.num[0] / 2.0) ** 3
+    sb.time_dt = _dt(sb)
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    return sb
+
+
 def two_spheres(distance=0.0, radius=0.5, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0), bonded=False,
                 gamma=0.0, k=1.16e9, mu=0.4, L=20.0, periodic=1, gravity=0.0, sid="pair") -> Spherebin:
     """Two spheres on the x axis, the layout of tests/bond_tests.py:42-50."""

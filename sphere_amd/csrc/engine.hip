@@ -938,7 +938,7 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y) {
 }
 
 int sph_set_search(sph_t* h, int mode) {
-    if (!h || mode < 0 || mode > 2) return SPH_E_INVALID;
+    if (!h || mode < 0 || mode > 3) return SPH_E_INVALID;
     h->d.search_mode = mode == 0 ? 2 : mode;
     destroy_graph(h);
     return SPH_OK;

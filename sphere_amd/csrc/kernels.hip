@@ -681,6 +681,172 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     if (st.err) atomicOr(d.flags, st.err);
 }
 
+// One wavefront per cell.  The candidates of the cell's 27-cell neighbourhood,
+// in the reference's visiting order (z, y, x cells, sorted order within a
+// cell; src/contactsearch.cuh:340-353), are up to 27 runs of the sorted
+// arrays; candidate t of the concatenation sits in lane t % 64 of chunk
+// t / 64, in registers for the first SEARCH_REG_CHUNKS chunks.  Each member of
+// the cell tests all candidates at once, one lane per candidate, and applies
+// the overlaps (a ballot) in lane order, i.e. in the serial loop's order.
+// While a member is processed its 32 slot words sit in lanes 0..31, so the
+// slot rule (src/contactsearch.cuh:252-271) is one ballot per overlap.
+constexpr int SEARCH_REG_CHUNKS = 2;
+
+struct Cand {
+    double x, y, z, r, dmx, dmy;
+    uint32_t sb, bid, code;
+    bool ok;
+};
+
+// Candidate t of the run list held in lanes 0..26 (start, exclusive prefix,
+// length, image code); `runs` has a bit per non-empty run.
+__device__ __forceinline__ Cand load_cand(const DemDev& d, uint32_t t, uint32_t T, uint64_t runs, uint32_t r_lo,
+                                          uint32_t r_pre, uint32_t r_len, uint32_t r_code) {
+    Cand c;
+    uint32_t sb = 0, code = 0;
+    for (uint64_t m = runs; m; m &= m - 1) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        const uint32_t pre = __builtin_amdgcn_readlane(r_pre, j);
+        const uint32_t len = __builtin_amdgcn_readlane(r_len, j);
+        if (t - pre < len) {
+            sb = __builtin_amdgcn_readlane(r_lo, j) + (t - pre);
+            code = __builtin_amdgcn_readlane(r_code, j);
+        }
+    }
+    c.ok = t < T;
+    c.sb = sb;
+    c.code = code;
+    if (c.ok) {
+        const double4 xb = d.a.x[sb];
+        c.x = xb.x; c.y = xb.y; c.z = xb.z; c.r = xb.w;
+        c.bid = d.a.id[sb];
+    } else {
+        c.x = c.y = c.z = c.r = 0.0;
+        c.bid = 0;
+    }
+    decode_image(d, code, c.dmx, c.dmy);
+    return c;
+}
+
+struct MemberState {
+    uint32_t mask, fresh, nc, err;
+    bool stop, loaded;
+};
+
+// Overlaps of the member at sorted position m with one chunk of candidates.
+__device__ __forceinline__ void member_chunk(const Cand& cb, uint32_t lane, uint32_t m, double4 xa,
+                                             const uint32_t* slot, uint32_t& w, bool& dirty, MemberState& st) {
+    const double ex = (xa.x - cb.x) + cb.dmx, ey = (xa.y - cb.y) + cb.dmy, ez = xa.z - cb.z;
+    const double d2 = ex * ex + ey * ey + ez * ez;
+    const double rsum = xa.w + cb.r;
+    bool hit = false;
+    if (__builtin_expect(cb.ok && d2 <= rsum * rsum * 1.0000001 && cb.sb != m, 0))
+        hit = sqrt(d2) - rsum < 0.0;  // exact test of src/contactsearch.cuh:242-248 (double sqrt)
+    uint64_t hits = __ballot(hit);
+    while (hits) {
+        const int l = __ffsll((unsigned long long)hits) - 1;
+        hits &= hits - 1;
+        const uint32_t bo = __builtin_amdgcn_readlane(cb.bid, l);
+        const uint32_t code = __builtin_amdgcn_readlane(cb.code, l);
+        if (!st.loaded) {
+            w = lane < (uint32_t)NC ? slot[lane] : 0u;
+            st.loaded = true;
+        }
+        const uint64_t match = __ballot(lane < (uint32_t)NC && ((st.mask >> lane) & 1u) && (w & SLOT_IDX_MASK) == bo);
+        int cpos;
+        if (match) {
+            cpos = __ffsll((unsigned long long)match) - 1;
+        } else {
+            const uint32_t empty = ~st.mask;
+            if (empty == 0u) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; return; }
+            cpos = 31 - __clz(empty);
+            st.fresh |= 1u << cpos;
+        }
+        if (lane == (uint32_t)cpos) { w = bo | (code << 28); dirty = true; }
+        st.mask |= 1u << cpos;
+        if (++st.nc > (uint32_t)NC) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; return; }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t c = xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (c >= d.K) return;
+    const uint32_t m0 = d.off[c], m1 = d.off[c + 1];
+    if (m0 == m1) return;
+    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
+    const int gx = (int)(c % d.num[0]), gy = (int)((c / d.num[0]) % d.num[1]), gz = (int)(c / (d.num[0] * d.num[1]));
+
+    // run j = lane (0..26): cell (gx + j%3 - 1, gy + (j/3)%3 - 1, gz + j/9 - 1);
+    // a row without periodic image is one run, held by its dx = -1 lane
+    uint32_t r_lo = 0, r_len = 0, r_code = 0;
+    if (lane < 27u) {
+        const int dz = (int)(lane / 9u) - 1, dy = (int)((lane / 3u) % 3u) - 1, dx = (int)(lane % 3u) - 1;
+        const int tz = gz + dz;
+        int ty = gy + dy, tx = gx + dx;
+        bool ok = tz >= 0 && tz < nz;
+        uint32_t cy = 0, cx = 0;
+        if (ok && (ty < 0 || ty == ny)) {
+            if (d.periodic == 1) { cy = ty < 0 ? 1u : 2u; ty = ty < 0 ? ny - 1 : 0; }
+            else ok = false;
+        }
+        if (ok) {
+            const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+            if (gx >= 1 && gx + 1 < nx && cy == 0) {
+                if (dx == -1) { r_lo = d.off[row + gx - 1]; r_len = d.off[row + gx + 2] - r_lo; }
+            } else {
+                if (tx < 0 || tx == nx) {
+                    if (d.periodic != 0) { cx = tx < 0 ? 1u : 2u; tx = tx < 0 ? nx - 1 : 0; }
+                    else ok = false;
+                }
+                if (ok) {
+                    r_lo = d.off[row + tx];
+                    r_len = d.off[row + tx + 1] - r_lo;
+                    r_code = cx | (cy << 2);
+                }
+            }
+        }
+    }
+    uint32_t incl = r_len;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if (lane >= (uint32_t)o) incl += v;
+    }
+    const uint32_t r_pre = incl - r_len;
+    const uint32_t T = __shfl(incl, 26, 64);
+    const uint64_t runs = __ballot(r_len != 0u);
+
+    Cand cand[SEARCH_REG_CHUNKS];
+#pragma unroll
+    for (int j = 0; j < SEARCH_REG_CHUNKS; ++j)
+        cand[j] = load_cand(d, 64u * j + lane, T, runs, r_lo, r_pre, r_len, r_code);
+
+    for (uint32_t m = m0; m < m1; ++m) {
+        const double4 xa = d.a.x[m];
+        const uint32_t o = d.a.id[m];
+        const uint32_t* slot = d.cslot + (size_t)o * NC;
+        MemberState st;
+        st.mask = d.a.cmask[m];
+        st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false; st.loaded = false;
+        uint32_t w = 0;
+        bool dirty = false;
+#pragma unroll
+        for (int j = 0; j < SEARCH_REG_CHUNKS; ++j)
+            if (64u * j < T && !st.stop) member_chunk(cand[j], lane, m, xa, slot, w, dirty, st);
+        for (uint32_t base = 64u * SEARCH_REG_CHUNKS; base < T && !st.stop; base += 64u) {
+            const Cand cb = load_cand(d, base + lane, T, runs, r_lo, r_pre, r_len, r_code);
+            member_chunk(cb, lane, m, xa, slot, w, dirty, st);
+        }
+        if (dirty) d.cslot[(size_t)o * NC + lane] = w;
+        if (lane == 0) {
+            d.tmask[m] = st.mask;
+            d.tfresh[m] = st.fresh;
+            if (st.err) atomicOr(d.flags, st.err);
+        }
+    }
+}
+
 // ------------------------------------------------------------ force kernel
 // One thread per sorted particle: contact forces from the occupied slots in
 // slot order, wall contacts, bond forces, leapfrog.
@@ -915,7 +1081,8 @@ void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BL
 void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
 void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_topology(const DemDev& d, hipStream_t s) {
-    if (d.search_mode == 2) k_topo_rows<<<nblk(d.np), BLOCK, 0, s>>>(d);
+    if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
+    else if (d.search_mode == 2) k_topo_rows<<<nblk(d.np), BLOCK, 0, s>>>(d);
     else k_topo<<<d.tiles_x * d.tiles_y * d.num[2], BLOCK, 0, s>>>(d);
 }
 void launch_forces(const DemDev& d, int precision, hipStream_t s) {

@@ -107,10 +107,29 @@ def test_bonded_lattice_parity(tmp_path):
 
 SEARCH_VARIANTS = [
     ("rows", lambda d: d.set_search(2)),
+    ("wave", lambda d: d.set_search(3)),
     ("tiles-auto", lambda d: d.set_search(1)),
     ("tiles-1x1", lambda d: (d.set_search(1), d.set_tile(1, 1))),
     ("tiles-wide", lambda d: (d.set_search(1), d.set_tile(6, 3))),
 ]
+
+
+COARSE_GRIDS = [((2, 2, 2), 0), ((1, 3, 2), 0), ((2, 1, 1), 0), ((3, 3, 2), 1), ((4, 3, 1), 1),
+                ((3, 1, 2), 2), ((3, 2, 2), 2)]
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("num,periodic", COARSE_GRIDS, ids=[f"{g[0]}-p{g[1]}" for g in COARSE_GRIDS])
+def test_coarse_grid_search_parity(tmp_path, mode, num, periodic):
+    """Cells far larger than the spheres: 10-40 members per cell, more than
+    128 candidates per neighbourhood, one- and two-cell axes, and three-cell
+    periodic axes where a neighbourhood wraps onto both ends of the row."""
+    _say(f"coarse grid num={num} periodic={periodic} search={mode}")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=periodic,
+                                gamma_n=0.0, gamma_t=10.0, sid=f"coarse-{mode}")
+    sb.num = np.array(num, dtype=np.uint32)
+    errs = run_pair(sb, tmp_path, [1, 50, 200], setup=lambda d: d.set_search(mode))
+    assert not errs, errs
 
 
 @pytest.mark.parametrize("name,setup", SEARCH_VARIANTS, ids=[v[0] for v in SEARCH_VARIANTS])
