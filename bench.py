@@ -6,13 +6,16 @@ a 2.07 x 2.07 x 4.13 m x/y-periodic box (the nominal 0.8 x 0.8 x 1.6 m box
 cannot hold that much solid), gravity, top wall at 10 kPa, contact
 parameters k_n = k_t = 1e6 N/m, gamma_n = gamma_t = 100 N s/m, mu = 0.5.
 
-A step is one full DEM time step (bonds, cell ids, sort, reorder, contact
-search, contact forces + integration, wall reduction + wall update) with all
-state resident in HBM.  Prints ONE JSON line on stdout.
+A step is one full DEM time step (bonds, sort, reorder, contact search,
+contact forces + integration + next cell ids, wall update) with all state
+resident in HBM.  Prints ONE JSON line on stdout.
 
-Per-kernel time comes from hipEvents recorded on the simulation's own stream
-around each stage, over a separate run of the same steps; the roofline entry
-is for the stage with the largest share of the step.
+Also measured: the mixed-precision mode on the same input, and both modes on
+a dense 1,048,576-sphere lattice (about six contacts per sphere), with the
+mixed mode's error against the FP64 path per output.  Per-kernel time comes
+from hipEvents recorded on the simulation's own stream around each stage,
+over a separate run of the same steps; the roofline entry is for the stage
+with the largest share of the step.
 """
 
 from __future__ import annotations
@@ -27,11 +30,19 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 1M spheres (FP64 vs mixed); adjoint/forward step time ratio"
+STAGES = ("bonds", "cellid", "sort", "reorder", "search", "force", "walls")
+STAGE_KERNELS = {
+    "bonds": "k_bonds", "cellid": "(in k_force)", "sort": "k_scan_*/k_fill_u32/k_scatter/k_cellsort",
+    "reorder": "k_reorder", "search": "k_topo_wave", "force": "k_force", "walls": "k_wallsum/k_walls",
+}
+PROFILE_KERNEL = {"reorder": "k_reorder", "search": "k_topo_wave", "force": "k_force<double>"}
 
 
 def log(msg):
@@ -44,11 +55,12 @@ def kernel_bytes(n, c, b, w, k):
     neighbour and bond-endpoint gathers counted as cache resident."""
     return {
         "bonds": 136 * b,                          # pair 8, delta 32, omega 32 read; delta, omega 64 write
-        "cellid": 32 * n,                          # xyz 24 read, key + index 8 write
+        "cellid": 0,                               # computed by the force kernel (counted there)
         "sort": 16 * n + 8 * k,                    # key + index read and write, cell bounds
         "reorder": 184 * n,                        # key, index, 11 doubles gathered and written
         "search": 32 * n + 32 * c,                 # own x,y,z,r; old + new slot, image per active slot
-        "force": 584 * n + 76 * c + 8 * w * n,     # interact (196N + 76C + 8WN) + integrate (388N)
+        # interact (196N + 76C + 8WN) + integrate (388N) + next cell id and count (8N)
+        "force": 592 * n + 76 * c + 8 * w * n,
         "walls": 8 * w * n,
     }
 
@@ -68,7 +80,7 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
     path = os.path.join(ROOT, "profiles", "c1_1M_pmc_traffic.md")
-    if not os.path.exists(path):
+    if not kernel or not os.path.exists(path):
         return None
     for line in open(path):
         m = re.match(r"\|\s*%s\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$" % re.escape(kernel), line)
@@ -96,10 +108,46 @@ def cpu_baseline(bin_path, n, sample_steps):
     return n * sample_steps / dt, dt
 
 
-STAGE_KERNELS = {
-    "bonds": "k_bonds", "cellid": "k_cellid", "sort": "k_scan_*/k_scatter/k_cellsort",
-    "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
-}
+def timed_run(bin_path, precision, steps, warmup, prof_steps):
+    """Warm-up, K timed steps (HIP graph replays), then a profiled segment
+    with per-stage hipEvents.  Returns timings and the final state."""
+    from sphere_amd.dem import DEM
+
+    dem = DEM(bin_path, verbosity=0)
+    n = dem.np
+    dem.set_precision(precision)
+    dem.step(warmup)
+    # sph_step returns after the device finished the steps (it reads back the
+    # error flags), so the timed region is bracketed by device syncs
+    t0 = time.perf_counter()
+    dem.step(steps)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    c = int((dem.get("contacts") != n).sum())
+    dem.set_profiling(True)
+    dem.step(prof_steps)
+    tim = dem.timing()
+    dem.set_profiling(False)
+    info = dem.info()
+    state = {"x": dem.get("x")[:, :3].copy(), "mvfd": dem.get("walls_mvfd") if info.nw else None,
+             "L": dem.get("time")}  # time kept for the record
+    dem.close()
+    stage_ms = {k: tim[f"{k}_ms"] / max(1, tim["steps"]) for k in STAGES}
+    return {"ms": ms, "contacts": c, "stage_ms": stage_ms, "tile": [tim["tile_x"], tim["tile_y"]],
+            "state": state, "steps_total": warmup + steps + prof_steps}
+
+
+def mixed_error(ref, mix, area, r_min):
+    """Error of the mixed run against the FP64 run after the same steps."""
+    dx = np.abs(mix["state"]["x"] - ref["state"]["x"])
+    out = {"steps": ref["steps_total"], "pos_max_abs_m": float(dx.max()),
+           "pos_max_rel_to_r_min": float(dx.max() / r_min)}
+    if ref["state"]["mvfd"] is not None:
+        s_ref = float(ref["state"]["mvfd"][0, 2]) / area
+        s_mix = float(mix["state"]["mvfd"][0, 2]) / area
+        out["wall_stress_fp64_Pa"] = s_ref
+        out["wall_stress_mixed_Pa"] = s_mix
+        out["wall_stress_rel"] = abs(s_mix - s_ref) / abs(s_ref) if s_ref != 0.0 else None
+    return out
 
 
 def main():
@@ -112,6 +160,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the copy-kernel bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -122,55 +171,38 @@ def main():
         sys.exit(2)
 
     from sphere_amd import configs
-    from sphere_amd.dem import DEM, bandwidth_probe
+    from sphere_amd.dem import bandwidth_probe
 
     if args.probe_only:
         gbs = bandwidth_probe(1 << 30, 20)
         print(json.dumps({"probe": "k_copy_probe", "bytes_per_launch": 2 * (1 << 30), "GBs": gbs}))
         return
 
+    prof_steps = min(args.steps, 50)
+    tmp = tempfile.mkdtemp(prefix="sphere-bench-")
     t0 = time.perf_counter()
     sb = configs.random_compression(n=args.n)
     log(f"generated {args.n} spheres in {time.perf_counter() - t0:.1f}s, grid {[int(v) for v in sb.num]}")
-    tmp = tempfile.mkdtemp(prefix="sphere-bench-")
     bin_path = os.path.join(tmp, "c1.bin")
     sb.writebin(bin_path)
     n = sb.np
     K = int(sb.num[0]) * int(sb.num[1]) * int(sb.num[2])
+    area = float(sb.L[0] * sb.L[1])
+    r_min = float(sb.radius.min())
 
-    def timed_run(precision):
-        dem = DEM(bin_path, verbosity=0)
-        dem.set_precision(precision)
-        dem.step(args.warmup)
-        # sph_step returns after the device finished the steps (it reads back
-        # the error flags), so the timed region is bracketed by device syncs
-        t0 = time.perf_counter()
-        dem.step(args.steps)
-        t1 = time.perf_counter()
-        ms = (t1 - t0) * 1e3 / args.steps
-        c = int((dem.get("contacts") != n).sum())
-        dem.set_profiling(True)
-        dem.step(min(args.steps, 50))
-        tim = dem.timing()
-        dem.set_profiling(False)
-        dem.close()
-        return ms, c, tim
-
-    ms, c, tim = timed_run(args.precision)
-    steps_prof = tim["steps"] or 1
-    stage_ms = {k: tim[f"{k}_ms"] / steps_prof for k in ("bonds", "cellid", "sort", "reorder", "search", "force", "walls")}
+    main_run = timed_run(bin_path, args.precision, args.steps, args.warmup, prof_steps)
+    ms, c, stage_ms = main_run["ms"], main_run["contacts"], main_run["stage_ms"]
     log(f"{args.precision}: {ms:.4f} ms/step, C/N={c / n:.3f}, stages {stage_ms}")
-    value = n / (ms * 1e-3)
     kb = kernel_bytes(n, c, sb.nb0, sb.nw, K)
     per_kernel = {k: {"ms": stage_ms[k], "kernels": STAGE_KERNELS[k], "algorithmic_bytes": kb[k],
-                      "GBs": (kb[k] / (stage_ms[k] * 1e-3) / 1e9) if stage_ms[k] > 0 else None}
-                  for k in stage_ms}
+                      "GBs": (kb[k] / (stage_ms[k] * 1e-3) / 1e9) if stage_ms[k] > 0 and kb[k] else None}
+                  for k in STAGES}
     dom = max(stage_ms, key=lambda k: stage_ms[k])
     achieved = kb[dom] / (stage_ms[dom] * 1e-3) / 1e9
     step_b = sum(kb.values())
     result = {
         "metric": METRIC,
-        "value": value,
+        "value": n / (ms * 1e-3),
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -182,21 +214,40 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 pair forces",
         "data": "synthetic (seeded lognormal random packing)",
         "config": {"workload": "1,048,576 lognormal spheres, random packing, 10 kPa top wall, 2.07x2.07x4.13 m x/y-periodic box",
-                   "n": n, "cells": K, "tile": [tim["tile_x"], tim["tile_y"]],
-                   "contacts_per_particle": c / n, "precision": args.precision},
+                   "n": n, "cells": K, "contacts_per_particle": c / n, "precision": args.precision},
         "per_kernel": per_kernel,
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(STAGE_KERNELS[dom]),
+                     "traffic": measured_traffic(PROFILE_KERNEL.get(dom)),
                      "algorithmic_bytes_per_launch": kb[dom]},
         "step_roofline": {"algorithmic_bytes": step_b, "achieved_GBs": step_b / (ms * 1e-3) / 1e9,
                           "frac": step_b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_mixed and args.precision == "fp64":
-        ms_m, c_m, tim_m = timed_run("mixed")
-        log(f"mixed: {ms_m:.4f} ms/step")
-        result["mixed"] = {"value": n / (ms_m * 1e-3), "ms_per_step": ms_m, "speedup_vs_fp64": ms / ms_m,
-                           "contacts_per_particle": c_m / n}
+        mix = timed_run(bin_path, "mixed", args.steps, args.warmup, prof_steps)
+        log(f"mixed: {mix['ms']:.4f} ms/step")
+        result["mixed"] = {"value": n / (mix["ms"] * 1e-3), "ms_per_step": mix["ms"],
+                           "speedup_vs_fp64": ms / mix["ms"], "contacts_per_particle": mix["contacts"] / n,
+                           "error_vs_fp64": mixed_error(main_run, mix, area, r_min)}
+    if not args.no_dense and args.precision == "fp64":
+        sbd = configs.dense_lattice()
+        dpath = os.path.join(tmp, "dense.bin")
+        sbd.writebin(dpath)
+        dn = sbd.np
+        d64 = timed_run(dpath, "fp64", args.steps, args.warmup, prof_steps)
+        dense = {"workload": "1,048,576-sphere simple-cubic lattice, spacing 0.99 diameter, x/y periodic, "
+                             "fixed top wall (every nearest-neighbour pair in contact)",
+                 "contacts_per_particle": d64["contacts"] / dn, "fp64": {"value": dn / (d64["ms"] * 1e-3),
+                 "ms_per_step": d64["ms"], "stage_ms": d64["stage_ms"]}}
+        log(f"dense fp64: {d64['ms']:.4f} ms/step, C/N={d64['contacts'] / dn:.2f}")
+        if not args.no_mixed:
+            dmx = timed_run(dpath, "mixed", args.steps, args.warmup, prof_steps)
+            log(f"dense mixed: {dmx['ms']:.4f} ms/step")
+            dense["mixed"] = {"value": dn / (dmx["ms"] * 1e-3), "ms_per_step": dmx["ms"],
+                              "speedup_vs_fp64": d64["ms"] / dmx["ms"], "stage_ms": dmx["stage_ms"],
+                              "error_vs_fp64": mixed_error(d64, dmx, float(sbd.L[0] * sbd.L[1]),
+                                                           float(sbd.radius.min()))}
+        result["dense"] = dense
     result["adjoint_forward_ratio"] = None
     if not args.no_cpu:
         rate, secs = cpu_baseline(bin_path, n, args.cpu_steps)

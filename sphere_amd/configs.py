@@ -211,29 +211,41 @@ def shear_cell(n=65536, box=None, stress=10e3, shear_velocity=0.1, gravity=-9.81
     return sb
 
 
-def dense_lattice(nside=(128, 128, 64), spacing=1.98e-2, radius=1e-2, jitter=1e-5, steps=2000,
-                  seed=BASE_SEED + 5, sid="dense-lattice") -> Spherebin:
-    """Confined simple-cubic crystal with every nearest-neighbour pair in
-    contact: spacing below 2 r, x/y periodic at the lattice period, the floor
-    and a fixed top wall each overlapping their layer by the same amount.
-    About six active contacts per sphere for the whole run, the contact load
-    the loose random packing does not reach within its time window."""
+def dense_lattice(cells=(64, 64, 64), radius=1e-2, overlap=2e-5, jitter=1e-6, gravity=0.0, steps=2000,
+                  seed=BASE_SEED + 5, sid="dense-fcc") -> Spherebin:
+    """Confined FCC crystal, 4 x 64^3 = 1,048,576 spheres, every nearest
+    neighbour pair overlapping by `overlap`: twelve contacts per sphere in the
+    interior for the whole run, the contact load the loose random packing
+    does not reach within its time window.  x/y periodic at the lattice
+    period; the floor and a fixed top wall overlap the outer layers by
+    overlap/2.  The normal dashpot is off (gamma_n = 0): model 2's normal
+    dashpot adds energy on approach (vel_n = -v_ab . n,
+    src/contactmodels.cuh:283-493) and blows a compressed crystal apart."""
     rng = np.random.default_rng(seed)
-    nx, ny, nz = nside
-    n = nx * ny * nz
+    r_eff = radius - overlap / 2.0
+    a = 2.0 * math.sqrt(2.0) * r_eff
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cx, cy, cz = cells
+    kk, jj, ii = np.meshgrid(np.arange(cz), np.arange(cy), np.arange(cx), indexing="ij")
+    corners = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float64)
+    pts = ((corners[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+    pts = pts[np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))]
+    pts[:, 0] += a / 4.0
+    pts[:, 1] += a / 4.0
+    pts[:, 2] += r_eff
+    n = pts.shape[0]
     sb = Spherebin(n, nw=1, sid=sid)
-    kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float64)
-    sb.x = (sites + 0.5) * spacing + rng.uniform(-jitter, jitter, size=(n, 3))
+    sb.x = pts + rng.uniform(-jitter, jitter, size=pts.shape)
     sb.radius = np.full(n, radius)
-    _contact_params(sb)
-    L = np.array([nx * spacing, ny * spacing, nz * spacing * 1.05])
+    _contact_params(sb, gamma_n=0.0)
+    top = float(pts[:, 2].max()) + r_eff
+    L = np.array([cx * a, cy * a, top * 1.05])
     _grid(sb, L, 2.0 * radius * 1.0001)
     sb.periodic = 1
-    sb.g = np.array([0.0, 0.0, -9.81])
+    sb.g = np.array([0.0, 0.0, gravity])
     sb.set_walls(1)
     sb.wmode = np.array([0], dtype=np.int32)
-    sb.w_x[0] = nz * spacing
+    sb.w_x[0] = top
     sb.w_m[0] = sb.rho * n * math.pi * (L[0] / sb.num[0] / 2.0) ** 3
     sb.time_dt = _dt(sb)
     sb.time_total = steps * sb.time_dt

@@ -11,10 +11,15 @@
 //   b  state at the start of a step, in the previous step's sorted order
 //   a  the same state gathered into this step's sorted order (reorder)
 //
-// The contact kernel reads a (own particle and neighbours) and writes the
-// integrated state back to b at the same sorted position, so neighbour reads
-// never race with updates.  Records are 32-byte double4 so each access fills
-// whole sectors.
+// Only the records other particles read (x, vel, angvel, id) are gathered
+// into a; the contact kernel reads them there (own particle and neighbours)
+// and writes the integrated values back to b at the same sorted position, so
+// neighbour reads never race with updates.  The records only the particle
+// itself reads (vel0, angvel0, angpos, the accumulators and the contact-slot
+// mask) are not gathered: the two sets' copies of them alternate between
+// steps, a step reading one set at the particle's previous position perm[s]
+// and writing the other at its new position s.  Records are 32-byte double4
+// so each access fills whole sectors.
 //
 // Contact history stays keyed by original index: a 32-bit occupancy mask
 // (moved with the particle) plus SPH_NC slots.  A slot word packs the
@@ -62,6 +67,12 @@ struct PState {
     uint32_t *cmask;  // occupied contact slots
 };
 
+// The particle-private records of one buffer set.
+struct OwnSet {
+    double4 *vel0, *angvel0, *angpos, *acc;
+    uint32_t* cmask;
+};
+
 // Everything a step kernel needs, passed by value as one kernel argument.
 struct DemDev {
     uint32_t np, nb0, nw, K;
@@ -77,6 +88,7 @@ struct DemDev {
     double lambda_bar, sigma_b, tau_b, devs_A, devs_f;
 
     PState a, b;
+    OwnSet ro, wo;      // private records: read at perm[s], written at s
     uint32_t* pos_of;   // original index -> sorted position of this step
 
     // per-step outputs, sorted order of the step
@@ -114,6 +126,7 @@ struct DemDev {
     uint32_t *bcsr_off, *bcsr_ent;  // per-particle bond lists, entry = k<<1 | side
 
     uint32_t* flags;
+    uint32_t* flags_next;  // cell-id errors of the next step's positions, merged at its start
 };
 
 // kernel launchers (kernels.hip); all enqueue on `s`

@@ -24,8 +24,14 @@ struct sph_dem {
     DemDev d{};
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // Captured steps, [lead][parity]: parity = the set holding the private
+    // records at the step's start; lead = 1 runs the previous step's wall
+    // update on a side stream beside the cell list and contact search.
+    hipGraph_t graph[2][2] = {};
+    hipGraphExec_t gexec[2][2] = {};
+    int own_cur = 0;  // 0: the current private records are in b, 1: in a
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool use_graph = true;
     int precision = SPH_PREC_FP64;
     int profiling = 0;
@@ -91,9 +97,17 @@ void unpad4(const std::vector<double>& a4, size_t n, double* out3) {
 }
 
 void destroy_graph(sph_dem* h) {
-    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    for (int l = 0; l < 2; ++l)
+        for (int k = 0; k < 2; ++k) {
+            if (h->gexec[l][k]) { (void)hipGraphExecDestroy(h->gexec[l][k]); h->gexec[l][k] = nullptr; }
+            if (h->graph[l][k]) { (void)hipGraphDestroy(h->graph[l][k]); h->graph[l][k] = nullptr; }
+        }
 }
+
+sph::OwnSet own_set(const sph::PState& p) { return {p.vel0, p.angvel0, p.angpos, p.acc, p.cmask}; }
+
+// the buffer set holding the current private records
+const sph::PState& own(const sph_dem* h) { return h->own_cur ? h->d.a : h->d.b; }
 
 void fill_params(sph_dem* h) {
     const HostState& s = h->hs;
@@ -218,6 +232,7 @@ int upload_all(sph_dem* h) {
     DemDev& d = h->d;
     const size_t n = s.np;
     // storage starts in original order: b holds the state, id and pos_of are the identity
+    h->own_cur = 0;
     std::vector<double> v4(4 * n), w4(4 * n), p4(4 * n), acc(4 * n), f4(4 * n), t4(4 * n);
     for (size_t i = 0; i < n; ++i) {
         for (int a = 0; a < 3; ++a) {
@@ -242,11 +257,11 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemcpy(d.b.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.b.vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
     // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel
-    HIP_TRY(hipMemcpy(d.b.vel0, s.vel.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).vel0, s.vel.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.b.angvel, w4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.b.angvel0, w4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.b.angpos, p4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.b.acc, acc.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).angvel0, w4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).angpos, p4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).acc, acc.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.b.id, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.a.id, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.pos_of, iota.data(), n * 4, hipMemcpyHostToDevice));
@@ -257,7 +272,7 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemcpy(d.rates, rates.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.cslot, 0, n * sph::NC * 4));
     HIP_TRY(hipMemset(d.cdt, 0, n * sph::NC * 3 * 8));
-    HIP_TRY(hipMemset(d.flags, 0, 4));
+    HIP_TRY(hipMemset(d.flags, 0, 8));
     HIP_TRY(hipMemset(d.key, 0, n * 4));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
     HIP_TRY(hipMemset(d.off, 0, ((size_t)d.K + 1) * 4));
@@ -360,7 +375,8 @@ int allocate(sph_dem* h) {
     ALLOC(d.bonds, nb); ALLOC(d.bdelta, nb); ALLOC(d.bomega, nb); ALLOC(d.bond_f, nb); ALLOC(d.bond_t, nb);
     ALLOC(d.bcsr_off, n + 1);
     d.bcsr_ent = nullptr;
-    ALLOC(d.flags, 1);
+    ALLOC(d.flags, 2);
+    d.flags_next = d.flags + 1;
     return SPH_OK;
 }
 
@@ -370,20 +386,40 @@ void free_all(sph_dem* h) {
         for (auto& e : h->ev) (void)hipEventDestroy(e);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+    if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+    h->fork_ev = h->join_ev = nullptr;
+    if (h->side) (void)hipStreamDestroy(h->side);
+    h->side = nullptr;
     if (h->stream) (void)hipStreamDestroy(h->stream);
     h->stream = nullptr;
 }
 
-void enqueue_step(sph_dem* h, bool timed) {
-    const DemDev& d = h->d;
+// Enqueues one step whose private records are read from set `parity`
+// (0: b, 1: a) and written to the other.  The wall update only feeds the next
+// step's force kernel: with `lead` the step starts with the previous step's
+// wall update on the side stream, joined before its own force kernel; with
+// `tail` it ends with its own wall update, otherwise that update is left
+// pending for the next step or the caller.
+void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail) {
+    DemDev d = h->d;
+    d.ro = own_set(parity ? h->d.a : h->d.b);
+    d.wo = own_set(parity ? h->d.b : h->d.a);
     hipStream_t s = h->stream;
+    if (lead) {
+        (void)hipEventRecord(h->fork_ev, s);
+        (void)hipStreamWaitEvent(h->side, h->fork_ev, 0);
+        sph::launch_walls(d, h->side);
+        (void)hipEventRecord(h->join_ev, h->side);
+    }
     if (timed) (void)hipEventRecord(h->ev[0], s);
     sph::launch_bonds(d, s);
     if (timed) (void)hipEventRecord(h->ev[1], s);
-    sph::launch_fill_u32(d.count, d.K, 0u, s);
-    sph::launch_cellid(d, s);
+    // cell ids and counts come from the previous step's force kernel (or
+    // prepare_keys), so the cell-id stage is empty
     if (timed) (void)hipEventRecord(h->ev[2], s);
     sph::launch_scan(d, s);
+    sph::launch_fill_u32(d.count, d.K, 0u, s);  // counts of the next step
     sph::launch_scatter(d, s);
     sph::launch_cellsort(d, s);
     if (timed) (void)hipEventRecord(h->ev[3], s);
@@ -391,20 +427,35 @@ void enqueue_step(sph_dem* h, bool timed) {
     if (timed) (void)hipEventRecord(h->ev[4], s);
     sph::launch_topology(d, s);
     if (timed) (void)hipEventRecord(h->ev[5], s);
+    if (lead) (void)hipStreamWaitEvent(s, h->join_ev, 0);
     sph::launch_forces(d, h->precision, s);
     if (timed) (void)hipEventRecord(h->ev[6], s);
-    sph::launch_walls(d, s);
+    if (tail) sph::launch_walls(d, s);
     if (timed) (void)hipEventRecord(h->ev[7], s);
 }
 
+// Cell ids and counts of the stored positions (after upload or sph_set of
+// positions): what a step's force kernel leaves for the next step.
+int prepare_keys(sph_dem* h) {
+    sph::launch_fill_u32(h->d.flags_next, 1, 0u, h->stream);
+    sph::launch_fill_u32(h->d.count, h->d.K, 0u, h->stream);
+    sph::launch_cellid(h->d, h->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SPH_OK;
+}
+
 int ensure_graph(sph_dem* h) {
-    if (h->gexec) return SPH_OK;
-    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(h, false);
-    hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamEndCapture(h->stream, &g));
-    h->graph = g;
-    HIP_TRY(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    for (int l = 0; l < 2; ++l)
+        for (int k = 0; k < 2; ++k) {
+            if (h->gexec[l][k]) continue;
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(h, false, k, l == 1, false);
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamEndCapture(h->stream, &g));
+            h->graph[l][k] = g;
+            HIP_TRY(hipGraphInstantiate(&h->gexec[l][k], g, nullptr, nullptr, 0));
+        }
     return SPH_OK;
 }
 
@@ -451,13 +502,13 @@ int download_all(sph_dem* h) {
     };
     if ((rc = fetch4(h, d.force, id, t))) return rc;
     unpad(s.force);
-    if ((rc = fetch4(h, d.b.angpos, id, t))) return rc;
+    if ((rc = fetch4(h, own(h).angpos, id, t))) return rc;
     unpad(s.angpos);
     if ((rc = fetch4(h, d.b.angvel, id, t))) return rc;
     unpad(s.angvel);
     if ((rc = fetch4(h, d.torque, id, t))) return rc;
     unpad(s.torque);
-    if ((rc = fetch4(h, d.b.acc, id, t))) return rc;
+    if ((rc = fetch4(h, own(h).acc, id, t))) return rc;
     for (size_t i = 0; i < n; ++i) {
         s.xysum[2 * i] = t[4 * i]; s.xysum[2 * i + 1] = t[4 * i + 1];
         s.es[i] = t[4 * i + 2]; s.ev[i] = t[4 * i + 3];
@@ -498,9 +549,13 @@ int open_common(sph_dem* h, int verbosity, int check_vals) {
     choose_tile(h);
     HIP_TRY(hipGetDevice(&h->device));
     HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming));
     if ((rc = allocate(h))) return rc;
     if ((rc = upload_all(h))) return rc;
     HIP_TRY(hipDeviceSynchronize());
+    if ((rc = prepare_keys(h))) return rc;
     if (verbosity > 0) {
         std::printf("sphere_dem: np=%u nb0=%u nw=%u grid=%ux%ux%u periodic=%d dt=%g\n", h->hs.np, h->hs.nb0,
                     h->hs.nw, h->hs.num[0], h->hs.num[1], h->hs.num[2], h->hs.periodic, h->hs.dt);
@@ -535,7 +590,8 @@ int run_steps(sph_dem* h, uint64_t n) {
         if ((rc = make_events(h))) return rc;
         sph_timing_t t{};
         for (uint64_t i = 0; i < n; ++i) {
-            enqueue_step(h, true);
+            enqueue_step(h, true, h->own_cur, false, true);
+            h->own_cur ^= 1;
             HIP_TRY(hipEventSynchronize(h->ev[7]));
             float ms[7];
             for (int k = 0; k < 7; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], h->ev[k], h->ev[k + 1]));
@@ -553,15 +609,26 @@ int run_steps(sph_dem* h, uint64_t n) {
         return check_flags(h);
     }
     if (h->use_graph && (rc = ensure_graph(h))) return rc;
+    // step i > 0 runs step i-1's wall update beside its cell list and search;
+    // the last step's wall update is enqueued before returning, also on error
+    bool pending = false;
     for (uint64_t i = 0; i < n; ++i) {
-        if (h->use_graph) HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
-        else enqueue_step(h, false);
+        const int lead = i > 0 ? 1 : 0;
+        if (h->use_graph) HIP_TRY(hipGraphLaunch(h->gexec[lead][h->own_cur], h->stream));
+        else enqueue_step(h, false, h->own_cur, lead == 1, false);
+        pending = true;
+        h->own_cur ^= 1;
         h->hs.t_cur += h->hs.dt;
         ++h->steps_done;
         if ((i + 1) % chunk == 0 && i + 1 < n) {
-            if ((rc = check_flags(h))) return rc;
+            if ((rc = check_flags(h))) {
+                sph::launch_walls(h->d, h->stream);
+                (void)hipStreamSynchronize(h->stream);
+                return rc;
+            }
         }
     }
+    if (pending) sph::launch_walls(h->d, h->stream);
     HIP_TRY(hipGetLastError());
     return check_flags(h);
 }
@@ -718,12 +785,12 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_ANGVEL: return gather(d.b.angvel, 0, 3);
         case SPH_FORCE: return gather(d.force, 0, 3);
         case SPH_TORQUE: return gather(d.torque, 0, 3);
-        case SPH_ANGPOS: return gather(d.b.angpos, 0, 3);
-        case SPH_VEL0: return gather(d.b.vel0, 0, 3);
-        case SPH_ANGVEL0: return gather(d.b.angvel0, 0, 3);
-        case SPH_XYSUM: return gather(d.b.acc, 0, 2);
-        case SPH_ES: return gather(d.b.acc, 2, 1);
-        case SPH_EV: return gather(d.b.acc, 3, 1);
+        case SPH_ANGPOS: return gather(own(h).angpos, 0, 3);
+        case SPH_VEL0: return gather(own(h).vel0, 0, 3);
+        case SPH_ANGVEL0: return gather(own(h).angvel0, 0, 3);
+        case SPH_XYSUM: return gather(own(h).acc, 0, 2);
+        case SPH_ES: return gather(own(h).acc, 2, 1);
+        case SPH_EV: return gather(own(h).acc, 3, 1);
         case SPH_ES_DOT: return gather(d.rates, 0, 1);
         case SPH_EV_DOT: return gather(d.rates, 1, 1);
         case SPH_P: return gather(d.rates, 2, 1);
@@ -759,7 +826,7 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_CONTACTS: case SPH_DISTMOD: case SPH_DELTA_T: {
             std::vector<uint32_t> smask(n), mask(n), slot(n * nc);
             if ((rc = fetch_ids(h, id))) return rc;
-            HIP_TRY(hipMemcpy(smask.data(), d.b.cmask, n * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(smask.data(), own(h).cmask, n * 4, hipMemcpyDeviceToHost));
             for (size_t q = 0; q < n; ++q) mask[id[q]] = smask[q];
             HIP_TRY(hipMemcpy(slot.data(), d.cslot, n * nc * 4, hipMemcpyDeviceToHost));
             auto occ = [&](size_t i, size_t k) { return ((mask[i] >> k) & 1u) != 0; };
@@ -831,17 +898,20 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
         return SPH_OK;
     };
     switch (f) {
-        case SPH_X: return scatter(d.b.x, 0, 4);
+        case SPH_X: {
+            const int r = scatter(d.b.x, 0, 4);
+            return r ? r : prepare_keys(h);
+        }
         case SPH_VEL: return scatter(d.b.vel, 0, 4);
         case SPH_ANGVEL: return scatter(d.b.angvel, 0, 3);
         case SPH_FORCE: return scatter(d.force, 0, 3);
         case SPH_TORQUE: return scatter(d.torque, 0, 3);
-        case SPH_ANGPOS: return scatter(d.b.angpos, 0, 3);
-        case SPH_VEL0: return scatter(d.b.vel0, 0, 3);
-        case SPH_ANGVEL0: return scatter(d.b.angvel0, 0, 3);
-        case SPH_XYSUM: return scatter(d.b.acc, 0, 2);
-        case SPH_ES: return scatter(d.b.acc, 2, 1);
-        case SPH_EV: return scatter(d.b.acc, 3, 1);
+        case SPH_ANGPOS: return scatter(own(h).angpos, 0, 3);
+        case SPH_VEL0: return scatter(own(h).vel0, 0, 3);
+        case SPH_ANGVEL0: return scatter(own(h).angvel0, 0, 3);
+        case SPH_XYSUM: return scatter(own(h).acc, 0, 2);
+        case SPH_ES: return scatter(own(h).acc, 2, 1);
+        case SPH_EV: return scatter(own(h).acc, 3, 1);
         case SPH_ES_DOT: return scatter(d.rates, 0, 1);
         case SPH_EV_DOT: return scatter(d.rates, 1, 1);
         case SPH_P: return scatter(d.rates, 2, 1);

@@ -4,8 +4,8 @@
 //   bonds      one thread per bond: parallel-bond force, torque, breakage
 //              (src/cohesion.cuh:8-233); forces go to per-bond records, no
 //              scatter
-//   cellid     one thread per particle: periodic wrap, cell id, atomic
-//              count (src/sorting.cuh:7-43)
+//   (cell ids  the previous step's force kernel computed them with the
+//              per-cell counts, src/sorting.cuh:7-43)
 //   scan       exclusive scan of the per-cell counts -> cell offsets
 //   scatter    counting-sort placement of the particles
 //   cellsort   one thread per cell: order each cell's members by original
@@ -97,19 +97,30 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint64_t n, uin
     if (i < n) p[i] = v;
 }
 
+// Cell id of a position at the start of a step: the periodic wrap the step's
+// integrate applies to it (src/integration.cuh:80-95) on a copy, then
+// src/sorting.cuh:7-19.  The position itself is wrapped when it is gathered
+// into the sorted order.
+__device__ __forceinline__ uint32_t cell_key(const DemDev& d, double x, double y, double z, uint32_t& err) {
+    (void)wrap_xy(d, x, y);
+    const int ix = cell_coord(d, x, 0, err);
+    const int iy = cell_coord(d, y, 1, err);
+    const int iz = cell_coord(d, z, 2, err);
+    return ((uint32_t)iz * d.num[1]) * d.num[0] + (uint32_t)iy * d.num[0] + (uint32_t)ix;
+}
+
+// Cell ids and per-cell counts of the stored positions, for a state that did
+// not come out of a step (after open or a position update); steps compute the
+// next step's ids in the force kernel.
 __global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
-    double4 x = d.b.x[i];
-    if (wrap_xy(d, x.x, x.y)) d.b.x[i] = x;
+    const double4 x = d.b.x[i];
     uint32_t err = 0;
-    const int ix = cell_coord(d, x.x, 0, err);
-    const int iy = cell_coord(d, x.y, 1, err);
-    const int iz = cell_coord(d, x.z, 2, err);
-    const uint32_t key = ((uint32_t)iz * d.num[1]) * d.num[0] + (uint32_t)iy * d.num[0] + (uint32_t)ix;
+    const uint32_t key = cell_key(d, x.x, x.y, x.z, err);
     d.key[i] = key;
     d.rank[i] = atomicAdd(&d.count[key], 1u);
-    if (err) atomicOr(d.flags, err);
+    if (err) atomicOr(d.flags_next, err);
 }
 
 // -------------------------------------------------------------------- scan
@@ -150,6 +161,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
 
 __global__ __launch_bounds__(BLOCK) void k_scan_top(DemDev d) {
     __shared__ uint32_t lds[BLOCK / 64];
+    // the cell-id errors belong to this step
+    if (threadIdx.x == 0) {
+        const uint32_t f = *d.flags_next;
+        if (f) {
+            atomicOr(d.flags, f);
+            *d.flags_next = 0u;
+        }
+    }
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < d.scan_blocks; b0 += BLOCK) {
         const uint32_t i = b0 + threadIdx.x;
@@ -236,14 +255,11 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
     if (q >= d.np) return;
     const uint32_t p = d.perm[q];
-    d.a.x[q] = d.b.x[p];
+    double4 x = d.b.x[p];
+    (void)wrap_xy(d, x.x, x.y);
+    d.a.x[q] = x;
     d.a.vel[q] = d.b.vel[p];
     d.a.angvel[q] = d.b.angvel[p];
-    d.a.vel0[q] = d.b.vel0[p];
-    d.a.angvel0[q] = d.b.angvel0[p];
-    d.a.angpos[q] = d.b.angpos[p];
-    d.a.acc[q] = d.b.acc[p];
-    d.a.cmask[q] = d.b.cmask[p];
     const uint32_t id = d.b.id[p];
     d.a.id[q] = id;
     d.pos_of[id] = q;
@@ -496,7 +512,7 @@ __device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& s
         const uint32_t o = d.a.id[s];
         uint32_t* slot = d.cslot + (size_t)o * NC;
         SlotState st;
-        st.mask = d.a.cmask[s];
+        st.mask = d.ro.cmask[d.perm[s]];
         st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false;
         for (int lay = 0; lay < 3 && !st.stop; ++lay) {
             for (int dy = 0; dy < 3 && !st.stop; ++dy) {
@@ -617,7 +633,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     const uint32_t o = d.a.id[s];
     uint32_t* slot = d.cslot + (size_t)o * NC;
     SlotState st;
-    st.mask = d.a.cmask[s];
+    st.mask = d.ro.cmask[d.perm[s]];
     st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false;
     for (int dz = -1; dz <= 1 && !st.stop; ++dz) {
         const int tz = gz + dz;
@@ -827,7 +843,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
         const uint32_t o = d.a.id[m];
         const uint32_t* slot = d.cslot + (size_t)o * NC;
         MemberState st;
-        st.mask = d.a.cmask[m];
+        st.mask = d.ro.cmask[d.perm[m]];
         st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false; st.loaded = false;
         uint32_t w = 0;
         bool dirty = false;
@@ -920,7 +936,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
 
     // ---- per-particle outputs (src/contactsearch.cuh:669-687)
     const double dt = d.dt;
-    double4 ea = d.a.acc[s];  // xysum.x, xysum.y, es, ev
+    const uint32_t prev = d.perm[s];  // position of the private records
+    double4 ea = d.ro.acc[prev];  // xysum.x, xysum.y, es, ev
     ea.z += acc.es_dot * dt;
     ea.w += acc.ev_dot * dt;
     d.rates[s] = make_double4(acc.es_dot, acc.ev_dot, acc.p, 0.0);
@@ -960,7 +977,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     }
     double px = xa.x, py = xa.y, pz = xa.z;
     (void)wrap_xy(d, px, py);
-    double4 v0 = d.a.vel0[s], w0 = d.a.angvel0[s], ap = d.a.angpos[s];
+    double4 v0 = d.ro.vel0[prev], w0 = d.ro.angvel0[prev], ap = d.ro.angpos[prev];
     v0.x += ax * dt; v0.y += ay * dt; v0.z += az * dt;
     w0.x += bx * dt; w0.y += by * dt; w0.z += bz * dt;
     px += v0.x * dt; py += v0.y * dt; pz += v0.z * dt;
@@ -972,15 +989,22 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     sy += vy * dt;
     ea.x += sx;
     ea.y += sy;
-    d.b.acc[s] = ea;
+    d.wo.acc[s] = ea;
     d.b.angvel[s] = make_double4(wx, wy, wz, 0.0);
-    d.b.angvel0[s] = w0;
+    d.wo.angvel0[s] = w0;
     d.b.vel[s] = make_double4(vx, vy, vz, va4.w);
-    d.b.vel0[s] = v0;
-    d.b.angpos[s] = ap;
+    d.wo.vel0[s] = v0;
+    d.wo.angpos[s] = ap;
     d.b.x[s] = make_double4(px, py, pz, radius);
     d.b.id[s] = o;
-    d.b.cmask[s] = mask;
+    d.wo.cmask[s] = mask;
+
+    // cell id and count of the new position for the next step's sort
+    uint32_t cerr = 0;
+    const uint32_t nkey = cell_key(d, px, py, pz, cerr);
+    d.key[s] = nkey;
+    d.rank[s] = atomicAdd(&d.count[nkey], 1u);
+    if (cerr) atomicOr(d.flags_next, cerr);
 }
 
 // ------------------------------------------------------------------- walls
