@@ -108,6 +108,9 @@ struct DemDev {
     uint32_t tile_x, tile_y, tiles_x, tiles_y;
     int search_mode;    // 1: LDS tiles, 2: one thread per sorted particle, HBM/L2 reads
     uint32_t* skey;     // cell id per sorted position
+    float4* xf;         // FP32 copy of a.x for the search's distance prefilter
+    float search_eps;   // prefilter margin (m), above the FP32 rounding of a distance
+    float Lf[2];        // L[0], L[1] in FP32 (periodic images in the prefilter)
     uint32_t* tmask;    // slot occupancy after the search, sorted order
     uint32_t* tfresh;   // slots taken from empty by the search, sorted order
 
@@ -132,7 +135,8 @@ struct DemDev {
 // kernel launchers (kernels.hip); all enqueue on `s`
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 void launch_bonds(const DemDev& d, hipStream_t s);
-void launch_cellid(const DemDev& d, hipStream_t s);
+void launch_cellid(const DemDev& d, hipStream_t s);  // keys of the stored positions
+void launch_count(const DemDev& d, hipStream_t s);   // per-cell counts and ranks from the keys
 void launch_scan(const DemDev& d, hipStream_t s);
 void launch_scatter(const DemDev& d, hipStream_t s);
 void launch_cellsort(const DemDev& d, hipStream_t s);

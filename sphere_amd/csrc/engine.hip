@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -108,6 +109,22 @@ sph::OwnSet own_set(const sph::PState& p) { return {p.vel0, p.angvel0, p.angpos,
 
 // the buffer set holding the current private records
 const sph::PState& own(const sph_dem* h) { return h->own_cur ? h->d.a : h->d.b; }
+
+// Margin of the search's FP32 distance prefilter: every coordinate is below
+// M = max |origo| + 2 L in magnitude, so converting, subtracting and shifting
+// by a periodic image errs by at most 3.5 ulp(M) per axis and 6.1 ulp(M) in
+// the distance; 16 ulp(M) plus 1e-6 of the largest radius sum also covers
+// the FP32 rounding of the squares.
+void update_search_eps(sph_dem* h) {
+    const HostState& s = h->hs;
+    double M = 0.0, rmax = 0.0;
+    for (int a = 0; a < 3; ++a) M = std::max(M, std::fabs(s.origo[a]) + 2.0 * std::fabs(s.L[a]));
+    for (uint32_t i = 0; i < s.np; ++i) rmax = std::max(rmax, s.x[4 * (size_t)i + 3]);
+    const double ulp = std::ldexp(1.0, std::ilogb(std::max(M, 1e-30)) - 23);
+    h->d.search_eps = (float)(16.0 * ulp + 2e-6 * rmax);
+    h->d.Lf[0] = (float)s.L[0];
+    h->d.Lf[1] = (float)s.L[1];
+}
 
 void fill_params(sph_dem* h) {
     const HostState& s = h->hs;
@@ -366,7 +383,7 @@ int allocate(sph_dem* h) {
     ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
     d.cdt32 = nullptr;
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
-    ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n);
+    ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
     ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
@@ -415,11 +432,10 @@ void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail) {
     if (timed) (void)hipEventRecord(h->ev[0], s);
     sph::launch_bonds(d, s);
     if (timed) (void)hipEventRecord(h->ev[1], s);
-    // cell ids and counts come from the previous step's force kernel (or
-    // prepare_keys), so the cell-id stage is empty
+    // cell ids come from the previous step's force kernel (or prepare_keys)
+    sph::launch_count(d, s);
     if (timed) (void)hipEventRecord(h->ev[2], s);
     sph::launch_scan(d, s);
-    sph::launch_fill_u32(d.count, d.K, 0u, s);  // counts of the next step
     sph::launch_scatter(d, s);
     sph::launch_cellsort(d, s);
     if (timed) (void)hipEventRecord(h->ev[3], s);
@@ -434,11 +450,10 @@ void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail) {
     if (timed) (void)hipEventRecord(h->ev[7], s);
 }
 
-// Cell ids and counts of the stored positions (after upload or sph_set of
-// positions): what a step's force kernel leaves for the next step.
+// Cell ids of the stored positions (after upload or sph_set of positions):
+// what a step's force kernel leaves for the next step.
 int prepare_keys(sph_dem* h) {
     sph::launch_fill_u32(h->d.flags_next, 1, 0u, h->stream);
-    sph::launch_fill_u32(h->d.count, h->d.K, 0u, h->stream);
     sph::launch_cellid(h->d, h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -546,6 +561,7 @@ int open_common(sph_dem* h, int verbosity, int check_vals) {
     if (check_vals == 1 && (rc = sph::check_values(h->hs, h->err))) return rc;
     if ((rc = validate_for_device(h))) return rc;
     fill_params(h);
+    update_search_eps(h);
     choose_tile(h);
     HIP_TRY(hipGetDevice(&h->device));
     HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -900,7 +916,11 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
     switch (f) {
         case SPH_X: {
             const int r = scatter(d.b.x, 0, 4);
-            return r ? r : prepare_keys(h);
+            if (r) return r;
+            std::memcpy(h->hs.x.data(), sd, n * 32);  // radii for the search margin
+            update_search_eps(h);
+            destroy_graph(h);
+            return prepare_keys(h);
         }
         case SPH_VEL: return scatter(d.b.vel, 0, 4);
         case SPH_ANGVEL: return scatter(d.b.angvel, 0, 3);

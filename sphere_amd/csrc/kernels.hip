@@ -109,18 +109,42 @@ __device__ __forceinline__ uint32_t cell_key(const DemDev& d, double x, double y
     return ((uint32_t)iz * d.num[1]) * d.num[0] + (uint32_t)iy * d.num[0] + (uint32_t)ix;
 }
 
-// Cell ids and per-cell counts of the stored positions, for a state that did
-// not come out of a step (after open or a position update); steps compute the
-// next step's ids in the force kernel.
+// Cell ids of the stored positions, for a state that did not come out of a
+// step (after open or a position update); steps compute the next step's ids
+// in the force kernel.
 __global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
     const double4 x = d.b.x[i];
     uint32_t err = 0;
-    const uint32_t key = cell_key(d, x.x, x.y, x.z, err);
-    d.key[i] = key;
-    d.rank[i] = atomicAdd(&d.count[key], 1u);
+    d.key[i] = cell_key(d, x.x, x.y, x.z, err);
     if (err) atomicOr(d.flags_next, err);
+}
+
+// Per-cell counts and each particle's rank among its cell's members (in any
+// order: the cell sort orders members by original index afterwards).
+// Storage is the previous step's sorted order, so a wavefront's keys come in
+// runs; one atomic per run.
+__global__ __launch_bounds__(BLOCK) void k_count(DemDev d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool act = i < d.np;
+    const uint32_t key = act ? d.key[i] : 0xffffffffu;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = act && (lane == 0u || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t acts = __ballot(act);
+    if (!act) return;
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 for lane 63)
+    const uint32_t start = 63u - (uint32_t)__clzll((long long)(heads & upto));
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t above = heads & ~upto;
+        const uint32_t end = above ? (uint32_t)__ffsll((long long)above) - 1u : 64u - (uint32_t)__clzll((long long)acts);
+        base = atomicAdd(&d.count[key], end - lane);
+    }
+    base = __shfl(base, (int)start, 64);
+    d.rank[i] = base + (lane - start);
 }
 
 // -------------------------------------------------------------------- scan
@@ -258,6 +282,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     double4 x = d.b.x[p];
     (void)wrap_xy(d, x.x, x.y);
     d.a.x[q] = x;
+    d.xf[q] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
     d.a.vel[q] = d.b.vel[p];
     d.a.angvel[q] = d.b.angvel[p];
     const uint32_t id = d.b.id[p];
@@ -621,20 +646,104 @@ __global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
 
 // One thread per sorted particle; the 27 cells are walked as 9 runs of the
 // sorted arrays (cells x-1, x, x+1 of a row have consecutive ids), the same
-// candidate order as k_topo, candidates read through L1/L2.
+// candidate order as k_topo, candidates read through L1/L2.  The hot loop
+// reads the FP32 copy of the positions and keeps only pairs within
+// (r_a + r_b + search_eps), a margin larger than the FP32 rounding of the
+// distance, so it never drops an overlap; the reference's exact test
+// sqrt(d2) - (r_a + r_b) < 0 then runs in FP64 on the few that remain
+// (rows_try).  The particle's slot words are cached in LDS from its first
+// overlap on and written back once.
+struct RowSlots {
+    uint32_t mask, fresh, nc, err, dirty;
+    bool stop, loaded;
+};
+
+// (Takes plain pointers rather than the DemDev, which a non-inlined call
+// would have to copy to scratch memory.)
+__device__ __noinline__ void rows_try(const double4* __restrict__ ax, const uint32_t* __restrict__ aid, double L0,
+                                      double L1, uint32_t* sw, const uint32_t* gslot, RowSlots& st, double4 xa,
+                                      uint32_t sb, uint32_t code) {
+    const double4 xb = ax[sb];
+    const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
+    const double dmx = cx == 1u ? L0 : (cx == 2u ? -L0 : 0.0);
+    const double dmy = cy == 1u ? L1 : (cy == 2u ? -L1 : 0.0);
+    const double ex = (xa.x - xb.x) + dmx, ey = (xa.y - xb.y) + dmy, ez = xa.z - xb.z;
+    // ((e + 0.0)^2 == e^2: d2 equals dot(xab, xab) of src/contactsearch.cuh:242-248)
+    const double d2 = ex * ex + ey * ey + ez * ez;
+    if (!(sqrt(d2) - (xa.w + xb.w) < 0.0)) return;
+    if (!st.loaded) {
+        for (uint32_t m = st.mask; m; m &= m - 1) {
+            const int i = __ffs(m) - 1;
+            sw[i * BLOCK] = gslot[i];
+        }
+        st.loaded = true;
+    }
+    // slot rule of src/contactsearch.cuh:252-271
+    const uint32_t bo = aid[sb];
+    int cpos = -1;
+    for (uint32_t m = st.mask; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        if ((sw[i * BLOCK] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+    }
+    if (cpos < 0) {
+        const uint32_t empty = ~st.mask;
+        if (empty == 0u) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; return; }
+        cpos = 31 - __clz(empty);
+        st.fresh |= 1u << cpos;
+    }
+    sw[cpos * BLOCK] = bo | (code << 28);
+    st.dirty |= 1u << cpos;
+    st.mask |= 1u << cpos;
+    if (++st.nc > (uint32_t)NC) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; }
+}
+
+// Candidates [lo, hi) of one run with periodic image (dmx, dmy), in batches
+// of ROW_BATCH independent loads; overlaps go to rows_try in candidate order.
+constexpr int ROW_BATCH = 8;
+
+__device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t hi, float dmx, float dmy,
+                                         uint32_t code, uint32_t s, float4 xaf, float ra_eps, double4 xa,
+                                         uint32_t* sw, const uint32_t* gslot, RowSlots& st) {
+    for (uint32_t sb0 = lo; sb0 < hi; sb0 += ROW_BATCH) {
+        float4 c[ROW_BATCH];
+#pragma unroll
+        for (int k = 0; k < ROW_BATCH; ++k)
+            c[k] = sb0 + k < hi ? d.xf[sb0 + k] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+        uint32_t hits = 0;
+#pragma unroll
+        for (int k = 0; k < ROW_BATCH; ++k) {
+            const float ex = (xaf.x - c[k].x) + dmx, ey = (xaf.y - c[k].y) + dmy, ez = xaf.z - c[k].z;
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            const float rs = ra_eps + c[k].w;
+            hits |= d2 <= rs * rs ? 1u << k : 0u;
+        }
+        if (__builtin_expect(hits != 0u, 0)) {
+            for (; hits; hits &= hits - 1) {
+                const uint32_t sb = sb0 + (uint32_t)(__ffs(hits) - 1);
+                if (sb == s) continue;
+                rows_try(d.a.x, d.a.id, d.L[0], d.L[1], sw, gslot, st, xa, sb, code);
+                if (st.stop) return;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
+    __shared__ uint32_t sw_all[NC * BLOCK];  // slot i of thread t at [i * BLOCK + t]
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
+    uint32_t* sw = sw_all + threadIdx.x;
     const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
     const uint32_t key = d.skey[s];
     const int gx = (int)(key % d.num[0]), gy = (int)((key / d.num[0]) % d.num[1]), gz = (int)(key / (d.num[0] * d.num[1]));
     const double4 xa = d.a.x[s];
-    const double ra = xa.w;
+    const float4 xaf = d.xf[s];
+    const float ra_eps = xaf.w + d.search_eps;
     const uint32_t o = d.a.id[s];
-    uint32_t* slot = d.cslot + (size_t)o * NC;
-    SlotState st;
+    uint32_t* gslot = d.cslot + (size_t)o * NC;
+    RowSlots st;
     st.mask = d.ro.cmask[d.perm[s]];
-    st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false;
+    st.fresh = 0; st.nc = 0; st.err = 0; st.dirty = 0; st.stop = false; st.loaded = false;
     for (int dz = -1; dz <= 1 && !st.stop; ++dz) {
         const int tz = gz + dz;
         if (tz < 0 || tz == nz) continue;
@@ -649,20 +758,8 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
             const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
             if (gx >= 1 && gx + 1 < nx && cy == 0) {
                 // no periodic image anywhere in the run: one contiguous range
-                const uint32_t lo = d.off[row + gx - 1], hi = d.off[row + gx + 2];
-                for (uint32_t sb = lo; sb < hi; ++sb) {
-                    const double4 xb = d.a.x[sb];
-                    const double ex = xa.x - xb.x, ey = xa.y - xb.y, ez = xa.z - xb.z;
-                    // ((e + 0.0)^2 == e^2: d2 equals dot(xab, xab) of the exact test)
-                    const double d2 = ex * ex + ey * ey + ez * ez;
-                    const double rsum = ra + xb.w;
-                    if (__builtin_expect(d2 <= rsum * rsum * 1.0000001, 0) && sb != s) {
-                        if (sqrt(d2) - rsum < 0.0) {
-                            slot_insert(slot, st, d.a.id[sb], 0u);
-                            if (st.stop) break;
-                        }
-                    }
-                }
+                rows_run(d, d.off[row + gx - 1], d.off[row + gx + 2], 0.0f, 0.0f, 0u, s, xaf, ra_eps, xa, sw,
+                         gslot, st);
                 continue;
             }
             for (int dx = -1; dx <= 1 && !st.stop; ++dx) {
@@ -673,24 +770,16 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
                     cx = tx < 0 ? 1u : 2u;
                     tx = tx < 0 ? nx - 1 : 0;
                 }
-                const uint32_t code = cx | (cy << 2);
-                double dmx, dmy;
-                decode_image(d, code, dmx, dmy);
-                const uint32_t lo = d.off[row + tx], hi = d.off[row + tx + 1];
-                for (uint32_t sb = lo; sb < hi; ++sb) {
-                    const double4 xb = d.a.x[sb];
-                    const double ex = (xa.x - xb.x) + dmx, ey = (xa.y - xb.y) + dmy, ez = xa.z - xb.z;
-                    const double d2 = ex * ex + ey * ey + ez * ez;
-                    const double rsum = ra + xb.w;
-                    if (__builtin_expect(d2 <= rsum * rsum * 1.0000001, 0) && sb != s) {
-                        if (sqrt(d2) - rsum < 0.0) {
-                            slot_insert(slot, st, d.a.id[sb], code);
-                            if (st.stop) break;
-                        }
-                    }
-                }
+                const float dmx = cx == 1u ? d.Lf[0] : (cx == 2u ? -d.Lf[0] : 0.0f);
+                const float dmy = cy == 1u ? d.Lf[1] : (cy == 2u ? -d.Lf[1] : 0.0f);
+                rows_run(d, d.off[row + tx], d.off[row + tx + 1], dmx, dmy, cx | (cy << 2), s, xaf, ra_eps, xa,
+                         sw, gslot, st);
             }
         }
+    }
+    for (uint32_t m = st.dirty; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        gslot[i] = sw[i * BLOCK];
     }
     d.tmask[s] = st.mask;
     d.tfresh[s] = st.fresh;
@@ -999,11 +1088,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     d.b.id[s] = o;
     d.wo.cmask[s] = mask;
 
-    // cell id and count of the new position for the next step's sort
+    // cell id of the new position for the next step's sort
     uint32_t cerr = 0;
-    const uint32_t nkey = cell_key(d, px, py, pz, cerr);
-    d.key[s] = nkey;
-    d.rank[s] = atomicAdd(&d.count[nkey], 1u);
+    d.key[s] = cell_key(d, px, py, pz, cerr);
     if (cerr) atomicOr(d.flags_next, cerr);
 }
 
@@ -1096,6 +1183,10 @@ void launch_bonds(const DemDev& d, hipStream_t s) {
     if (d.nb0) k_bonds<<<nblk(d.nb0), BLOCK, 0, s>>>(d);
 }
 void launch_cellid(const DemDev& d, hipStream_t s) { k_cellid<<<nblk(d.np), BLOCK, 0, s>>>(d); }
+void launch_count(const DemDev& d, hipStream_t s) {
+    k_fill_u32<<<nblk(d.K), BLOCK, 0, s>>>(d.count, d.K, 0u);
+    k_count<<<nblk(d.np), BLOCK, 0, s>>>(d);
+}
 void launch_scan(const DemDev& d, hipStream_t s) {
     k_scan_reduce<<<d.scan_blocks, BLOCK, 0, s>>>(d);
     k_scan_top<<<1, BLOCK, 0, s>>>(d);
