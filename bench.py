@@ -11,7 +11,7 @@ contact forces + integration + next cell ids, wall update) with all state
 resident in HBM.  Prints ONE JSON line on stdout.
 
 Also measured: the mixed-precision mode on the same input, and both modes on
-a dense 1,048,576-sphere lattice (about six contacts per sphere), with the
+a dense 1,048,576-sphere FCC crystal (twelve contacts per sphere), with the
 mixed mode's error against the FP64 path per output.  Per-kernel time comes
 from hipEvents recorded on the simulation's own stream around each stage,
 over a separate run of the same steps; the roofline entry is for the stage
@@ -39,10 +39,10 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 1M spheres (FP64 vs mixed); adjoint/forward step time ratio"
 STAGES = ("bonds", "cellid", "sort", "reorder", "search", "force", "walls")
 STAGE_KERNELS = {
-    "bonds": "k_bonds", "cellid": "(in k_force)", "sort": "k_scan_*/k_fill_u32/k_scatter/k_cellsort",
-    "reorder": "k_reorder", "search": "k_topo_wave", "force": "k_force", "walls": "k_wallsum/k_walls",
+    "bonds": "k_bonds", "cellid": "k_fill_u32/k_count (ids from k_force)", "sort": "k_scan_*/k_scatter/k_cellsort",
+    "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
-PROFILE_KERNEL = {"reorder": "k_reorder", "search": "k_topo_wave", "force": "k_force<double>"}
+PROFILE_KERNEL = {"reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force<double>", "cellid": "k_count"}
 
 
 def log(msg):
@@ -55,7 +55,7 @@ def kernel_bytes(n, c, b, w, k):
     neighbour and bond-endpoint gathers counted as cache resident."""
     return {
         "bonds": 136 * b,                          # pair 8, delta 32, omega 32 read; delta, omega 64 write
-        "cellid": 0,                               # computed by the force kernel (counted there)
+        "cellid": 12 * n,                          # count: key read, rank write, one atomic per run (ids come from force)
         "sort": 16 * n + 8 * k,                    # key + index read and write, cell bounds
         "reorder": 184 * n,                        # key, index, 11 doubles gathered and written
         "search": 32 * n + 32 * c,                 # own x,y,z,r; old + new slot, image per active slot
@@ -235,8 +235,8 @@ def main():
         sbd.writebin(dpath)
         dn = sbd.np
         d64 = timed_run(dpath, "fp64", args.steps, args.warmup, prof_steps)
-        dense = {"workload": "1,048,576-sphere simple-cubic lattice, spacing 0.99 diameter, x/y periodic, "
-                             "fixed top wall (every nearest-neighbour pair in contact)",
+        dense = {"workload": "1,048,576-sphere FCC crystal (4 x 64^3), nearest neighbours overlapping by 2e-5 m, "
+                             "x/y periodic, floor and fixed top wall, gamma_n = 0",
                  "contacts_per_particle": d64["contacts"] / dn, "fp64": {"value": dn / (d64["ms"] * 1e-3),
                  "ms_per_step": d64["ms"], "stage_ms": d64["stage_ms"]}}
         log(f"dense fp64: {d64['ms']:.4f} ms/step, C/N={d64['contacts'] / dn:.2f}")

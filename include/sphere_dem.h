@@ -181,6 +181,23 @@ int sph_set_search(sph_t* h, int mode);
  * (k_copy_probe) and returns the achieved read+write rate in GB/s. */
 int sph_bandwidth_probe(uint64_t bytes, int reps, double* gbs);
 
+/* Adjoint of the time step, with no counterpart in the reference: gradients
+ * of a time-mean force objective
+ *     J = 1/nsteps sum_{steps} sum_i w_i . F_i
+ * (F_i the particle's total force of the step, w_i = weights[3i..3i+2] in
+ * the file's particle order; e.g. w_i = (-1/(L0 L1), 0, 0) on the driven
+ * layer gives the mean shear stress) with respect to
+ * theta = (k_n, k_t, gamma_n, mu), mu = mu_s = mu_d.
+ * sph_adjoint_forward runs nsteps steps from the current state, keeping a
+ * device checkpoint every seg_len steps, and returns J.  The handle is left
+ * at the end of the run.  sph_adjoint_backward recomputes each segment from
+ * its checkpoint with a per-step tape (about 1 KB per particle and step) and
+ * runs the discrete adjoint backwards; grad[4] = dJ/dtheta.  Contact set and
+ * force-law branches are those of the forward run.  Handles with bonds
+ * return SPH_E_UNSUPPORTED. */
+int sph_adjoint_forward(sph_t* h, uint64_t nsteps, uint32_t seg_len, const double* weights, double* J);
+int sph_adjoint_backward(sph_t* h, double grad[4]);
+
 /* Setup helper (host only): seeded non-overlapping random sequential packing
  * of n spheres with the given radii in box[3] (periodic: 0 none, 1 x+y, 2 x),
  * largest first; xyz[3n] receives the centres.  Stands in for the

@@ -26,7 +26,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["kernels.hip", "engine.hip", "binio.cpp", "generate.cpp"]
+SOURCES = ["kernels.hip", "engine.hip", "adjoint.hip", "binio.cpp", "generate.cpp"]
 
 
 def _compile(src: str) -> str:

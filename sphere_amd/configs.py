@@ -253,6 +253,28 @@ def dense_lattice(cells=(64, 64, 64), radius=1e-2, overlap=2e-5, jitter=1e-6, gr
     return sb
 
 
+def fcc_shear(cells=(16, 16, 64), shear_velocity=0.1, stress=10e3, gamma_n=0.0, sid="fcc-shear",
+              **kw) -> Spherebin:
+    """Dense shear cell: the confined FCC crystal of dense_lattice with its
+    top layer driven at a fixed horizontal velocity and its bottom layer held
+    (fixvel, python/sphere.py:1297-1356), under a stress-controlled top wall.
+    4 x 16 x 16 x 64 = 65,536 spheres by default.  Stands in for a compacted
+    random packing, which the seeded random placement cannot produce without
+    a long settling run."""
+    sb = dense_lattice(cells=cells, sid=sid, **kw)
+    sb.gamma_n = float(gamma_n)
+    sb.gamma_wn = 0.0
+    z = sb.x[:, 2]
+    a = 2.0 * math.sqrt(2.0) * float(sb.radius[0])
+    top = z > z.max() - 0.25 * a
+    bot = z < z.min() + 0.25 * a
+    sb.fixvel[top | bot] = 1.0
+    sb.vel[top, 0] = shear_velocity
+    sb.wmode = np.array([1], dtype=np.int32)
+    sb.w_devs[0] = float(stress)
+    return sb
+
+
 def two_spheres(distance=0.0, radius=0.5, v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0), bonded=False,
                 gamma=0.0, k=1.16e9, mu=0.4, L=20.0, periodic=1, gravity=0.0, sid="pair") -> Spherebin:
     """Two spheres on the x axis, the layout of tests/bond_tests.py:42-50."""

@@ -16,34 +16,12 @@
 
 #include "binio.h"
 #include "dem_internal.h"
+#include "engine_internal.h"
 
 using sph::DemDev;
 using sph::HostState;
+using sph::StepHooks;
 
-struct sph_dem {
-    HostState hs;
-    DemDev d{};
-    hipStream_t stream = nullptr;
-    std::vector<void*> allocs;
-    // Captured steps, [lead][parity]: parity = the set holding the private
-    // records at the step's start; lead = 1 runs the previous step's wall
-    // update on a side stream beside the cell list and contact search.
-    hipGraph_t graph[2][2] = {};
-    hipGraphExec_t gexec[2][2] = {};
-    int own_cur = 0;  // 0: the current private records are in b, 1: in a
-    hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-    bool use_graph = true;
-    int precision = SPH_PREC_FP64;
-    int profiling = 0;
-    int verbosity = 0;
-    sph_timing_t timing{};
-    hipEvent_t ev[8] = {};
-    bool events_made = false;
-    std::string err;
-    uint64_t steps_done = 0;
-    int device = 0;
-};
 
 namespace {
 
@@ -412,13 +390,13 @@ void free_all(sph_dem* h) {
     h->stream = nullptr;
 }
 
-// Enqueues one step whose private records are read from set `parity`
-// (0: b, 1: a) and written to the other.  The wall update only feeds the next
-// step's force kernel: with `lead` the step starts with the previous step's
-// wall update on the side stream, joined before its own force kernel; with
-// `tail` it ends with its own wall update, otherwise that update is left
-// pending for the next step or the caller.
-void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail) {
+}  // namespace
+
+// The wall update only feeds the next step's force kernel: with `lead` the
+// step starts with the previous step's wall update on the side stream, joined
+// before its own force kernel; with `tail` it ends with its own wall update,
+// otherwise that update is left pending for the next step or the caller.
+void sph::enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail, const StepHooks* hooks) {
     DemDev d = h->d;
     d.ro = own_set(parity ? h->d.a : h->d.b);
     d.wo = own_set(parity ? h->d.b : h->d.a);
@@ -443,12 +421,16 @@ void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail) {
     if (timed) (void)hipEventRecord(h->ev[4], s);
     sph::launch_topology(d, s);
     if (timed) (void)hipEventRecord(h->ev[5], s);
+    if (hooks && hooks->before_force) hooks->before_force(d, s);
     if (lead) (void)hipStreamWaitEvent(s, h->join_ev, 0);
     sph::launch_forces(d, h->precision, s);
+    if (hooks && hooks->after_force) hooks->after_force(d, s);
     if (timed) (void)hipEventRecord(h->ev[6], s);
     if (tail) sph::launch_walls(d, s);
     if (timed) (void)hipEventRecord(h->ev[7], s);
 }
+
+namespace {
 
 // Cell ids of the stored positions (after upload or sph_set of positions):
 // what a step's force kernel leaves for the next step.
@@ -465,7 +447,7 @@ int ensure_graph(sph_dem* h) {
         for (int k = 0; k < 2; ++k) {
             if (h->gexec[l][k]) continue;
             HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(h, false, k, l == 1, false);
+            sph::enqueue_step(h, false, k, l == 1, false);
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamEndCapture(h->stream, &g));
             h->graph[l][k] = g;
@@ -606,7 +588,7 @@ int run_steps(sph_dem* h, uint64_t n) {
         if ((rc = make_events(h))) return rc;
         sph_timing_t t{};
         for (uint64_t i = 0; i < n; ++i) {
-            enqueue_step(h, true, h->own_cur, false, true);
+            sph::enqueue_step(h, true, h->own_cur, false, true);
             h->own_cur ^= 1;
             HIP_TRY(hipEventSynchronize(h->ev[7]));
             float ms[7];
@@ -631,7 +613,7 @@ int run_steps(sph_dem* h, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {
         const int lead = i > 0 ? 1 : 0;
         if (h->use_graph) HIP_TRY(hipGraphLaunch(h->gexec[lead][h->own_cur], h->stream));
-        else enqueue_step(h, false, h->own_cur, lead == 1, false);
+        else sph::enqueue_step(h, false, h->own_cur, lead == 1, false);
         pending = true;
         h->own_cur ^= 1;
         h->hs.t_cur += h->hs.dt;

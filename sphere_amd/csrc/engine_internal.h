@@ -1,0 +1,63 @@
+// engine_internal.h -- the handle behind the C-ABI, shared by the step
+// engine (engine.hip) and the adjoint driver (adjoint.hip).
+#ifndef SPHERE_ENGINE_INTERNAL_H_
+#define SPHERE_ENGINE_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "binio.h"
+#include "dem_internal.h"
+
+namespace sph {
+struct AdjointState;  // adjoint.hip
+}
+
+struct sph_dem {
+    sph::HostState hs;
+    sph::DemDev d{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    // Captured steps, [lead][parity]: parity = the set holding the private
+    // records at the step's start; lead = 1 runs the previous step's wall
+    // update on a side stream beside the cell list and contact search.
+    hipGraph_t graph[2][2] = {};
+    hipGraphExec_t gexec[2][2] = {};
+    int own_cur = 0;  // 0: the current private records are in b, 1: in a
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool use_graph = true;
+    int precision = SPH_PREC_FP64;
+    int profiling = 0;
+    int verbosity = 0;
+    sph_timing_t timing{};
+    hipEvent_t ev[8] = {};
+    bool events_made = false;
+    std::string err;
+    uint64_t steps_done = 0;
+    int device = 0;
+    // checkpoints, tape and adjoint buffers of sph_adjoint_* (adjoint.hip)
+    std::unique_ptr<sph::AdjointState, void (*)(sph::AdjointState*)> adj{nullptr, nullptr};
+};
+
+namespace sph {
+
+// Work inserted into one step: before_force runs after the contact search
+// (the step's sorted state and contact slots are final, the springs are not
+// yet updated), after_force after the force kernel.
+struct StepHooks {
+    std::function<void(const DemDev&, hipStream_t)> before_force, after_force;
+};
+
+// Enqueues one step whose private records are read from set `parity` (0: b,
+// 1: a) and written to the other; `lead` runs the previous step's wall
+// update on the side stream first, `tail` ends with this step's.
+void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail, const StepHooks* hooks = nullptr);
+
+}  // namespace sph
+
+#endif

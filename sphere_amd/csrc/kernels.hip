@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dem_internal.h"
+#include "vec3.h"
 
 namespace sph {
 
@@ -37,26 +38,6 @@ namespace {
 constexpr double PI_REF = 3.14159265358979;   // src/constants.h:14
 constexpr double PI_WALL = 3.141596654;       // src/integration.cuh:258
 
-// ------------------------------------------------------------ small vectors
-template <class T> struct V3 { T x, y, z; };
-
-template <class T> __device__ __forceinline__ V3<T> mk(T x, T y, T z) { V3<T> r; r.x = x; r.y = y; r.z = z; return r; }
-template <class T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <class T> __device__ __forceinline__ T vlen(V3<T> a) { return sqrt(dot(a, a)); }
-template <class T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b) {
-    return mk<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-template <class T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return mk<T>(a.x + b.x, a.y + b.y, a.z + b.z); }
-template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return mk<T>(a.x - b.x, a.y - b.y, a.z - b.z); }
-// scalar * vector and vector * scalar keep the operand order of
-// src/vector_arithmetic.h:150-157
-template <class T> __device__ __forceinline__ V3<T> operator*(T s, V3<T> a) { return mk<T>(s * a.x, s * a.y, s * a.z); }
-template <class T> __device__ __forceinline__ V3<T> operator*(V3<T> a, T s) { return mk<T>(a.x * s, a.y * s, a.z * s); }
-template <class T> __device__ __forceinline__ V3<T> operator/(V3<T> a, T s) { return mk<T>(a.x / s, a.y / s, a.z / s); }
-template <class T> __device__ __forceinline__ V3<T> operator-(V3<T> a) { return mk<T>(-a.x, -a.y, -a.z); }
-
-__device__ __forceinline__ V3<double> xyz(double4 a) { return mk<double>(a.x, a.y, a.z); }
-template <class T> __device__ __forceinline__ V3<T> cvt(V3<double> a) { return mk<T>((T)a.x, (T)a.y, (T)a.z); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
 // walks one contiguous range of sorted particles and its L2 holds that

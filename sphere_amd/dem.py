@@ -135,6 +135,8 @@ EXPORTS = {
                               C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
     "sph_pack_rsa": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
                                C.c_void_p]),
+    "sph_adjoint_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
+    "sph_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -283,6 +285,21 @@ class DEM:
         cur.update({k: float(v) for k, v in kw.items()})
         p = SphParams(**cur)
         self._check(self._lib.sph_set_params(self._h, C.byref(p)))
+
+    def adjoint_forward(self, nsteps: int, weights, seg_len: int = 100) -> float:
+        """Runs `nsteps` steps keeping checkpoints every `seg_len` steps and
+        returns J = mean over the steps of sum_i weights[i] . F_i."""
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(self.np, 3))
+        J = C.c_double()
+        self._check(self._lib.sph_adjoint_forward(self._h, int(nsteps), int(seg_len),
+                                                  w.ctypes.data_as(C.c_void_p), C.byref(J)))
+        return J.value
+
+    def adjoint_backward(self) -> np.ndarray:
+        """dJ/d(k_n, k_t, gamma_n, mu) of the last adjoint_forward run."""
+        g = (C.c_double * 4)()
+        self._check(self._lib.sph_adjoint_backward(self._h, g))
+        return np.array(list(g), dtype=np.float64)
 
     def set_precision(self, prec):
         if isinstance(prec, str):

@@ -1,0 +1,144 @@
+"""The hand-written adjoint (sph_adjoint_forward / sph_adjoint_backward and
+the torch.autograd.Function over it) against central finite differences of
+the same objective, relative step 1e-6."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from sphere_amd import configs
+
+THETA0 = (1e6, 1e6, 0.5, 0.5)  # k_n, k_t, gamma_n, mu
+
+
+def _case():
+    return configs.fcc_shear(cells=(4, 4, 8), gamma_n=THETA0[2], sid="adj-fcc-shear")
+
+
+def _say(msg):
+    print(f"[adjoint] {msg}", flush=True)
+
+
+def test_shear_weights_select_driven_layer():
+    from sphere_amd.adjoint import shear_stress_weights
+
+    sb = _case()
+    w = shear_stress_weights(sb)
+    driven = (sb.fixvel > 0) & (sb.vel[:, 0] > 0)
+    assert driven.sum() == 32
+    assert np.all(w[driven, 0] == -1.0 / (sb.L[0] * sb.L[1]))
+    assert np.all(w[~driven] == 0.0) and np.all(w[:, 1:] == 0.0)
+
+
+def _fd(sb, w, nsteps, precision="fp64", rel=1e-6):
+    from sphere_amd.adjoint import objective_value
+
+    th0 = np.array(THETA0)
+    fd = np.zeros(4)
+    for k in range(4):
+        h = rel * abs(th0[k])
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (objective_value(sb, tp, nsteps, w, precision) - objective_value(sb, tm, nsteps, w, precision)) / (2 * h)
+    return fd
+
+
+@pytest.mark.gpu
+def test_adjoint_matches_central_differences():
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    _say("fp64 adjoint vs central differences, 512 spheres, 200 steps")
+    sb = _case()
+    w = shear_stress_weights(sb)
+    nsteps = 200
+    dem = DEM(sb, verbosity=0)
+    dem.set_params(k_n=THETA0[0], k_t=THETA0[1], gamma_n=THETA0[2], mu_s=THETA0[3], mu_d=THETA0[3])
+    J = dem.adjoint_forward(nsteps, w, seg_len=64)  # 4 segments, the last partial
+    g = dem.adjoint_backward()
+    dem.close()
+    fd = _fd(sb, w, nsteps)
+    _say(f"J={J:.6e} adjoint={g} fd={fd}")
+    scale = np.abs(fd).max()
+    err = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    assert np.all(err <= 1e-4), f"relative error {err}"
+
+
+@pytest.mark.gpu
+def test_autograd_function_gives_the_adjoint():
+    import torch
+
+    from sphere_amd.adjoint import mean_shear_stress, shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    _say("autograd wrapper")
+    sb = _case()
+    theta = torch.tensor(THETA0, dtype=torch.float64, requires_grad=True)
+    J = mean_shear_stress(theta, sb, 120, seg_len=50)
+    J.backward()
+    dem = DEM(sb, verbosity=0)
+    dem.set_params(k_n=THETA0[0], k_t=THETA0[1], gamma_n=THETA0[2], mu_s=THETA0[3], mu_d=THETA0[3])
+    J2 = dem.adjoint_forward(120, shear_stress_weights(sb), seg_len=50)
+    g2 = dem.adjoint_backward()
+    dem.close()
+    assert float(J) == J2
+    assert np.array_equal(theta.grad.numpy(), g2)
+
+
+@pytest.mark.gpu
+def test_adjoint_leaves_the_forward_end_state():
+    """Forward + backward leave the handle exactly where a plain run of the
+    same steps ends (checkpoints restore every array bit for bit)."""
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    _say("end state after backward")
+    sb = _case()
+    a = DEM(sb, verbosity=0)
+    a.adjoint_forward(90, shear_stress_weights(sb), seg_len=40)
+    a.adjoint_backward()
+    b = DEM(sb, verbosity=0)
+    b.step(90)
+    for f in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t", "walls_nx", "walls_mvfd"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.get("x"), b.get("x"))
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_mixed_adjoint_close_to_fp64_differences():
+    """Mixed mode (FP32 pair forces): its adjoint against the FP64 central
+    differences; FP32 rounding bounds the agreement, not the adjoint."""
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    _say("mixed adjoint vs fp64 central differences")
+    sb = _case()
+    w = shear_stress_weights(sb)
+    nsteps = 200
+    dem = DEM(sb, verbosity=0)
+    dem.set_precision("mixed")
+    dem.set_params(k_n=THETA0[0], k_t=THETA0[1], gamma_n=THETA0[2], mu_s=THETA0[3], mu_d=THETA0[3])
+    dem.adjoint_forward(nsteps, w, seg_len=64)
+    g = dem.adjoint_backward()
+    dem.close()
+    fd = _fd(sb, w, nsteps)
+    err = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    _say(f"mixed adjoint={g} fp64 fd={fd} rel err={err}")
+    assert np.all(err <= 1e-2), f"relative error {err}"
+
+
+@pytest.mark.gpu
+def test_bonded_input_is_unsupported():
+    from sphere_amd.dem import DEM, Unsupported
+
+    sb = configs.two_spheres(bonded=True)
+    dem = DEM(sb, verbosity=0)
+    with pytest.raises(Unsupported):
+        dem.adjoint_forward(5, np.zeros((2, 3)))
+    dem.close()
