@@ -43,6 +43,8 @@ STAGE_KERNELS = {
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
 PROFILE_KERNEL = {"reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force<double>", "cellid": "k_count"}
+SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply"}
+SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half"}
 
 
 def log(msg):
@@ -133,7 +135,7 @@ def timed_run(bin_path, precision, steps, warmup, prof_steps):
     dem.close()
     stage_ms = {k: tim[f"{k}_ms"] / max(1, tim["steps"]) for k in STAGES}
     return {"ms": ms, "contacts": c, "stage_ms": stage_ms, "tile": [tim["tile_x"], tim["tile_y"]],
-            "state": state, "steps_total": warmup + steps + prof_steps}
+            "search_mode": int(tim["search_mode"]), "state": state, "steps_total": warmup + steps + prof_steps}
 
 
 def mixed_error(ref, mix, area, r_min):
@@ -150,6 +152,43 @@ def mixed_error(ref, mix, area, r_min):
     return out
 
 
+def adjoint_run(tmp, nsteps, seg=100):
+    """Mean-shear-stress gradient on the 65,536-sphere dense shear cell:
+    plain forward time against the adjoint's forward (with checkpoints every
+    `seg` steps) and backward (recompute with tape + reverse sweep)."""
+    from sphere_amd import configs
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    sb = configs.fcc_shear(gamma_n=0.5)
+    path = os.path.join(tmp, "shear.bin")
+    sb.writebin(path)
+    w = shear_stress_weights(sb)
+    dem = DEM(path, verbosity=0)
+    dem.step(20)
+    dem.close()
+    dem = DEM(path, verbosity=0)
+    t0 = time.perf_counter()
+    dem.step(nsteps)
+    t_plain = time.perf_counter() - t0
+    dem.close()
+    dem = DEM(path, verbosity=0)
+    t0 = time.perf_counter()
+    J = dem.adjoint_forward(nsteps, w, seg_len=seg)
+    t1 = time.perf_counter()
+    g = dem.adjoint_backward()
+    t2 = time.perf_counter()
+    dem.close()
+    log(f"adjoint: plain {t_plain * 1e3 / nsteps:.4f} ms/step, forward+checkpoints {(t1 - t0) * 1e3 / nsteps:.4f}, "
+        f"backward {(t2 - t1) * 1e3 / nsteps:.4f}; J={J:.6e} grad={g}")
+    return {"workload": f"{sb.np:,}-sphere dense FCC shear cell, top layer at 0.1 m/s, 10 kPa wall, {nsteps} steps, "
+                        f"checkpoints every {seg}", "objective": "mean shear stress (Pa)", "J": J,
+            "grad": dict(zip(("k_n", "k_t", "gamma_n", "mu"), [float(v) for v in g])),
+            "plain_ms_per_step": t_plain * 1e3 / nsteps, "forward_ms_per_step": (t1 - t0) * 1e3 / nsteps,
+            "backward_ms_per_step": (t2 - t1) * 1e3 / nsteps,
+            "backward_over_forward": (t2 - t1) / t_plain, "total_over_forward": (t2 - t0) / t_plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +200,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the copy-kernel bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -194,7 +235,9 @@ def main():
     ms, c, stage_ms = main_run["ms"], main_run["contacts"], main_run["stage_ms"]
     log(f"{args.precision}: {ms:.4f} ms/step, C/N={c / n:.3f}, stages {stage_ms}")
     kb = kernel_bytes(n, c, sb.nb0, sb.nw, K)
-    per_kernel = {k: {"ms": stage_ms[k], "kernels": STAGE_KERNELS[k], "algorithmic_bytes": kb[k],
+    stage_kernels = dict(STAGE_KERNELS, search=SEARCH_KERNELS.get(main_run["search_mode"], "?"))
+    profile_kernel = dict(PROFILE_KERNEL, search=SEARCH_PROFILE.get(main_run["search_mode"]))
+    per_kernel = {k: {"ms": stage_ms[k], "kernels": stage_kernels[k], "algorithmic_bytes": kb[k],
                       "GBs": (kb[k] / (stage_ms[k] * 1e-3) / 1e9) if stage_ms[k] > 0 and kb[k] else None}
                   for k in STAGES}
     dom = max(stage_ms, key=lambda k: stage_ms[k])
@@ -214,11 +257,12 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 pair forces",
         "data": "synthetic (seeded lognormal random packing)",
         "config": {"workload": "1,048,576 lognormal spheres, random packing, 10 kPa top wall, 2.07x2.07x4.13 m x/y-periodic box",
-                   "n": n, "cells": K, "contacts_per_particle": c / n, "precision": args.precision},
+                   "n": n, "cells": K, "contacts_per_particle": c / n, "precision": args.precision,
+                   "search_kernel": stage_kernels["search"]},
         "per_kernel": per_kernel,
-        "roofline": {"bound": "hbm", "kernel": STAGE_KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": stage_kernels[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(PROFILE_KERNEL.get(dom)),
+                     "traffic": measured_traffic(profile_kernel.get(dom)),
                      "algorithmic_bytes_per_launch": kb[dom]},
         "step_roofline": {"algorithmic_bytes": step_b, "achieved_GBs": step_b / (ms * 1e-3) / 1e9,
                           "frac": step_b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -238,7 +282,8 @@ def main():
         dense = {"workload": "1,048,576-sphere FCC crystal (4 x 64^3), nearest neighbours overlapping by 2e-5 m, "
                              "x/y periodic, floor and fixed top wall, gamma_n = 0",
                  "contacts_per_particle": d64["contacts"] / dn, "fp64": {"value": dn / (d64["ms"] * 1e-3),
-                 "ms_per_step": d64["ms"], "stage_ms": d64["stage_ms"]}}
+                 "ms_per_step": d64["ms"], "stage_ms": d64["stage_ms"],
+                 "search_kernel": SEARCH_KERNELS.get(d64["search_mode"])}}
         log(f"dense fp64: {d64['ms']:.4f} ms/step, C/N={d64['contacts'] / dn:.2f}")
         if not args.no_mixed:
             dmx = timed_run(dpath, "mixed", args.steps, args.warmup, prof_steps)
@@ -249,6 +294,9 @@ def main():
                                                            float(sbd.radius.min()))}
         result["dense"] = dense
     result["adjoint_forward_ratio"] = None
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_run(tmp, args.adjoint_steps)
+        result["adjoint_forward_ratio"] = result["adjoint"]["backward_over_forward"]
     if not args.no_cpu:
         rate, secs = cpu_baseline(bin_path, n, args.cpu_steps)
         result["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",

@@ -104,6 +104,8 @@ typedef struct sph_timing {
     uint64_t steps;
     double search_ms, force_ms;  /* contact_ms = search_ms + force_ms */
     uint32_t tile_x, tile_y;     /* cells per contact-search block */
+    int32_t search_mode;         /* contact-search kernel in use (sph_set_search) */
+    int32_t search_auto;         /* 1 when the kernel is chosen by timing */
 } sph_timing_t;
 
 /* DEM::DEM + readbin + checkValues + allocateGlobalDeviceMemory +
@@ -173,7 +175,10 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y);
 
 /* Contact-search kernel: 1 = one block per tile with the neighbourhood staged
  * in LDS, 2 = one thread per sorted particle reading through L1/L2, 3 = one
- * wavefront per cell, one lane per candidate, 0 = the default (2). */
+ * wavefront per cell, one lane per candidate, 4 = half shell (each pair
+ * tested once, hits replayed in the reference's order), 0 = automatic (the
+ * default): every 4096 steps sph_step times one step with each of 2 and 4
+ * and keeps the faster.  All kernels give bit-identical results. */
 int sph_set_search(sph_t* h, int mode);
 
 /* Measurement helper with no counterpart in the reference: copies `bytes`

@@ -56,7 +56,8 @@ struct Lam {
     double* wall;                  // NWALL x (pos, vel, vel0)
 };
 
-// One taped step (the a set after the search, before the force kernel).
+// One taped step: its sorted input state, slots and springs after the contact
+// search, before the force kernel.
 struct TapeView {
     const double4 *x, *vel, *angvel;
     const uint32_t *id, *pos_of, *tmask, *tfresh, *cslot;
@@ -491,10 +492,31 @@ __global__ __launch_bounds__(256) void k_adj_objective(DemDev d, const double4* 
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
     if (s < d.np) {
-        const double4 w = wobj[d.a.id[s]], f = d.force[s];
+        const double4 w = wobj[d.sid[s]], f = d.force[s];
         v = w.x * f.x + w.y * f.y + w.z * f.z;
     }
     block_add(v, J, red);
+}
+
+// The step's sorted input state as the force kernel sees it (records at
+// perm[q], positions wrapped as the step wraps them), for the tape.
+__global__ __launch_bounds__(256) void k_adj_tape(DemDev d, double4* tx, double4* tv, double4* tw, uint32_t* tid) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.np) return;
+    const uint32_t p = d.perm[q];
+    double4 x = d.ro.x[p];
+    if (d.periodic == 1 || d.periodic == 2) {
+        if (x.x < d.origo[0]) x.x += d.L[0];
+        if (x.x > d.L[0]) x.x -= d.L[0];
+    }
+    if (d.periodic == 1) {
+        if (x.y < d.origo[1]) x.y += d.L[1];
+        if (x.y > d.L[1]) x.y -= d.L[1];
+    }
+    tx[q] = x;
+    tv[q] = d.ro.vel[p];
+    tw[q] = d.ro.angvel[p];
+    tid[q] = d.sid[q];
 }
 
 inline uint32_t nb(uint64_t n) { return (uint32_t)((n + 255) / 256); }
@@ -751,12 +773,9 @@ int sph_adjoint_backward(sph_t* h, double grad_out[4]) {
         // recompute the segment, taping each step before its force kernel
         for (uint64_t j = 0; j < m; ++j) {
             sph::StepHooks hooks;
-            hooks.before_force = [a, j, n, nc](const sph::DemDev& d, hipStream_t s) {
+            hooks.before_force = [a, j, n](const sph::DemDev& d, hipStream_t s) {
                 const size_t o1 = j * n, on = j * n * nc;
-                (void)hipMemcpyAsync(a->tx + o1, d.a.x, n * 32, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->tv + o1, d.a.vel, n * 32, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->tw + o1, d.a.angvel, n * 32, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->tid + o1, d.a.id, n * 4, hipMemcpyDeviceToDevice, s);
+                sph::k_adj_tape<<<sph::nb(n), 256, 0, s>>>(d, a->tx + o1, a->tv + o1, a->tw + o1, a->tid + o1);
                 (void)hipMemcpyAsync(a->tpos + o1, d.pos_of, n * 4, hipMemcpyDeviceToDevice, s);
                 (void)hipMemcpyAsync(a->tmask + o1, d.tmask, n * 4, hipMemcpyDeviceToDevice, s);
                 (void)hipMemcpyAsync(a->tfresh + o1, d.tfresh, n * 4, hipMemcpyDeviceToDevice, s);

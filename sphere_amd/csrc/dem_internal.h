@@ -6,20 +6,14 @@
 // particles in.  Each particle carries its original index (`id`), which is
 // what the reference's semantics are stated in (the stable sort's tie-break,
 // contact partners, bond endpoints, the wall-force reduction order and the
-// output file order).  Two buffer sets hold the state:
-//
-//   b  state at the start of a step, in the previous step's sorted order
-//   a  the same state gathered into this step's sorted order (reorder)
-//
-// Only the records other particles read (x, vel, angvel, id) are gathered
-// into a; the contact kernel reads them there (own particle and neighbours)
-// and writes the integrated values back to b at the same sorted position, so
-// neighbour reads never race with updates.  The records only the particle
-// itself reads (vel0, angvel0, angpos, the accumulators and the contact-slot
-// mask) are not gathered: the two sets' copies of them alternate between
-// steps, a step reading one set at the particle's previous position perm[s]
-// and writing the other at its new position s.  Records are 32-byte double4
-// so each access fills whole sectors.
+// output file order).  Two buffer sets alternate between steps: a step reads
+// every record from one set (`ro`) at the particle's previous sorted position
+// perm[s] and writes the updated record to the other (`wo`) at its new
+// sorted position s, so the next step finds its input in consecutive order
+// and neighbour reads never race with updates.  The sort itself only moves
+// indices, an FP32 copy of the positions for the contact search's prefilter
+// and the sorted ids (`sid`).  Records are 32-byte double4 so each access
+// fills whole sectors.
 //
 // Contact history stays keyed by original index: a 32-bit occupancy mask
 // (moved with the particle) plus SPH_NC slots.  A slot word packs the
@@ -67,10 +61,10 @@ struct PState {
     uint32_t *cmask;  // occupied contact slots
 };
 
-// The particle-private records of one buffer set.
+// The records of one buffer set, as a step reads (ro) or writes (wo) them.
 struct OwnSet {
-    double4 *vel0, *angvel0, *angpos, *acc;
-    uint32_t* cmask;
+    double4 *x, *vel, *angvel, *vel0, *angvel0, *angpos, *acc;
+    uint32_t *id, *cmask;
 };
 
 // Everything a step kernel needs, passed by value as one kernel argument.
@@ -87,8 +81,9 @@ struct DemDev {
     double gamma_wn, gamma_wt, mu_ws, mu_wd, rho, kappa;
     double lambda_bar, sigma_b, tau_b, devs_A, devs_f;
 
-    PState a, b;
-    OwnSet ro, wo;      // private records: read at perm[s], written at s
+    PState a, b;        // storage of the two buffer sets
+    OwnSet ro, wo;      // this step's input (at perm[s]) and output (at s)
+    uint32_t* sid;      // original index per sorted position of this step
     uint32_t* pos_of;   // original index -> sorted position of this step
 
     // per-step outputs, sorted order of the step
@@ -106,11 +101,13 @@ struct DemDev {
 
     // contact search tiles: TX x TY cells of one z plane per block
     uint32_t tile_x, tile_y, tiles_x, tiles_y;
-    int search_mode;    // 1: LDS tiles, 2: one thread per sorted particle, HBM/L2 reads
+    int search_mode;    // 1: LDS tiles, 2: rows per particle, 3: wave per cell, 4: half shell
     uint32_t* skey;     // cell id per sorted position
     float4* xf;         // FP32 copy of a.x for the search's distance prefilter
     float search_eps;   // prefilter margin (m), above the FP32 rounding of a distance
     float Lf[2];        // L[0], L[1] in FP32 (periodic images in the prefilter)
+    uint32_t* hcnt;     // half-shell search: overlaps found per particle
+    uint64_t* hits;     // np * NC keys: offset index << 40 | image << 32 | sorted position
     uint32_t* tmask;    // slot occupancy after the search, sorted order
     uint32_t* tfresh;   // slots taken from empty by the search, sorted order
 

@@ -56,25 +56,6 @@ template <class T> int dalloc(sph_dem* h, T** p, size_t count) {
         if (rc_) return rc_;                           \
     } while (0)
 
-std::vector<double> pad4(const std::vector<double>& a3, size_t n, const double* w = nullptr) {
-    std::vector<double> out(4 * n);
-    for (size_t i = 0; i < n; ++i) {
-        out[4 * i + 0] = a3[3 * i + 0];
-        out[4 * i + 1] = a3[3 * i + 1];
-        out[4 * i + 2] = a3[3 * i + 2];
-        out[4 * i + 3] = w ? w[i] : 0.0;
-    }
-    return out;
-}
-
-void unpad4(const std::vector<double>& a4, size_t n, double* out3) {
-    for (size_t i = 0; i < n; ++i) {
-        out3[3 * i + 0] = a4[4 * i + 0];
-        out3[3 * i + 1] = a4[4 * i + 1];
-        out3[3 * i + 2] = a4[4 * i + 2];
-    }
-}
-
 void destroy_graph(sph_dem* h) {
     for (int l = 0; l < 2; ++l)
         for (int k = 0; k < 2; ++k) {
@@ -83,7 +64,9 @@ void destroy_graph(sph_dem* h) {
         }
 }
 
-sph::OwnSet own_set(const sph::PState& p) { return {p.vel0, p.angvel0, p.angpos, p.acc, p.cmask}; }
+sph::OwnSet own_set(const sph::PState& p) {
+    return {p.x, p.vel, p.angvel, p.vel0, p.angvel0, p.angpos, p.acc, p.id, p.cmask};
+}
 
 // the buffer set holding the current private records
 const sph::PState& own(const sph_dem* h) { return h->own_cur ? h->d.a : h->d.b; }
@@ -249,16 +232,17 @@ int upload_all(sph_dem* h) {
         rates[4 * i + 1] = s.ev_dot[i];
         rates[4 * i + 2] = s.p[i];
     }
-    HIP_TRY(hipMemcpy(d.b.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.b.vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).x, s.x.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
     // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel
     HIP_TRY(hipMemcpy(own(h).vel0, s.vel.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.b.angvel, w4.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).angvel, w4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).angvel0, w4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).angpos, p4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).acc, acc.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.b.id, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.a.id, iota.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.sid, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.pos_of, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.b.cmask, 0, n * 4));
     HIP_TRY(hipMemset(d.a.cmask, 0, n * 4));
@@ -272,6 +256,7 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
     HIP_TRY(hipMemset(d.off, 0, ((size_t)d.K + 1) * 4));
     HIP_TRY(hipMemcpy(d.a.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d.perm, 0, n * 4));
     // walls: entries past nw keep the defaults of src/contactsearch.cuh:416-427
     double wnx[24] = {0.0, 0.0, -1.0, s.L[2], -1.0, 0.0, 0.0, s.L[0], 1.0, 0.0, 0.0, 0.0,
                       0.0, -1.0, 0.0, s.L[1], 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -341,7 +326,7 @@ int fetch4(sph_dem* h, const double4* src, const std::vector<uint32_t>& id, std:
 
 int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) {
     id.resize(h->hs.np);
-    HIP_TRY(hipMemcpy(id.data(), h->d.b.id, h->hs.np * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(id.data(), own(h).id, h->hs.np * 4, hipMemcpyDeviceToHost));
     return SPH_OK;
 }
 
@@ -362,6 +347,7 @@ int allocate(sph_dem* h) {
     d.cdt32 = nullptr;
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
     ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
+    ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
     ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
@@ -435,8 +421,10 @@ namespace {
 // Cell ids of the stored positions (after upload or sph_set of positions):
 // what a step's force kernel leaves for the next step.
 int prepare_keys(sph_dem* h) {
+    DemDev d = h->d;
+    d.ro = own_set(own(h));
     sph::launch_fill_u32(h->d.flags_next, 1, 0u, h->stream);
-    sph::launch_cellid(h->d, h->stream);
+    sph::launch_cellid(d, h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));
     return SPH_OK;
@@ -489,9 +477,9 @@ int download_all(sph_dem* h) {
     std::vector<uint32_t> id;
     if ((rc = fetch_ids(h, id))) return rc;
     std::vector<double> t;
-    if ((rc = fetch4(h, d.b.x, id, t))) return rc;
+    if ((rc = fetch4(h, own(h).x, id, t))) return rc;
     s.x = t;
-    if ((rc = fetch4(h, d.b.vel, id, t))) return rc;
+    if ((rc = fetch4(h, own(h).vel, id, t))) return rc;
     s.vel = t;
     auto unpad = [&](std::vector<double>& dst) {
         for (size_t i = 0; i < n; ++i)
@@ -501,7 +489,7 @@ int download_all(sph_dem* h) {
     unpad(s.force);
     if ((rc = fetch4(h, own(h).angpos, id, t))) return rc;
     unpad(s.angpos);
-    if ((rc = fetch4(h, d.b.angvel, id, t))) return rc;
+    if ((rc = fetch4(h, own(h).angvel, id, t))) return rc;
     unpad(s.angvel);
     if ((rc = fetch4(h, d.torque, id, t))) return rc;
     unpad(s.torque);
@@ -580,6 +568,37 @@ int make_events(sph_dem* h) {
     return SPH_OK;
 }
 
+// Times the contact search of two steps with each of the row (2) and
+// half-shell (4) kernels and keeps the faster; the steps are ordinary steps
+// of the run (both kernels give the same bits).
+int tune_search(sph_dem* h) {
+    int rc;
+    if ((rc = make_events(h))) return rc;
+    const int modes[2] = {2, 4};
+    float best[2] = {1e30f, 1e30f};
+    for (int k = 0; k < 2; ++k) {
+        h->d.search_mode = modes[k];
+        for (int rep = 0; rep < 2; ++rep) {
+            sph::enqueue_step(h, true, h->own_cur, false, true);
+            h->own_cur ^= 1;
+            h->hs.t_cur += h->hs.dt;
+            ++h->steps_done;
+            HIP_TRY(hipEventSynchronize(h->ev[7]));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
+            best[k] = std::min(best[k], ms);
+        }
+    }
+    h->d.search_mode = best[1] < best[0] ? 4 : 2;
+    destroy_graph(h);
+    h->tuned = true;
+    h->steps_since_tune = 0;
+    if (h->verbosity > 1)
+        std::printf("sphere_dem: contact search %.4f ms (rows) vs %.4f ms (half shell): mode %d\n", best[0], best[1],
+                    h->d.search_mode);
+    return check_flags(h);
+}
+
 int run_steps(sph_dem* h, uint64_t n) {
     if (n == 0) return SPH_OK;
     int rc;
@@ -603,9 +622,16 @@ int run_steps(sph_dem* h, uint64_t n) {
         t.steps = n;
         t.tile_x = h->d.tile_x;
         t.tile_y = h->d.tile_y;
+        t.search_mode = h->d.search_mode;
+        t.search_auto = h->search_auto ? 1 : 0;
         h->timing = t;
         return check_flags(h);
     }
+    if (h->search_auto && n >= 8 && (!h->tuned || h->steps_since_tune >= 4096)) {
+        if ((rc = tune_search(h))) return rc;
+        n -= 4;
+    }
+    h->steps_since_tune += n;
     if (h->use_graph && (rc = ensure_graph(h))) return rc;
     // step i > 0 runs step i-1's wall update beside its cell list and search;
     // the last step's wall update is enqueued before returning, also on error
@@ -777,10 +803,33 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         return SPH_OK;
     };
     switch (f) {
-        case SPH_X: return gather(d.b.x, 0, 4);
-        case SPH_VEL: return gather(d.b.vel, 0, 4);
-        case SPH_X_SORTED: return copy(d.a.x);
-        case SPH_ANGVEL: return gather(d.b.angvel, 0, 3);
+        case SPH_X: return gather(own(h).x, 0, 4);
+        case SPH_VEL: return gather(own(h).vel, 0, 4);
+        case SPH_X_SORTED: {
+            // the last step's input positions (the other set) in its sorted
+            // order, wrapped as the step wrapped them
+            const sph::PState& in = h->own_cur ? d.b : d.a;
+            std::vector<double> xs(4 * n);
+            std::vector<uint32_t> perm(n);
+            HIP_TRY(hipMemcpy(xs.data(), in.x, n * 32, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(perm.data(), d.perm, n * 4, hipMemcpyDeviceToHost));
+            const HostState& hs = h->hs;
+            for (size_t q = 0; q < n; ++q) {
+                double v[4];
+                std::memcpy(v, &xs[4 * (size_t)perm[q]], 32);
+                if (hs.periodic == 1 || hs.periodic == 2) {
+                    if (v[0] < hs.origo[0]) v[0] += hs.L[0];
+                    if (v[0] > hs.L[0]) v[0] -= hs.L[0];
+                }
+                if (hs.periodic == 1) {
+                    if (v[1] < hs.origo[1]) v[1] += hs.L[1];
+                    if (v[1] > hs.L[1]) v[1] -= hs.L[1];
+                }
+                std::memcpy(dd + 4 * q, v, 32);
+            }
+            return SPH_OK;
+        }
+        case SPH_ANGVEL: return gather(own(h).angvel, 0, 3);
         case SPH_FORCE: return gather(d.force, 0, 3);
         case SPH_TORQUE: return gather(d.torque, 0, 3);
         case SPH_ANGPOS: return gather(own(h).angpos, 0, 3);
@@ -792,13 +841,13 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_ES_DOT: return gather(d.rates, 0, 1);
         case SPH_EV_DOT: return gather(d.rates, 1, 1);
         case SPH_P: return gather(d.rates, 2, 1);
-        case SPH_SORT_IDX: return copy(d.a.id);
+        case SPH_SORT_IDX: return copy(d.sid);
         case SPH_CELL_KEYS: case SPH_CELL_START: case SPH_CELL_END: {
             std::vector<uint32_t> off(K + 1);
             HIP_TRY(hipMemcpy(off.data(), d.off, (K + 1) * 4, hipMemcpyDeviceToHost));
             if (f == SPH_CELL_KEYS) {
                 std::vector<uint32_t> sid(n);
-                HIP_TRY(hipMemcpy(sid.data(), d.a.id, n * 4, hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(sid.data(), d.sid, n * 4, hipMemcpyDeviceToHost));
                 for (size_t c = 0; c < K; ++c)
                     for (uint32_t q = off[c]; q < off[c + 1]; ++q) du[sid[q]] = (uint32_t)c;
                 return SPH_OK;
@@ -836,7 +885,7 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
             }
             if (f == SPH_DISTMOD) {
                 std::vector<double> x;
-                if ((rc = fetch4(h, d.b.x, id, x))) return rc;
+                if ((rc = fetch4(h, own(h).x, id, x))) return rc;
                 for (size_t i = 0; i < n; ++i)
                     for (size_t k = 0; k < nc; ++k) {
                         double* o4 = dd + 4 * (i * nc + k);
@@ -897,15 +946,15 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
     };
     switch (f) {
         case SPH_X: {
-            const int r = scatter(d.b.x, 0, 4);
+            const int r = scatter(own(h).x, 0, 4);
             if (r) return r;
             std::memcpy(h->hs.x.data(), sd, n * 32);  // radii for the search margin
             update_search_eps(h);
             destroy_graph(h);
             return prepare_keys(h);
         }
-        case SPH_VEL: return scatter(d.b.vel, 0, 4);
-        case SPH_ANGVEL: return scatter(d.b.angvel, 0, 3);
+        case SPH_VEL: return scatter(own(h).vel, 0, 4);
+        case SPH_ANGVEL: return scatter(own(h).angvel, 0, 3);
         case SPH_FORCE: return scatter(d.force, 0, 3);
         case SPH_TORQUE: return scatter(d.torque, 0, 3);
         case SPH_ANGPOS: return scatter(own(h).angpos, 0, 3);
@@ -1010,7 +1059,9 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y) {
 }
 
 int sph_set_search(sph_t* h, int mode) {
-    if (!h || mode < 0 || mode > 3) return SPH_E_INVALID;
+    if (!h || mode < 0 || mode > 4) return SPH_E_INVALID;
+    h->search_auto = mode == 0;
+    h->tuned = false;
     h->d.search_mode = mode == 0 ? 2 : mode;
     destroy_graph(h);
     return SPH_OK;

@@ -40,6 +40,10 @@ struct sph_dem {
     std::string err;
     uint64_t steps_done = 0;
     int device = 0;
+    // automatic contact-search choice (search mode 0)
+    bool search_auto = true;
+    uint64_t steps_since_tune = 0;
+    bool tuned = false;
     // checkpoints, tape and adjoint buffers of sph_adjoint_* (adjoint.hip)
     std::unique_ptr<sph::AdjointState, void (*)(sph::AdjointState*)> adj{nullptr, nullptr};
 };

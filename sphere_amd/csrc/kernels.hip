@@ -63,6 +63,44 @@ __device__ __forceinline__ bool wrap_xy(const DemDev& d, double& x, double& y) {
     return changed;
 }
 
+// Start-of-step position of sorted particle q: its stored record at the
+// previous position perm[q], wrapped as the step's integrate wraps it.
+__device__ __forceinline__ double4 spos(const DemDev& d, uint32_t q) {
+    double4 x = d.ro.x[d.perm[q]];
+    (void)wrap_xy(d, x.x, x.y);
+    return x;
+}
+
+// The same for non-inlined helpers, which take this small view by value
+// instead of the DemDev.
+struct PosView {
+    const double4* x;
+    const uint32_t* perm;
+    double ox, oy, Lx, Ly;
+    int periodic;
+};
+
+__device__ __forceinline__ PosView pos_view(const DemDev& d) {
+    PosView v;
+    v.x = d.ro.x; v.perm = d.perm;
+    v.ox = d.origo[0]; v.oy = d.origo[1]; v.Lx = d.L[0]; v.Ly = d.L[1];
+    v.periodic = d.periodic;
+    return v;
+}
+
+__device__ __forceinline__ double4 pv_pos(const PosView& v, uint32_t q) {
+    double4 x = v.x[v.perm[q]];
+    if (v.periodic == 1 || v.periodic == 2) {
+        if (x.x < v.ox) x.x += v.Lx;
+        if (x.x > v.Lx) x.x -= v.Lx;
+    }
+    if (v.periodic == 1) {
+        if (x.y < v.oy) x.y += v.Ly;
+        if (x.y > v.Ly) x.y -= v.Ly;
+    }
+    return x;
+}
+
 // floor((v - origo)/(L/num)) (src/sorting.cuh:12-14), clamped to the grid.
 __device__ __forceinline__ int cell_coord(const DemDev& d, double v, int a, uint32_t& err) {
     const double f = floor((v - d.origo[a]) / d.cell_len[a]);
@@ -96,7 +134,7 @@ __device__ __forceinline__ uint32_t cell_key(const DemDev& d, double x, double y
 __global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
-    const double4 x = d.b.x[i];
+    const double4 x = d.ro.x[i];
     uint32_t err = 0;
     d.key[i] = cell_key(d, x.x, x.y, x.z, err);
     if (err) atomicOr(d.flags_next, err);
@@ -220,7 +258,7 @@ __device__ __forceinline__ void sort_small(const DemDev& d, uint32_t* p, uint32_
     uint64_t v[W];
 #pragma unroll
     for (int j = 0; j < W; ++j)
-        v[j] = (uint32_t)j < n ? ((uint64_t)d.b.id[p[j]] << 32) | p[j] : ~0ull;
+        v[j] = (uint32_t)j < n ? ((uint64_t)d.ro.id[p[j]] << 32) | p[j] : ~0ull;
 #pragma unroll
     for (int r = 0; r < W; ++r) {
 #pragma unroll
@@ -248,9 +286,9 @@ __global__ __launch_bounds__(BLOCK) void k_cellsort(DemDev d) {
     else {
         for (uint32_t i = 1; i < n; ++i) {
             const uint32_t v = p[i];
-            const uint32_t key = d.b.id[v];
+            const uint32_t key = d.ro.id[v];
             uint32_t j = i;
-            while (j > 0 && d.b.id[p[j - 1]] > key) { p[j] = p[j - 1]; --j; }
+            while (j > 0 && d.ro.id[p[j - 1]] > key) { p[j] = p[j - 1]; --j; }
             p[j] = v;
         }
     }
@@ -260,14 +298,11 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
     if (q >= d.np) return;
     const uint32_t p = d.perm[q];
-    double4 x = d.b.x[p];
+    double4 x = d.ro.x[p];
     (void)wrap_xy(d, x.x, x.y);
-    d.a.x[q] = x;
     d.xf[q] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
-    d.a.vel[q] = d.b.vel[p];
-    d.a.angvel[q] = d.b.angvel[p];
-    const uint32_t id = d.b.id[p];
-    d.a.id[q] = id;
+    const uint32_t id = d.ro.id[p];
+    d.sid[q] = id;
     d.pos_of[id] = q;
     d.skey[q] = d.key[p];
 }
@@ -288,9 +323,9 @@ __global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
     V3<double> omega0_t = mk(o0.x, o0.y, o0.z);
     const double omega0_n = o0.w;
     const uint32_t pi = d.pos_of[bond.x], pj = d.pos_of[bond.y];
-    const double4 x_i = d.b.x[pi], x_j = d.b.x[pj];
-    const double4 vel_i = d.b.vel[pi], vel_j = d.b.vel[pj];
-    const V3<double> angvel_i = xyz(d.b.angvel[pi]), angvel_j = xyz(d.b.angvel[pj]);
+    const double4 x_i = d.ro.x[pi], x_j = d.ro.x[pj];
+    const double4 vel_i = d.ro.vel[pi], vel_j = d.ro.vel[pj];
+    const V3<double> angvel_i = xyz(d.ro.angvel[pi]), angvel_j = xyz(d.ro.angvel[pj]);
     const double R_bar = d.lambda_bar * fmin(x_i.w, x_j.w);
     const double A = PI_REF * R_bar * R_bar;
     const double I = 0.25 * PI_REF * R_bar * R_bar * R_bar * R_bar;
@@ -483,7 +518,7 @@ __device__ __noinline__ void slot_insert(uint32_t* slot, SlotState& st, uint32_t
 template <bool FROM_LDS>
 __device__ __forceinline__ double4 staged_pos(const DemDev& d, const TopoShared& sh, uint32_t lb, uint32_t sb) {
     if constexpr (FROM_LDS) return sh.pos[lb];
-    else return d.a.x[sb];
+    else return spos(d, sb);
 }
 
 // Walks the 27 cells of every home particle of the tile: 9 rows (z, y order),
@@ -515,7 +550,7 @@ __device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& s
         const uint32_t s = sh.cglob[eh] + (li - sh.cstart[eh]);
         const double4 xa = staged_pos<FROM_LDS>(d, sh, li, s);
         const double ra = xa.w;
-        const uint32_t o = d.a.id[s];
+        const uint32_t o = d.sid[s];
         uint32_t* slot = d.cslot + (size_t)o * NC;
         SlotState st;
         st.mask = d.ro.cmask[d.perm[s]];
@@ -546,7 +581,7 @@ __device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& s
                         if (__builtin_expect(d2 <= rsum * rsum * 1.0000001, 0) && lb != li) {
                             // exact test of src/contactsearch.cuh:242-248 (double sqrt)
                             if (sqrt(d2) - rsum < 0.0) {
-                                slot_insert(slot, st, d.a.id[gb + lb], code);
+                                slot_insert(slot, st, d.sid[gb + lb], code);
                                 if (st.stop) break;
                             }
                         }
@@ -615,7 +650,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (sh.cstart[mid] <= i) lo = mid; else hi = mid - 1;
             }
-            sh.pos[i] = d.a.x[sh.cglob[lo] + (i - sh.cstart[lo])];
+            sh.pos[i] = spos(d, sh.cglob[lo] + (i - sh.cstart[lo]));
         }
     }
     __syncthreads();
@@ -641,13 +676,12 @@ struct RowSlots {
 
 // (Takes plain pointers rather than the DemDev, which a non-inlined call
 // would have to copy to scratch memory.)
-__device__ __noinline__ void rows_try(const double4* __restrict__ ax, const uint32_t* __restrict__ aid, double L0,
-                                      double L1, uint32_t* sw, const uint32_t* gslot, RowSlots& st, double4 xa,
-                                      uint32_t sb, uint32_t code) {
-    const double4 xb = ax[sb];
+__device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ sid, uint32_t* sw,
+                                      const uint32_t* gslot, RowSlots& st, double4 xa, uint32_t sb, uint32_t code) {
+    const double4 xb = pv_pos(pv, sb);
     const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
-    const double dmx = cx == 1u ? L0 : (cx == 2u ? -L0 : 0.0);
-    const double dmy = cy == 1u ? L1 : (cy == 2u ? -L1 : 0.0);
+    const double dmx = cx == 1u ? pv.Lx : (cx == 2u ? -pv.Lx : 0.0);
+    const double dmy = cy == 1u ? pv.Ly : (cy == 2u ? -pv.Ly : 0.0);
     const double ex = (xa.x - xb.x) + dmx, ey = (xa.y - xb.y) + dmy, ez = xa.z - xb.z;
     // ((e + 0.0)^2 == e^2: d2 equals dot(xab, xab) of src/contactsearch.cuh:242-248)
     const double d2 = ex * ex + ey * ey + ez * ez;
@@ -660,7 +694,7 @@ __device__ __noinline__ void rows_try(const double4* __restrict__ ax, const uint
         st.loaded = true;
     }
     // slot rule of src/contactsearch.cuh:252-271
-    const uint32_t bo = aid[sb];
+    const uint32_t bo = sid[sb];
     int cpos = -1;
     for (uint32_t m = st.mask; m; m &= m - 1) {
         const int i = __ffs(m) - 1;
@@ -702,7 +736,7 @@ __device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t 
             for (; hits; hits &= hits - 1) {
                 const uint32_t sb = sb0 + (uint32_t)(__ffs(hits) - 1);
                 if (sb == s) continue;
-                rows_try(d.a.x, d.a.id, d.L[0], d.L[1], sw, gslot, st, xa, sb, code);
+                rows_try(pos_view(d), d.sid, sw, gslot, st, xa, sb, code);
                 if (st.stop) return;
             }
         }
@@ -717,10 +751,10 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
     const uint32_t key = d.skey[s];
     const int gx = (int)(key % d.num[0]), gy = (int)((key / d.num[0]) % d.num[1]), gz = (int)(key / (d.num[0] * d.num[1]));
-    const double4 xa = d.a.x[s];
+    const double4 xa = spos(d, s);
     const float4 xaf = d.xf[s];
     const float ra_eps = xaf.w + d.search_eps;
-    const uint32_t o = d.a.id[s];
+    const uint32_t o = d.sid[s];
     uint32_t* gslot = d.cslot + (size_t)o * NC;
     RowSlots st;
     st.mask = d.ro.cmask[d.perm[s]];
@@ -767,6 +801,177 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     if (st.err) atomicOr(d.flags, st.err);
 }
 
+// ---------------------------------------------------- half-shell search
+// Every unordered pair is tested once, by its lower sorted position, with the
+// same FP32 prefilter and FP64 exact test as k_topo_rows (the exact test is
+// symmetric: swapping a and b negates every component of the distance
+// exactly).  An overlap is recorded in both particles' hit lists under its
+// key in each one's reference visiting order -- cell offset index
+// (dz+1)*9 + (dy+1)*3 + (dx+1), then sorted position (src/contactsearch.cuh:
+// 340-353) -- and k_topo_apply replays each list in key order through the
+// slot rule.
+constexpr uint32_t NO_SPLIT = 0xffffffffu;
+
+__device__ __noinline__ uint32_t half_hit(PosView pv, uint32_t* hcnt, uint64_t* hits, double4 xa, uint32_t s,
+                                          uint32_t sb, uint32_t vi, uint32_t code) {
+    const double4 xb = pv_pos(pv, sb);
+    const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
+    const double dmx = cx == 1u ? pv.Lx : (cx == 2u ? -pv.Lx : 0.0);
+    const double dmy = cy == 1u ? pv.Ly : (cy == 2u ? -pv.Ly : 0.0);
+    const double ex = (xa.x - xb.x) + dmx, ey = (xa.y - xb.y) + dmy, ez = xa.z - xb.z;
+    const double d2 = ex * ex + ey * ey + ez * ez;
+    if (!(sqrt(d2) - (xa.w + xb.w) < 0.0)) return 0u;
+    const uint32_t rx = cx == 1u ? 2u : (cx == 2u ? 1u : 0u), ry = cy == 1u ? 2u : (cy == 2u ? 1u : 0u);
+    const uint32_t rcode = rx | (ry << 2);
+    uint32_t err = 0;
+    const uint32_t k = atomicAdd(&hcnt[s], 1u);
+    if (k < (uint32_t)NC) hits[(size_t)s * NC + k] = ((uint64_t)vi << 40) | ((uint64_t)code << 32) | sb;
+    else err |= FLAG_CONTACT_OVERFLOW;
+    const uint32_t k2 = atomicAdd(&hcnt[sb], 1u);
+    if (k2 < (uint32_t)NC) hits[(size_t)sb * NC + k2] = ((uint64_t)(26u - vi) << 40) | ((uint64_t)rcode << 32) | s;
+    else err |= FLAG_CONTACT_OVERFLOW;
+    return err;
+}
+
+// Candidates (s, hi) of one run; the run's cells start their offset index at
+// vi0 and advance at sorted positions m1 and m2.
+__device__ __forceinline__ uint32_t half_run(const DemDev& d, uint32_t lo, uint32_t hi, uint32_t m1, uint32_t m2,
+                                             uint32_t vi0, float dmx, float dmy, uint32_t code, uint32_t s,
+                                             float4 xaf, float ra_eps, double4 xa) {
+    uint32_t err = 0;
+    if (lo <= s) lo = s + 1;
+    for (uint32_t sb0 = lo; sb0 < hi; sb0 += ROW_BATCH) {
+        float4 c[ROW_BATCH];
+#pragma unroll
+        for (int k = 0; k < ROW_BATCH; ++k)
+            c[k] = sb0 + k < hi ? d.xf[sb0 + k] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+        uint32_t hits = 0;
+#pragma unroll
+        for (int k = 0; k < ROW_BATCH; ++k) {
+            const float ex = (xaf.x - c[k].x) + dmx, ey = (xaf.y - c[k].y) + dmy, ez = xaf.z - c[k].z;
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            const float rs = ra_eps + c[k].w;
+            hits |= d2 <= rs * rs ? 1u << k : 0u;
+        }
+        if (__builtin_expect(hits != 0u, 0)) {
+            for (; hits; hits &= hits - 1) {
+                const uint32_t sb = sb0 + (uint32_t)(__ffs(hits) - 1);
+                const uint32_t vi = vi0 + (sb >= m1 ? 1u : 0u) + (sb >= m2 ? 1u : 0u);
+                err |= half_hit(pos_view(d), d.hcnt, d.hits, xa, s, sb, vi, code);
+            }
+        }
+    }
+    return err;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_topo_half(DemDev d) {
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= d.np) return;
+    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
+    const uint32_t key = d.skey[s];
+    const int gx = (int)(key % d.num[0]), gy = (int)((key / d.num[0]) % d.num[1]), gz = (int)(key / (d.num[0] * d.num[1]));
+    const double4 xa = spos(d, s);
+    const float4 xaf = d.xf[s];
+    const float ra_eps = xaf.w + d.search_eps;
+    uint32_t err = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int tz = gz + dz;
+        if (tz < 0 || tz == nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int ty = gy + dy;
+            uint32_t cy = 0;
+            if (ty < 0 || ty == ny) {
+                if (d.periodic != 1) continue;
+                cy = ty < 0 ? 1u : 2u;
+                ty = ty < 0 ? ny - 1 : 0;
+            }
+            const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+            const uint32_t vrow = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3);
+            if (gx >= 1 && gx + 1 < nx && cy == 0) {
+                const uint32_t hi = d.off[row + gx + 2];
+                if (hi <= s + 1) continue;  // the whole run lies at or below s
+                err |= half_run(d, d.off[row + gx - 1], hi, d.off[row + gx], d.off[row + gx + 1], vrow, 0.0f, 0.0f,
+                                0u, s, xaf, ra_eps, xa);
+                continue;
+            }
+            for (int dx = -1; dx <= 1; ++dx) {
+                int tx = gx + dx;
+                uint32_t cx = 0;
+                if (tx < 0 || tx == nx) {
+                    if (d.periodic == 0) continue;
+                    cx = tx < 0 ? 1u : 2u;
+                    tx = tx < 0 ? nx - 1 : 0;
+                }
+                const float dmx = cx == 1u ? d.Lf[0] : (cx == 2u ? -d.Lf[0] : 0.0f);
+                const float dmy = cy == 1u ? d.Lf[1] : (cy == 2u ? -d.Lf[1] : 0.0f);
+                err |= half_run(d, d.off[row + tx], d.off[row + tx + 1], NO_SPLIT, NO_SPLIT, vrow + (uint32_t)(dx + 1),
+                                dmx, dmy, cx | (cy << 2), s, xaf, ra_eps, xa);
+            }
+        }
+    }
+    if (err) atomicOr(d.flags, err);
+}
+
+// Replays each particle's hits in visiting-key order through the slot rule
+// (src/contactsearch.cuh:252-271); the keys (at most NC) are insertion-sorted
+// in place, the slot words cached in LDS.
+constexpr int APPLY_BLOCK = 128;
+
+__global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
+    __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
+    const uint32_t s = blockIdx.x * APPLY_BLOCK + threadIdx.x;
+    if (s >= d.np) return;
+    const uint32_t cnt = d.hcnt[s];
+    const uint32_t mask0 = d.ro.cmask[d.perm[s]];
+    if (cnt == 0u) {
+        d.tmask[s] = mask0;
+        d.tfresh[s] = 0u;
+        return;
+    }
+    const uint32_t n = cnt < (uint32_t)NC ? cnt : (uint32_t)NC;
+    uint64_t* k = d.hits + (size_t)s * NC;
+    for (uint32_t i = 1; i < n; ++i) {
+        const uint64_t v = k[i];
+        uint32_t j = i;
+        while (j > 0 && k[j - 1] > v) { k[j] = k[j - 1]; --j; }
+        k[j] = v;
+    }
+    uint32_t* sw = sw_all + threadIdx.x;
+    const uint32_t o = d.sid[s];
+    uint32_t* gslot = d.cslot + (size_t)o * NC;
+    for (uint32_t m = mask0; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        sw[i * APPLY_BLOCK] = gslot[i];
+    }
+    uint32_t mask = mask0, fresh = 0, dirty = 0, err = 0;
+    for (uint32_t h = 0; h < n; ++h) {
+        const uint64_t v = k[h];
+        const uint32_t sb = (uint32_t)(v & 0xffffffffu), code = (uint32_t)(v >> 32) & 15u;
+        const uint32_t bo = d.sid[sb];
+        int cpos = -1;
+        for (uint32_t m = mask; m; m &= m - 1) {
+            const int i = __ffs(m) - 1;
+            if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+        }
+        if (cpos < 0) {
+            const uint32_t empty = ~mask;
+            if (empty == 0u) { err |= FLAG_CONTACT_OVERFLOW; break; }
+            cpos = 31 - __clz(empty);
+            fresh |= 1u << cpos;
+        }
+        sw[cpos * APPLY_BLOCK] = bo | (code << 28);
+        dirty |= 1u << cpos;
+        mask |= 1u << cpos;
+    }
+    for (uint32_t m = dirty; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        gslot[i] = sw[i * APPLY_BLOCK];
+    }
+    d.tmask[s] = mask;
+    d.tfresh[s] = fresh;
+    if (err) atomicOr(d.flags, err);
+}
+
 // One wavefront per cell.  The candidates of the cell's 27-cell neighbourhood,
 // in the reference's visiting order (z, y, x cells, sorted order within a
 // cell; src/contactsearch.cuh:340-353), are up to 27 runs of the sorted
@@ -803,9 +1008,9 @@ __device__ __forceinline__ Cand load_cand(const DemDev& d, uint32_t t, uint32_t 
     c.sb = sb;
     c.code = code;
     if (c.ok) {
-        const double4 xb = d.a.x[sb];
+        const double4 xb = spos(d, sb);
         c.x = xb.x; c.y = xb.y; c.z = xb.z; c.r = xb.w;
-        c.bid = d.a.id[sb];
+        c.bid = d.sid[sb];
     } else {
         c.x = c.y = c.z = c.r = 0.0;
         c.bid = 0;
@@ -909,8 +1114,8 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
         cand[j] = load_cand(d, 64u * j + lane, T, runs, r_lo, r_pre, r_len, r_code);
 
     for (uint32_t m = m0; m < m1; ++m) {
-        const double4 xa = d.a.x[m];
-        const uint32_t o = d.a.id[m];
+        const double4 xa = spos(d, m);
+        const uint32_t o = d.sid[m];
         const uint32_t* slot = d.cslot + (size_t)o * NC;
         MemberState st;
         st.mask = d.ro.cmask[d.perm[m]];
@@ -940,8 +1145,10 @@ template <class T>
 __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
-    const uint32_t o = d.a.id[s];
-    const double4 xa = d.a.x[s];
+    const uint32_t o = d.sid[s];
+    const uint32_t prev = d.perm[s];  // where this particle's records are stored
+    double4 xa = d.ro.x[prev];
+    (void)wrap_xy(d, xa.x, xa.y);
     const double ra = xa.w;
 
     // contact slots after the search kernel
@@ -950,8 +1157,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     const uint32_t* slot = d.cslot + (size_t)o * NC;
 
     // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
-    const double4 va4 = d.a.vel[s];
-    const V3<double> va = xyz(va4), wa = xyz(d.a.angvel[s]);
+    const double4 va4 = d.ro.vel[prev];
+    const V3<double> va = xyz(va4), wa = xyz(d.ro.angvel[prev]);
     Acc acc;
     acc.F = mk(0.0, 0.0, 0.0);
     acc.T = mk(0.0, 0.0, 0.0);
@@ -960,8 +1167,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         const int i = __ffs(m) - 1;
         const uint32_t w = slot[i];
         const uint32_t bo = w & SLOT_IDX_MASK;
-        const uint32_t sb = d.pos_of[bo];
-        const double4 xb = d.a.x[sb];
+        const uint32_t pb = d.perm[d.pos_of[bo]];
+        double4 xb = d.ro.x[pb];
+        (void)wrap_xy(d, xb.x, xb.y);
         double dmx, dmy;
         decode_image(d, w >> 28, dmx, dmy);
         const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
@@ -970,7 +1178,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         if (delta_n < 0.0) {
             const size_t mp = (size_t)o * NC + i;
             V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
-            contact_linear<T>(d, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.a.vel[sb])), cvt<T>(xyz(d.a.angvel[sb])),
+            contact_linear<T>(d, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.ro.vel[pb])), cvt<T>(xyz(d.ro.angvel[pb])),
                               (T)ra, (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
             SlotStore<T>::store(d, mp, dt0);
         } else {
@@ -1006,7 +1214,6 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
 
     // ---- per-particle outputs (src/contactsearch.cuh:669-687)
     const double dt = d.dt;
-    const uint32_t prev = d.perm[s];  // position of the private records
     double4 ea = d.ro.acc[prev];  // xysum.x, xysum.y, es, ev
     ea.z += acc.es_dot * dt;
     ea.w += acc.ev_dot * dt;
@@ -1060,13 +1267,13 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     ea.x += sx;
     ea.y += sy;
     d.wo.acc[s] = ea;
-    d.b.angvel[s] = make_double4(wx, wy, wz, 0.0);
+    d.wo.angvel[s] = make_double4(wx, wy, wz, 0.0);
     d.wo.angvel0[s] = w0;
-    d.b.vel[s] = make_double4(vx, vy, vz, va4.w);
+    d.wo.vel[s] = make_double4(vx, vy, vz, va4.w);
     d.wo.vel0[s] = v0;
     d.wo.angpos[s] = ap;
-    d.b.x[s] = make_double4(px, py, pz, radius);
-    d.b.id[s] = o;
+    d.wo.x[s] = make_double4(px, py, pz, radius);
+    d.wo.id[s] = o;
     d.wo.cmask[s] = mask;
 
     // cell id of the new position for the next step's sort
@@ -1177,7 +1384,11 @@ void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BL
 void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
 void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_topology(const DemDev& d, hipStream_t s) {
-    if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
+    if (d.search_mode == 4) {
+        k_fill_u32<<<nblk(d.np), BLOCK, 0, s>>>(d.hcnt, d.np, 0u);
+        k_topo_half<<<nblk(d.np), BLOCK, 0, s>>>(d);
+        k_topo_apply<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
+    } else if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
     else if (d.search_mode == 2) k_topo_rows<<<nblk(d.np), BLOCK, 0, s>>>(d);
     else k_topo<<<d.tiles_x * d.tiles_y * d.num[2], BLOCK, 0, s>>>(d);
 }

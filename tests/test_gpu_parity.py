@@ -108,6 +108,7 @@ def test_bonded_lattice_parity(tmp_path):
 SEARCH_VARIANTS = [
     ("rows", lambda d: d.set_search(2)),
     ("wave", lambda d: d.set_search(3)),
+    ("half", lambda d: d.set_search(4)),
     ("tiles-auto", lambda d: d.set_search(1)),
     ("tiles-1x1", lambda d: (d.set_search(1), d.set_tile(1, 1))),
     ("tiles-wide", lambda d: (d.set_search(1), d.set_tile(6, 3))),
@@ -118,7 +119,7 @@ COARSE_GRIDS = [((2, 2, 2), 0), ((1, 3, 2), 0), ((2, 1, 1), 0), ((3, 3, 2), 1), 
                 ((3, 1, 2), 2), ((3, 2, 2), 2)]
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("num,periodic", COARSE_GRIDS, ids=[f"{g[0]}-p{g[1]}" for g in COARSE_GRIDS])
 def test_coarse_grid_search_parity(tmp_path, mode, num, periodic):
     """Cells far larger than the spheres: 10-40 members per cell, more than
