@@ -131,7 +131,7 @@ def timed_run(bin_path, precision, steps, warmup, prof_steps):
     dem.set_profiling(False)
     info = dem.info()
     state = {"x": dem.get("x")[:, :3].copy(), "mvfd": dem.get("walls_mvfd") if info.nw else None,
-             "L": dem.get("time")}  # time kept for the record
+             "wall_pp0": dem.get("wall_pp")[0].copy() if info.nw else None}
     dem.close()
     stage_ms = {k: tim[f"{k}_ms"] / max(1, tim["steps"]) for k in STAGES}
     return {"ms": ms, "contacts": c, "stage_ms": stage_ms, "tile": [tim["tile_x"], tim["tile_y"]],
@@ -143,9 +143,11 @@ def mixed_error(ref, mix, area, r_min):
     dx = np.abs(mix["state"]["x"] - ref["state"]["x"])
     out = {"steps": ref["steps_total"], "pos_max_abs_m": float(dx.max()),
            "pos_max_rel_to_r_min": float(dx.max() / r_min)}
-    if ref["state"]["mvfd"] is not None:
-        s_ref = float(ref["state"]["mvfd"][0, 2]) / area
-        s_mix = float(mix["state"]["mvfd"][0, 2]) / area
+    if ref["state"]["wall_pp0"] is not None:
+        # normal stress on the top wall: its per-particle contact forces summed
+        # (the same sum the wall update uses), over the wall area
+        s_ref = float(np.sum(ref["state"]["wall_pp0"])) / area
+        s_mix = float(np.sum(mix["state"]["wall_pp0"])) / area
         out["wall_stress_fp64_Pa"] = s_ref
         out["wall_stress_mixed_Pa"] = s_mix
         out["wall_stress_rel"] = abs(s_mix - s_ref) / abs(s_ref) if s_ref != 0.0 else None

@@ -255,6 +255,8 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.key, 0, n * 4));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
     HIP_TRY(hipMemset(d.off, 0, ((size_t)d.K + 1) * 4));
+    HIP_TRY(hipMemset(d.count, 0, (size_t)d.K * 4));
+    HIP_TRY(hipMemset(d.hcnt, 0, n * 4));
     HIP_TRY(hipMemcpy(d.a.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
     // walls: entries past nw keep the defaults of src/contactsearch.cuh:416-427

@@ -238,7 +238,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
     uint32_t off = block_exclusive_scan(sum, lds, total) + d.scan_bsum[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < SCAN_PER_THREAD; ++j) {
-        if (base + j < d.K) d.off[base + j] = off;
+        if (base + j < d.K) {
+            d.off[base + j] = off;
+            d.count[base + j] = 0u;  // zero for the next step's counts
+        }
         off += c[j];
         if (base + j + 1 == d.K) d.off[d.K] = off;
     }
@@ -923,6 +926,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
     if (s >= d.np) return;
     const uint32_t cnt = d.hcnt[s];
     const uint32_t mask0 = d.ro.cmask[d.perm[s]];
+    if (cnt != 0u) d.hcnt[s] = 0u;  // zero for the next search
     if (cnt == 0u) {
         d.tmask[s] = mask0;
         d.tfresh[s] = 0u;
@@ -1371,10 +1375,8 @@ void launch_bonds(const DemDev& d, hipStream_t s) {
     if (d.nb0) k_bonds<<<nblk(d.nb0), BLOCK, 0, s>>>(d);
 }
 void launch_cellid(const DemDev& d, hipStream_t s) { k_cellid<<<nblk(d.np), BLOCK, 0, s>>>(d); }
-void launch_count(const DemDev& d, hipStream_t s) {
-    k_fill_u32<<<nblk(d.K), BLOCK, 0, s>>>(d.count, d.K, 0u);
-    k_count<<<nblk(d.np), BLOCK, 0, s>>>(d);
-}
+// (the counts start at zero: k_scan_down clears them after use)
+void launch_count(const DemDev& d, hipStream_t s) { k_count<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_scan(const DemDev& d, hipStream_t s) {
     k_scan_reduce<<<d.scan_blocks, BLOCK, 0, s>>>(d);
     k_scan_top<<<1, BLOCK, 0, s>>>(d);
@@ -1384,8 +1386,7 @@ void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BL
 void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
 void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_topology(const DemDev& d, hipStream_t s) {
-    if (d.search_mode == 4) {
-        k_fill_u32<<<nblk(d.np), BLOCK, 0, s>>>(d.hcnt, d.np, 0u);
+    if (d.search_mode == 4) {  // hcnt starts at zero: k_topo_apply clears it
         k_topo_half<<<nblk(d.np), BLOCK, 0, s>>>(d);
         k_topo_apply<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
     } else if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);

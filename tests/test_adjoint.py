@@ -9,11 +9,20 @@ import pytest
 
 from sphere_amd import configs
 
-THETA0 = (1e6, 1e6, 0.5, 0.5)  # k_n, k_t, gamma_n, mu
+THETA0 = (1e6, 1e6, 20.0, 0.5)  # k_n, k_t, gamma_n, mu
+# Central differences of a dense granular run are only as good as the run is
+# smooth in theta: over many steps the roundoff of J(theta +- h) is amplified
+# by the chaotic dynamics and some contact or slip branch flips, so the check
+# uses a short horizon (the adjoint itself has no horizon limit).
+FD_STEPS = 30
 
 
 def _case():
     return configs.fcc_shear(cells=(4, 4, 8), gamma_n=THETA0[2], sid="adj-fcc-shear")
+
+
+def _relerr(g, fd):
+    return np.abs(g - fd) / np.abs(fd)
 
 
 def _say(msg):
@@ -50,19 +59,17 @@ def test_adjoint_matches_central_differences():
     from sphere_amd.adjoint import shear_stress_weights
     from sphere_amd.dem import DEM
 
-    _say("fp64 adjoint vs central differences, 512 spheres, 200 steps")
+    _say(f"fp64 adjoint vs central differences, 512 spheres, {FD_STEPS} steps")
     sb = _case()
     w = shear_stress_weights(sb)
-    nsteps = 200
     dem = DEM(sb, verbosity=0)
     dem.set_params(k_n=THETA0[0], k_t=THETA0[1], gamma_n=THETA0[2], mu_s=THETA0[3], mu_d=THETA0[3])
-    J = dem.adjoint_forward(nsteps, w, seg_len=64)  # 4 segments, the last partial
+    J = dem.adjoint_forward(FD_STEPS, w, seg_len=8)  # 4 segments, the last partial
     g = dem.adjoint_backward()
     dem.close()
-    fd = _fd(sb, w, nsteps)
-    _say(f"J={J:.6e} adjoint={g} fd={fd}")
-    scale = np.abs(fd).max()
-    err = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    fd = _fd(sb, w, FD_STEPS)
+    err = _relerr(g, fd)
+    _say(f"J={J:.6e} adjoint={g} fd={fd} relative error={err}")
     assert np.all(err <= 1e-4), f"relative error {err}"
 
 
@@ -83,8 +90,10 @@ def test_autograd_function_gives_the_adjoint():
     J2 = dem.adjoint_forward(120, shear_stress_weights(sb), seg_len=50)
     g2 = dem.adjoint_backward()
     dem.close()
-    assert float(J) == J2
-    assert np.array_equal(theta.grad.numpy(), g2)
+    # partner contributions and block sums are FP64 atomics, so the last
+    # bits depend on their order
+    assert abs(float(J) - J2) <= 1e-12 * abs(J2)
+    np.testing.assert_allclose(theta.grad.numpy(), g2, rtol=1e-10, atol=0.0)
 
 
 @pytest.mark.gpu
@@ -120,15 +129,14 @@ def test_mixed_adjoint_close_to_fp64_differences():
     _say("mixed adjoint vs fp64 central differences")
     sb = _case()
     w = shear_stress_weights(sb)
-    nsteps = 200
     dem = DEM(sb, verbosity=0)
     dem.set_precision("mixed")
     dem.set_params(k_n=THETA0[0], k_t=THETA0[1], gamma_n=THETA0[2], mu_s=THETA0[3], mu_d=THETA0[3])
-    dem.adjoint_forward(nsteps, w, seg_len=64)
+    dem.adjoint_forward(FD_STEPS, w, seg_len=8)
     g = dem.adjoint_backward()
     dem.close()
-    fd = _fd(sb, w, nsteps)
-    err = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    fd = _fd(sb, w, FD_STEPS)
+    err = _relerr(g, fd)
     _say(f"mixed adjoint={g} fp64 fd={fd} rel err={err}")
     assert np.all(err <= 1e-2), f"relative error {err}"
 
