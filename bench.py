@@ -43,8 +43,9 @@ STAGE_KERNELS = {
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
 PROFILE_KERNEL = {"reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force<double>", "cellid": "k_count"}
-SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply"}
-SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half"}
+SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply",
+                  5: "k_list_search (k_topo_rows<true> on rebuild steps)"}
+SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false>", 3: "k_topo_wave", 4: "k_topo_half", 5: "k_list_search"}
 
 
 def log(msg):

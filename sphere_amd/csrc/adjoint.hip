@@ -641,6 +641,8 @@ int restore_state(sph_dem* h, const Checkpoint& c) {
     h->own_cur = c.own_cur;
     h->hs.t_cur = c.t_cur;
     h->steps_done = c.steps_done;
+    // neighbour lists (search mode 5) describe another state: rebuild
+    launch_fill_u32(h->d.nstate + 1, 1, 1u, h->stream);
     return SPH_OK;
 }
 

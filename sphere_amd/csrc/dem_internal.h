@@ -40,6 +40,7 @@ constexpr int SCAN_PER_THREAD = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_PER_THREAD;
 constexpr int TOPO_CAP = 1024;          // staged neighbour particles per search block
 constexpr int TOPO_MAX_CELLS = 384;     // staged cells per search block
+constexpr int NLIST = 48;               // neighbour-list entries per particle (search mode 5)
 
 // device error flags (atomicOr'd by kernels, read after each sph_step chunk)
 enum : uint32_t {
@@ -59,12 +60,13 @@ struct PState {
     double4 *acc;     // xysum.x, xysum.y, es, ev
     uint32_t *id;     // original index
     uint32_t *cmask;  // occupied contact slots
+    uint32_t *lidx;   // search mode 5: the particle's neighbour list (its sorted position at the build)
 };
 
 // The records of one buffer set, as a step reads (ro) or writes (wo) them.
 struct OwnSet {
     double4 *x, *vel, *angvel, *vel0, *angvel0, *angpos, *acc;
-    uint32_t *id, *cmask;
+    uint32_t *id, *cmask, *lidx;
 };
 
 // Everything a step kernel needs, passed by value as one kernel argument.
@@ -101,11 +103,18 @@ struct DemDev {
 
     // contact search tiles: TX x TY cells of one z plane per block
     uint32_t tile_x, tile_y, tiles_x, tiles_y;
-    int search_mode;    // 1: LDS tiles, 2: rows per particle, 3: wave per cell, 4: half shell
+    int search_mode;    // 1: LDS tiles, 2: rows per particle, 3: wave per cell, 4: half shell, 5: lists
     uint32_t* skey;     // cell id per sorted position
     float4* xf;         // FP32 copy of a.x for the search's distance prefilter
     float search_eps;   // prefilter margin (m), above the FP32 rounding of a distance
     float Lf[2];        // L[0], L[1] in FP32 (periodic images in the prefilter)
+    uint32_t* nlist;    // mode 5: np * NLIST partners (their list index), list index = sorted position at the build
+    uint32_t* ncnt;     // mode 5: list length per list index
+    float4* xref;       // mode 5: position at the last list build, per list index
+    uint32_t* linv;     // mode 5: list index -> sorted position of this step
+    uint32_t* slidx;    // mode 5: list index per sorted position of this step
+    uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed
+    float list_skin;    // mode 5: list reach beyond r_a + r_b (m)
     uint32_t* hcnt;     // half-shell search: overlaps found per particle
     uint64_t* hits;     // np * NC keys: offset index << 40 | image << 32 | sorted position
     uint32_t* tmask;    // slot occupancy after the search, sorted order

@@ -65,7 +65,7 @@ void destroy_graph(sph_dem* h) {
 }
 
 sph::OwnSet own_set(const sph::PState& p) {
-    return {p.x, p.vel, p.angvel, p.vel0, p.angvel0, p.angpos, p.acc, p.id, p.cmask};
+    return {p.x, p.vel, p.angvel, p.vel0, p.angvel0, p.angpos, p.acc, p.id, p.cmask, p.lidx};
 }
 
 // the buffer set holding the current private records
@@ -83,6 +83,12 @@ void update_search_eps(sph_dem* h) {
     for (uint32_t i = 0; i < s.np; ++i) rmax = std::max(rmax, s.x[4 * (size_t)i + 3]);
     const double ulp = std::ldexp(1.0, std::ilogb(std::max(M, 1e-30)) - 23);
     h->d.search_eps = (float)(16.0 * ulp + 2e-6 * rmax);
+    // neighbour-list reach beyond r_a + r_b: a tenth of the smallest cell
+    // (at least 2 r_max / 10), so lists hold the touching and nearly touching
+    // spheres and a rebuild is due after 0.45 of that displacement
+    double cmin = s.L[0] / s.num[0];
+    for (int a = 1; a < 3; ++a) cmin = std::min(cmin, s.L[a] / s.num[a]);
+    h->d.list_skin = (float)(0.1 * cmin);
     h->d.Lf[0] = (float)s.L[0];
     h->d.Lf[1] = (float)s.L[1];
 }
@@ -257,6 +263,13 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.off, 0, ((size_t)d.K + 1) * 4));
     HIP_TRY(hipMemset(d.count, 0, (size_t)d.K * 4));
     HIP_TRY(hipMemset(d.hcnt, 0, n * 4));
+    HIP_TRY(hipMemset(d.ncnt, 0, n * 4));
+    HIP_TRY(hipMemset(d.a.lidx, 0, n * 4));
+    HIP_TRY(hipMemset(d.b.lidx, 0, n * 4));
+    {
+        const uint32_t ns[4] = {0u, 1u, 0u, 0u};  // neighbour lists: rebuild at the first step
+        HIP_TRY(hipMemcpy(d.nstate, ns, 16, hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy(d.a.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
     // walls: entries past nw keep the defaults of src/contactsearch.cuh:416-427
@@ -334,7 +347,7 @@ int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) {
 
 int alloc_pstate(sph_dem* h, sph::PState& p, size_t n) {
     ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.vel0, n); ALLOC(p.angvel0, n);
-    ALLOC(p.angpos, n); ALLOC(p.acc, n); ALLOC(p.id, n); ALLOC(p.cmask, n);
+    ALLOC(p.angpos, n); ALLOC(p.acc, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n);
     return SPH_OK;
 }
 
@@ -350,6 +363,7 @@ int allocate(sph_dem* h) {
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
     ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
     ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
+    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
     ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
@@ -431,6 +445,10 @@ int prepare_keys(sph_dem* h) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     return SPH_OK;
 }
+
+// Neighbour lists (search mode 5) no longer describe the state: rebuild them
+// at the next step.
+void request_list_rebuild(sph_dem* h) { sph::launch_fill_u32(h->d.nstate + 1, 1, 1u, h->stream); }
 
 int ensure_graph(sph_dem* h) {
     for (int l = 0; l < 2; ++l)
@@ -570,17 +588,21 @@ int make_events(sph_dem* h) {
     return SPH_OK;
 }
 
-// Times the contact search of two steps with each of the row (2) and
-// half-shell (4) kernels and keeps the faster; the steps are ordinary steps
-// of the run (both kernels give the same bits).
+// Times the contact search of four steps with each of the row (2),
+// half-shell (4) and neighbour-list (5) kernels and keeps the fastest by the
+// mean of the last three (the first warms up, or builds the lists); the steps
+// are ordinary steps of the run (all kernels give the same bits).
+constexpr uint64_t TUNE_STEPS = 12;
+
 int tune_search(sph_dem* h) {
     int rc;
     if ((rc = make_events(h))) return rc;
-    const int modes[2] = {2, 4};
-    float best[2] = {1e30f, 1e30f};
-    for (int k = 0; k < 2; ++k) {
+    const int modes[3] = {2, 4, 5};
+    float mean[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 3; ++k) {
         h->d.search_mode = modes[k];
-        for (int rep = 0; rep < 2; ++rep) {
+        if (modes[k] == 5) request_list_rebuild(h);
+        for (int rep = 0; rep < 4; ++rep) {
             sph::enqueue_step(h, true, h->own_cur, false, true);
             h->own_cur ^= 1;
             h->hs.t_cur += h->hs.dt;
@@ -588,16 +610,20 @@ int tune_search(sph_dem* h) {
             HIP_TRY(hipEventSynchronize(h->ev[7]));
             float ms = 0.0f;
             HIP_TRY(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
-            best[k] = std::min(best[k], ms);
+            if (rep > 0) mean[k] += ms / 3.0f;
         }
     }
-    h->d.search_mode = best[1] < best[0] ? 4 : 2;
+    int best = 0;
+    for (int k = 1; k < 3; ++k)
+        if (mean[k] < mean[best]) best = k;
+    h->d.search_mode = modes[best];
+    if (modes[best] == 5) request_list_rebuild(h);
     destroy_graph(h);
     h->tuned = true;
     h->steps_since_tune = 0;
     if (h->verbosity > 1)
-        std::printf("sphere_dem: contact search %.4f ms (rows) vs %.4f ms (half shell): mode %d\n", best[0], best[1],
-                    h->d.search_mode);
+        std::printf("sphere_dem: contact search %.4f (rows) %.4f (half shell) %.4f (lists) ms: mode %d\n", mean[0],
+                    mean[1], mean[2], h->d.search_mode);
     return check_flags(h);
 }
 
@@ -629,9 +655,9 @@ int run_steps(sph_dem* h, uint64_t n) {
         h->timing = t;
         return check_flags(h);
     }
-    if (h->search_auto && n >= 8 && (!h->tuned || h->steps_since_tune >= 4096)) {
+    if (h->search_auto && n >= 2 * TUNE_STEPS && (!h->tuned || h->steps_since_tune >= 4096)) {
         if ((rc = tune_search(h))) return rc;
-        n -= 4;
+        n -= TUNE_STEPS;
     }
     h->steps_since_tune += n;
     if (h->use_graph && (rc = ensure_graph(h))) return rc;
@@ -953,6 +979,7 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
             std::memcpy(h->hs.x.data(), sd, n * 32);  // radii for the search margin
             update_search_eps(h);
             destroy_graph(h);
+            request_list_rebuild(h);
             return prepare_keys(h);
         }
         case SPH_VEL: return scatter(own(h).vel, 0, 4);
@@ -1061,10 +1088,11 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y) {
 }
 
 int sph_set_search(sph_t* h, int mode) {
-    if (!h || mode < 0 || mode > 4) return SPH_E_INVALID;
+    if (!h || mode < 0 || mode > 5) return SPH_E_INVALID;
     h->search_auto = mode == 0;
     h->tuned = false;
     h->d.search_mode = mode == 0 ? 2 : mode;
+    if (mode == 5) request_list_rebuild(h);
     destroy_graph(h);
     return SPH_OK;
 }

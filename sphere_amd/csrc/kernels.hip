@@ -204,13 +204,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
 
 __global__ __launch_bounds__(BLOCK) void k_scan_top(DemDev d) {
     __shared__ uint32_t lds[BLOCK / 64];
-    // the cell-id errors belong to this step
+    // the cell-id errors belong to this step; so does a neighbour-list
+    // rebuild requested by the last step's force kernel (or a failed build)
     if (threadIdx.x == 0) {
         const uint32_t f = *d.flags_next;
         if (f) {
             atomicOr(d.flags, f);
             *d.flags_next = 0u;
         }
+        d.nstate[0] = d.nstate[1] | d.nstate[2];
+        d.nstate[1] = 0u;
+        d.nstate[2] = 0u;
     }
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < d.scan_blocks; b0 += BLOCK) {
@@ -308,6 +312,11 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     d.sid[q] = id;
     d.pos_of[id] = q;
     d.skey[q] = d.key[p];
+    if (d.search_mode == 5) {
+        const uint32_t li = d.ro.lidx[p];
+        d.linv[li] = q;
+        d.slidx[q] = li;
+    }
 }
 
 // ------------------------------------------------------------------- bonds
@@ -719,21 +728,37 @@ __device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ s
 // of ROW_BATCH independent loads; overlaps go to rows_try in candidate order.
 constexpr int ROW_BATCH = 8;
 
+// With BUILD, the pass also records every candidate within the list reach
+// (r_a + r_b + skin, same prefilter margin) in the particle's neighbour list.
+template <bool BUILD>
 __device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t hi, float dmx, float dmy,
-                                         uint32_t code, uint32_t s, float4 xaf, float ra_eps, double4 xa,
-                                         uint32_t* sw, const uint32_t* gslot, RowSlots& st) {
+                                         uint32_t code, uint32_t s, float4 xaf, float ra_eps, float ra_skin,
+                                         double4 xa, uint32_t* sw, const uint32_t* gslot, RowSlots& st,
+                                         uint32_t* list, uint32_t& lcnt) {
     for (uint32_t sb0 = lo; sb0 < hi; sb0 += ROW_BATCH) {
         float4 c[ROW_BATCH];
 #pragma unroll
         for (int k = 0; k < ROW_BATCH; ++k)
             c[k] = sb0 + k < hi ? d.xf[sb0 + k] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
-        uint32_t hits = 0;
+        uint32_t hits = 0, near = 0;
 #pragma unroll
         for (int k = 0; k < ROW_BATCH; ++k) {
             const float ex = (xaf.x - c[k].x) + dmx, ey = (xaf.y - c[k].y) + dmy, ez = xaf.z - c[k].z;
             const float d2 = ex * ex + ey * ey + ez * ez;
             const float rs = ra_eps + c[k].w;
             hits |= d2 <= rs * rs ? 1u << k : 0u;
+            if constexpr (BUILD) {
+                const float rl = ra_skin + c[k].w;
+                near |= d2 <= rl * rl ? 1u << k : 0u;
+            }
+        }
+        if constexpr (BUILD) {
+            for (; near; near &= near - 1) {
+                const uint32_t sb = sb0 + (uint32_t)(__ffs(near) - 1);
+                if (sb == s) continue;
+                if (lcnt < (uint32_t)NLIST) list[(size_t)lcnt * d.np] = sb;  // the partner's list index
+                ++lcnt;
+            }
         }
         if (__builtin_expect(hits != 0u, 0)) {
             for (; hits; hits &= hits - 1) {
@@ -746,8 +771,12 @@ __device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t 
     }
 }
 
+// BUILD (search mode 5): runs only on steps whose neighbour lists must be
+// rebuilt, and then also rebuilds the lists.
+template <bool BUILD>
 __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     __shared__ uint32_t sw_all[NC * BLOCK];  // slot i of thread t at [i * BLOCK + t]
+    if (BUILD && d.nstate[0] == 0u) return;
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
     uint32_t* sw = sw_all + threadIdx.x;
@@ -757,8 +786,11 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     const double4 xa = spos(d, s);
     const float4 xaf = d.xf[s];
     const float ra_eps = xaf.w + d.search_eps;
+    const float ra_skin = ra_eps + d.list_skin;
     const uint32_t o = d.sid[s];
     uint32_t* gslot = d.cslot + (size_t)o * NC;
+    uint32_t* list = BUILD ? d.nlist + s : nullptr;  // entry k at list[k * np]
+    uint32_t lcnt = 0;
     RowSlots st;
     st.mask = d.ro.cmask[d.perm[s]];
     st.fresh = 0; st.nc = 0; st.err = 0; st.dirty = 0; st.stop = false; st.loaded = false;
@@ -776,8 +808,8 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
             const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
             if (gx >= 1 && gx + 1 < nx && cy == 0) {
                 // no periodic image anywhere in the run: one contiguous range
-                rows_run(d, d.off[row + gx - 1], d.off[row + gx + 2], 0.0f, 0.0f, 0u, s, xaf, ra_eps, xa, sw,
-                         gslot, st);
+                rows_run<BUILD>(d, d.off[row + gx - 1], d.off[row + gx + 2], 0.0f, 0.0f, 0u, s, xaf, ra_eps,
+                                ra_skin, xa, sw, gslot, st, list, lcnt);
                 continue;
             }
             for (int dx = -1; dx <= 1 && !st.stop; ++dx) {
@@ -790,8 +822,8 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
                 }
                 const float dmx = cx == 1u ? d.Lf[0] : (cx == 2u ? -d.Lf[0] : 0.0f);
                 const float dmy = cy == 1u ? d.Lf[1] : (cy == 2u ? -d.Lf[1] : 0.0f);
-                rows_run(d, d.off[row + tx], d.off[row + tx + 1], dmx, dmy, cx | (cy << 2), s, xaf, ra_eps, xa,
-                         sw, gslot, st);
+                rows_run<BUILD>(d, d.off[row + tx], d.off[row + tx + 1], dmx, dmy, cx | (cy << 2), s, xaf,
+                                ra_eps, ra_skin, xa, sw, gslot, st, list, lcnt);
             }
         }
     }
@@ -802,6 +834,11 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     d.tmask[s] = st.mask;
     d.tfresh[s] = st.fresh;
     if (st.err) atomicOr(d.flags, st.err);
+    if constexpr (BUILD) {  // list index = this sorted position
+        d.ncnt[s] = lcnt < (uint32_t)NLIST ? lcnt : (uint32_t)NLIST;
+        d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
+        if (lcnt > (uint32_t)NLIST || st.stop) atomicOr(&d.nstate[2], 1u);  // list unusable: rebuild next step
+    }
 }
 
 // ---------------------------------------------------- half-shell search
@@ -920,6 +957,63 @@ __global__ __launch_bounds__(BLOCK) void k_topo_half(DemDev d) {
 // in place, the slot words cached in LDS.
 constexpr int APPLY_BLOCK = 128;
 
+// Applies the n (<= NC) hit keys k of sorted particle s through the slot
+// rule as if in key order.  A partner occurs at most once among the hits, so
+// a hit whose partner already holds a slot lands in that slot whatever the
+// order; only the new contacts take empty slots in key order, and only they
+// are sorted.  sw is the particle's LDS column of slot words (stride
+// APPLY_BLOCK).
+__device__ __forceinline__ void apply_hits(const DemDev& d, uint32_t s, uint64_t* k, uint32_t n, uint32_t mask0,
+                                           uint32_t* sw, uint32_t err) {
+    const uint32_t o = d.sid[s];
+    uint32_t* gslot = d.cslot + (size_t)o * NC;
+    for (uint32_t m = mask0; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        sw[i * APPLY_BLOCK] = gslot[i];
+    }
+    uint32_t mask = mask0, fresh = 0, dirty = 0, nnew = 0;
+    for (uint32_t h = 0; h < n; ++h) {
+        const uint64_t v = k[h];
+        const uint32_t sb = (uint32_t)(v & 0xffffffffu), code = (uint32_t)(v >> 32) & 15u;
+        const uint32_t bo = d.sid[sb];
+        int cpos = -1;
+        for (uint32_t m = mask0; m; m &= m - 1) {
+            const int i = __ffs(m) - 1;
+            if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+        }
+        if (cpos >= 0) {
+            sw[cpos * APPLY_BLOCK] = bo | (code << 28);
+            dirty |= 1u << cpos;
+        } else {
+            k[nnew++] = v;  // compacted in place (nnew <= h)
+        }
+    }
+    for (uint32_t i = 1; i < nnew; ++i) {
+        const uint64_t v = k[i];
+        uint32_t j = i;
+        while (j > 0 && k[j - 1] > v) { k[j] = k[j - 1]; --j; }
+        k[j] = v;
+    }
+    for (uint32_t h = 0; h < nnew; ++h) {
+        const uint64_t v = k[h];
+        const uint32_t sb = (uint32_t)(v & 0xffffffffu), code = (uint32_t)(v >> 32) & 15u;
+        const uint32_t empty = ~mask;
+        if (empty == 0u) { err |= FLAG_CONTACT_OVERFLOW; break; }
+        const int cpos = 31 - __clz(empty);
+        fresh |= 1u << cpos;
+        sw[cpos * APPLY_BLOCK] = d.sid[sb] | (code << 28);
+        dirty |= 1u << cpos;
+        mask |= 1u << cpos;
+    }
+    for (uint32_t m = dirty; m; m &= m - 1) {
+        const int i = __ffs(m) - 1;
+        gslot[i] = sw[i * APPLY_BLOCK];
+    }
+    d.tmask[s] = mask;
+    d.tfresh[s] = fresh;
+    if (err) atomicOr(d.flags, err);
+}
+
 __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
     const uint32_t s = blockIdx.x * APPLY_BLOCK + threadIdx.x;
@@ -933,37 +1027,160 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
         return;
     }
     const uint32_t n = cnt < (uint32_t)NC ? cnt : (uint32_t)NC;
-    uint64_t* k = d.hits + (size_t)s * NC;
-    for (uint32_t i = 1; i < n; ++i) {
-        const uint64_t v = k[i];
-        uint32_t j = i;
-        while (j > 0 && k[j - 1] > v) { k[j] = k[j - 1]; --j; }
-        k[j] = v;
-    }
+    apply_hits(d, s, d.hits + (size_t)s * NC, n, mask0, sw_all + threadIdx.x, 0u);
+}
+
+// ------------------------------------------------- neighbour-list search
+// Search mode 5.  k_topo_rows<true> rebuilds, per particle, the list of every
+// sphere within r_a + r_b + skin (original indices) and keeps the position of
+// the build; the force kernel requests a rebuild once any sphere has moved
+// more than 0.45 skin since.  Until then no pair outside the lists can
+// overlap (both spheres would have had to close a gap of more than skin), so
+// testing the list pairs finds exactly the overlaps the full search finds.
+// The overlaps are keyed by the reference's visiting order -- the offset of
+// the partner's cell (periodic wrap included, as findDistMod resolves it,
+// src/contactsearch.cuh:12-74) and its sorted position -- and replayed
+// through the slot rule in that order.
+__device__ __forceinline__ bool cell_offset(int g, int b, int n, bool wrap, int& dxo, uint32_t& code) {
+    const int raw = b - g;
+    code = 0u;
+    if (raw >= -1 && raw <= 1) { dxo = raw; return true; }
+    if (wrap && g == 0 && b == n - 1) { dxo = -1; code = 1u; return true; }   // x - 1 wraps to n - 1: +L
+    if (wrap && g == n - 1 && b == 0) { dxo = 1; code = 2u; return true; }    // x + 1 wraps to 0: -L
+    return false;
+}
+
+constexpr int LIST_BATCH = 4;
+
+// Full FP64 treatment of one list partner that passed the prefilter: its cell
+// offset and image as the reference's search would reach it and the exact
+// overlap test; returns the hit key, or 0 for no overlap (a real key is
+// never 0: its sorted position differs from the particle's own).
+__device__ __forceinline__ uint64_t list_pair(const DemDev& d, uint32_t sb, uint32_t kb, double4 x, double4 xa,
+                                              int gx, int gy, int gz) {
+    const uint32_t nx = d.num[0], ny = d.num[1];
+    int dx, dy, dz;
+    uint32_t cx, cy, cz;
+    if (!cell_offset(gz, (int)(kb / (nx * ny)), (int)d.num[2], false, dz, cz)) return 0ull;
+    if (!cell_offset(gy, (int)((kb / nx) % ny), (int)ny, d.periodic == 1, dy, cy)) return 0ull;
+    if (!cell_offset(gx, (int)(kb % nx), (int)nx, d.periodic != 0, dx, cx)) return 0ull;
+    const uint32_t code = cx | (cy << 2);
+    (void)wrap_xy(d, x.x, x.y);
+    const double dmx = cx == 1u ? d.L[0] : (cx == 2u ? -d.L[0] : 0.0);
+    const double dmy = cy == 1u ? d.L[1] : (cy == 2u ? -d.L[1] : 0.0);
+    const double ex = (xa.x - x.x) + dmx, ey = (xa.y - x.y) + dmy, ez = xa.z - x.z;
+    const double d2 = ex * ex + ey * ey + ez * ez;
+    if (!(sqrt(d2) - (xa.w + x.w) < 0.0)) return 0ull;  // exact test of src/contactsearch.cuh:242-248
+    const uint32_t vi = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+    return ((uint64_t)vi << 40) | ((uint64_t)code << 32) | sb;
+}
+
+// An overlap whose partner already holds a slot is applied on the spot (its
+// slot does not depend on the order, a partner occurs once); the others are
+// collected and given empty slots in key order at the end.
+__global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
+    __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
+    if (d.nstate[0] != 0u) return;  // k_topo_rows<true> rebuilds the lists this step
+    const uint32_t s = blockIdx.x * APPLY_BLOCK + threadIdx.x;
+    if (s >= d.np) return;
     uint32_t* sw = sw_all + threadIdx.x;
-    const uint32_t o = d.sid[s];
-    uint32_t* gslot = d.cslot + (size_t)o * NC;
-    for (uint32_t m = mask0; m; m &= m - 1) {
-        const int i = __ffs(m) - 1;
-        sw[i * APPLY_BLOCK] = gslot[i];
+    const uint32_t p = d.perm[s];
+    const float4 xaf = d.xf[s];
+    const float ra_eps = xaf.w + d.search_eps;
+    const uint32_t li = d.slidx[s];
+    const uint32_t* list = d.nlist + li;  // entry k at list[k * np]
+    const uint32_t nl = d.ncnt[li];
+    const uint32_t mask0 = d.ro.cmask[p];
+    const bool px = d.periodic != 0, py = d.periodic == 1;
+    uint64_t* hk = d.hits + (size_t)s * NC;
+    uint32_t n = 0, nnew = 0, err = 0, dirty = 0;
+    bool have_a = false;
+    double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
+    int gx = 0, gy = 0, gz = 0;
+    uint32_t* gslot = nullptr;
+    for (uint32_t i0 = 0; i0 < nl; i0 += LIST_BATCH) {
+        uint32_t sb[LIST_BATCH];
+        float4 c[LIST_BATCH];
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) sb[k] = i0 + k < nl ? d.linv[list[(size_t)(i0 + k) * d.np]] : s;
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) c[k] = d.xf[sb[k]];
+        // FP32 prefilter at the minimum image (an overlap is always closer
+        // than half a period), the same margin as the cell search
+        uint32_t pass = 0;
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) {
+            float ex = xaf.x - c[k].x, ey = xaf.y - c[k].y;
+            const float ez = xaf.z - c[k].z;
+            if (px) ex -= d.Lf[0] * rintf(ex / d.Lf[0]);
+            if (py) ey -= d.Lf[1] * rintf(ey / d.Lf[1]);
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            const float rs = ra_eps + c[k].w;
+            pass |= (i0 + k < nl && d2 <= rs * rs) ? 1u << k : 0u;
+        }
+        if (pass == 0u) continue;
+        if (!have_a) {
+            xa = spos(d, s);
+            const uint32_t key = d.skey[s];
+            gx = (int)(key % d.num[0]);
+            gy = (int)((key / d.num[0]) % d.num[1]);
+            gz = (int)(key / (d.num[0] * d.num[1]));
+            gslot = d.cslot + (size_t)d.sid[s] * NC;
+            for (uint32_t m = mask0; m; m &= m - 1) {
+                const int i = __ffs(m) - 1;
+                sw[i * APPLY_BLOCK] = gslot[i];
+            }
+            have_a = true;
+        }
+        // the batch's keys, FP64 positions and original indices, independent loads
+        uint32_t kb[LIST_BATCH], bo[LIST_BATCH];
+        double4 xb[LIST_BATCH];
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) {
+            kb[k] = d.skey[sb[k]];
+            xb[k] = d.ro.x[d.perm[sb[k]]];
+            bo[k] = d.sid[sb[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) {
+            if (!((pass >> k) & 1u)) continue;
+            const uint64_t v = list_pair(d, sb[k], kb[k], xb[k], xa, gx, gy, gz);
+            if (v == 0ull) continue;
+            ++n;
+            int cpos = -1;
+            for (uint32_t m = mask0; m; m &= m - 1) {
+                const int i = __ffs(m) - 1;
+                if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) { cpos = i; break; }
+            }
+            if (cpos >= 0) {
+                sw[cpos * APPLY_BLOCK] = bo[k] | ((uint32_t)(v >> 32) & 15u) << 28;
+                dirty |= 1u << cpos;
+            } else if (nnew < (uint32_t)NC) {
+                hk[nnew++] = v;
+            }
+        }
     }
-    uint32_t mask = mask0, fresh = 0, dirty = 0, err = 0;
-    for (uint32_t h = 0; h < n; ++h) {
-        const uint64_t v = k[h];
-        const uint32_t sb = (uint32_t)(v & 0xffffffffu), code = (uint32_t)(v >> 32) & 15u;
-        const uint32_t bo = d.sid[sb];
-        int cpos = -1;
-        for (uint32_t m = mask; m; m &= m - 1) {
-            const int i = __ffs(m) - 1;
-            if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
-        }
-        if (cpos < 0) {
-            const uint32_t empty = ~mask;
-            if (empty == 0u) { err |= FLAG_CONTACT_OVERFLOW; break; }
-            cpos = 31 - __clz(empty);
-            fresh |= 1u << cpos;
-        }
-        sw[cpos * APPLY_BLOCK] = bo | (code << 28);
+    if (n > (uint32_t)NC) err |= FLAG_CONTACT_OVERFLOW;
+    if (!have_a) {
+        d.tmask[s] = mask0;
+        d.tfresh[s] = 0u;
+        return;
+    }
+    // new contacts: empty slots in key order
+    for (uint32_t i = 1; i < nnew; ++i) {
+        const uint64_t v = hk[i];
+        uint32_t j = i;
+        while (j > 0 && hk[j - 1] > v) { hk[j] = hk[j - 1]; --j; }
+        hk[j] = v;
+    }
+    uint32_t mask = mask0, fresh = 0;
+    for (uint32_t h = 0; h < nnew; ++h) {
+        const uint64_t v = hk[h];
+        const uint32_t empty = ~mask;
+        if (empty == 0u) { err |= FLAG_CONTACT_OVERFLOW; break; }
+        const int cpos = 31 - __clz(empty);
+        fresh |= 1u << cpos;
+        sw[cpos * APPLY_BLOCK] = d.sid[(uint32_t)(v & 0xffffffffu)] | ((uint32_t)(v >> 32) & 15u) << 28;
         dirty |= 1u << cpos;
         mask |= 1u << cpos;
     }
@@ -1284,6 +1501,20 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     uint32_t cerr = 0;
     d.key[s] = cell_key(d, px, py, pz, cerr);
     if (cerr) atomicOr(d.flags_next, cerr);
+
+    // neighbour lists (search mode 5): the list index travels with the
+    // particle; request a rebuild once it moved 0.45 skin since the build
+    if (d.search_mode == 5) {
+        const uint32_t li = d.nstate[0] ? s : d.slidx[s];
+        d.wo.lidx[s] = li;
+        const float4 xr = d.xref[li];
+        float ddx = (float)px - xr.x, ddy = (float)py - xr.y;
+        const float ddz = (float)pz - xr.z;
+        if (d.periodic == 1 || d.periodic == 2) ddx -= d.Lf[0] * rintf(ddx / d.Lf[0]);
+        if (d.periodic == 1) ddy -= d.Lf[1] * rintf(ddy / d.Lf[1]);
+        const float lim = 0.45f * d.list_skin;
+        if (ddx * ddx + ddy * ddy + ddz * ddz > lim * lim) atomicOr(&d.nstate[1], 1u);
+    }
 }
 
 // ------------------------------------------------------------------- walls
@@ -1386,11 +1617,14 @@ void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BL
 void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
 void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_topology(const DemDev& d, hipStream_t s) {
-    if (d.search_mode == 4) {  // hcnt starts at zero: k_topo_apply clears it
+    if (d.search_mode == 5) {  // one of the two returns at once (nstate[0])
+        k_topo_rows<true><<<nblk(d.np), BLOCK, 0, s>>>(d);
+        k_list_search<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
+    } else if (d.search_mode == 4) {  // hcnt starts at zero: k_topo_apply clears it
         k_topo_half<<<nblk(d.np), BLOCK, 0, s>>>(d);
         k_topo_apply<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
     } else if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
-    else if (d.search_mode == 2) k_topo_rows<<<nblk(d.np), BLOCK, 0, s>>>(d);
+    else if (d.search_mode == 2) k_topo_rows<false><<<nblk(d.np), BLOCK, 0, s>>>(d);
     else k_topo<<<d.tiles_x * d.tiles_y * d.num[2], BLOCK, 0, s>>>(d);
 }
 void launch_forces(const DemDev& d, int precision, hipStream_t s) {

@@ -40,11 +40,44 @@ def test_small_settle_parity(tmp_path, periodic):
     assert not errs, errs
 
 
-def test_c0_settle_parity(tmp_path):
-    """4,096-sphere x/y-periodic settling case, 2,000 steps, checked at intervals."""
-    _say("c0 settle 4096")
+@pytest.mark.parametrize("search", [0, 5])
+def test_c0_settle_parity(tmp_path, search):
+    """4,096-sphere x/y-periodic settling case, 2,000 steps, checked at
+    intervals; with the automatic kernel choice and with neighbour lists
+    forced (the falling spheres trigger list rebuilds)."""
+    _say(f"c0 settle 4096 search={search}")
     sb = configs.lattice_settle()
-    errs = run_pair(sb, tmp_path, [1, 99, 400, 1500])
+    errs = run_pair(sb, tmp_path, [1, 99, 400, 1500], setup=lambda d: d.set_search(search))
+    assert not errs, errs
+
+
+@pytest.mark.parametrize("search", [2, 4, 5])
+def test_position_update_midway(tmp_path, search):
+    """sph_set of positions between steps (cell ids and neighbour lists must
+    follow): a sphere is moved into contact with another one."""
+    from oracle_lib import Oracle
+    from parity import all_fields, compare_all
+    from sphere_amd.dem import DEM
+
+    _say(f"position update search={search}")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=1,
+                                gamma_n=0.0, gamma_t=10.0, sid=f"setx-{search}")
+    orc = Oracle.from_spherebin(sb, tmp_path)
+    dem = DEM(sb, verbosity=0)
+    dem.set_search(search)
+    dem.step(40)
+    assert orc.step(40) == 0
+    x = dem.get("x")
+    assert np.array_equal(x, orc.get("x"))
+    # sphere 7 slides along x onto its neighbour 8, overlapping it slightly
+    x[7, :3] = x[8, :3] - np.array([0.999 * (x[7, 3] + x[8, 3]), 0.0, 0.0])
+    dem.set("x", x)
+    orc.set("x", x)
+    dem.step(60)
+    assert orc.step(60) == 0
+    errs = compare_all(dem, orc, all_fields(orc))
+    dem.close()
+    orc.close()
     assert not errs, errs
 
 
@@ -109,6 +142,7 @@ SEARCH_VARIANTS = [
     ("rows", lambda d: d.set_search(2)),
     ("wave", lambda d: d.set_search(3)),
     ("half", lambda d: d.set_search(4)),
+    ("lists", lambda d: d.set_search(5)),
     ("tiles-auto", lambda d: d.set_search(1)),
     ("tiles-1x1", lambda d: (d.set_search(1), d.set_tile(1, 1))),
     ("tiles-wide", lambda d: (d.set_search(1), d.set_tile(6, 3))),
@@ -119,7 +153,7 @@ COARSE_GRIDS = [((2, 2, 2), 0), ((1, 3, 2), 0), ((2, 1, 1), 0), ((3, 3, 2), 1), 
                 ((3, 1, 2), 2), ((3, 2, 2), 2)]
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("num,periodic", COARSE_GRIDS, ids=[f"{g[0]}-p{g[1]}" for g in COARSE_GRIDS])
 def test_coarse_grid_search_parity(tmp_path, mode, num, periodic):
     """Cells far larger than the spheres: 10-40 members per cell, more than
