@@ -357,6 +357,12 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
         // 1. leapfrog backward
         const V3<double> Lx = xyz(a.next.x[o]), Lv0 = xyz(a.next.v0[o]), Lv = xyz(a.next.v[o]);
         const V3<double> Lw0 = xyz(a.next.w0[o]), Lw = xyz(a.next.w[o]);
+        // this particle's entries of next are consumed: clear them, so next
+        // is zero when it becomes cur (accumulated with atomics) a step earlier
+        const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+        a.next.x[o] = z4;
+        a.next.v[o] = z4;
+        a.next.w[o] = z4;
         const V3<double> Lv0t = Lv0 + Lx * dt + Lv;
         const V3<double> La = Lv0t * dt + Lv * (0.5 * dt);
         const V3<double> Lw0t = Lw0 + Lw;
@@ -498,25 +504,48 @@ __global__ __launch_bounds__(256) void k_adj_objective(DemDev d, const double4* 
     block_add(v, J, red);
 }
 
-// The step's sorted input state as the force kernel sees it (records at
-// perm[q], positions wrapped as the step wraps them), for the tape.
-__global__ __launch_bounds__(256) void k_adj_tape(DemDev d, double4* tx, double4* tv, double4* tw, uint32_t* tid) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= d.np) return;
-    const uint32_t p = d.perm[q];
-    double4 x = d.ro.x[p];
-    if (d.periodic == 1 || d.periodic == 2) {
-        if (x.x < d.origo[0]) x.x += d.L[0];
-        if (x.x > d.L[0]) x.x -= d.L[0];
+// One taped step: the sorted input state as the force kernel sees it
+// (records at perm[q], positions wrapped as the step wraps them), the
+// inverse permutation, slot masks, slot words, incoming springs and walls.
+struct TapeOut {
+    double4 *x, *vel, *angvel, *wnx, *wmvfd;
+    uint32_t *id, *pos_of, *tmask, *tfresh, *cslot;
+    double* cdt;
+    float* cdt32;
+};
+
+__global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (q < d.np) {
+        const uint32_t p = d.perm[q];
+        double4 x = d.ro.x[p];
+        if (d.periodic == 1 || d.periodic == 2) {
+            if (x.x < d.origo[0]) x.x += d.L[0];
+            if (x.x > d.L[0]) x.x -= d.L[0];
+        }
+        if (d.periodic == 1) {
+            if (x.y < d.origo[1]) x.y += d.L[1];
+            if (x.y > d.L[1]) x.y -= d.L[1];
+        }
+        t.x[q] = x;
+        t.vel[q] = d.ro.vel[p];
+        t.angvel[q] = d.ro.angvel[p];
+        t.id[q] = d.sid[q];
+        t.pos_of[q] = d.pos_of[q];
+        t.tmask[q] = d.tmask[q];
+        t.tfresh[q] = d.tfresh[q];
     }
-    if (d.periodic == 1) {
-        if (x.y < d.origo[1]) x.y += d.L[1];
-        if (x.y > d.L[1]) x.y -= d.L[1];
+    if (q < 6) {
+        t.wnx[q] = d.wnx[q];
+        t.wmvfd[q] = d.wmvfd[q];
     }
-    tx[q] = x;
-    tv[q] = d.ro.vel[p];
-    tw[q] = d.ro.angvel[p];
-    tid[q] = d.sid[q];
+    const uint64_t ns = (uint64_t)d.np * NC;
+    for (uint64_t i = q; i < ns; i += stride) t.cslot[i] = d.cslot[i];
+    if (t.cdt32)
+        for (uint64_t i = q; i < 3 * ns; i += stride) t.cdt32[i] = d.cdt32[i];
+    else
+        for (uint64_t i = q; i < 3 * ns; i += stride) t.cdt[i] = d.cdt[i];
 }
 
 inline uint32_t nb(uint64_t n) { return (uint32_t)((n + 255) / 256); }
@@ -697,6 +726,11 @@ int sph_adjoint_forward(sph_t* h, uint64_t nsteps, uint32_t seg_len, const doubl
         h->err = "sph_adjoint_forward: upload failed";
         return SPH_E_HIP;
     }
+    // pick the contact-search kernel on a throw-away copy of the state
+    if (h->search_auto && !h->tuned) {
+        if ((rc = sph::save_state(h, a->end)) || (rc = sph::tune_search(h)) || (rc = sph::restore_state(h, a->end)))
+            return rc;
+    }
     const uint64_t nseg = (nsteps + seg_len - 1) / seg_len;
     a->cps.resize(nseg);
     sph::StepHooks hooks;
@@ -761,6 +795,7 @@ int sph_adjoint_backward(sph_t* h, double grad_out[4]) {
         zero((double*)l.w0, 4 * n); zero((double*)l.w, 4 * n); zero(l.dt, n * nc * 3); zero(l.wall, 18);
     };
     zero_lam(a->lam[0]);
+    zero_lam(a->lam[1]);    // (each adjoint step clears the x, v, w of lambda it consumed)
     zero(a->scal + 1, 10);  // theta, lfw
     int nxt = 0;            // lam[nxt] holds lambda of the step after the one processed
     std::vector<int> wmode(6, 0);
@@ -776,16 +811,15 @@ int sph_adjoint_backward(sph_t* h, double grad_out[4]) {
         for (uint64_t j = 0; j < m; ++j) {
             sph::StepHooks hooks;
             hooks.before_force = [a, j, n](const sph::DemDev& d, hipStream_t s) {
-                const size_t o1 = j * n, on = j * n * nc;
-                sph::k_adj_tape<<<sph::nb(n), 256, 0, s>>>(d, a->tx + o1, a->tv + o1, a->tw + o1, a->tid + o1);
-                (void)hipMemcpyAsync(a->tpos + o1, d.pos_of, n * 4, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->tmask + o1, d.tmask, n * 4, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->tfresh + o1, d.tfresh, n * 4, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->tslot + on, d.cslot, n * nc * 4, hipMemcpyDeviceToDevice, s);
-                if (a->tcdt32) (void)hipMemcpyAsync(a->tcdt32 + 3 * on, d.cdt32, n * nc * 12, hipMemcpyDeviceToDevice, s);
-                else (void)hipMemcpyAsync(a->tcdt + 3 * on, d.cdt, n * nc * 24, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->twn + 6 * j, d.wnx, 6 * 32, hipMemcpyDeviceToDevice, s);
-                (void)hipMemcpyAsync(a->twm + 6 * j, d.wmvfd, 6 * 32, hipMemcpyDeviceToDevice, s);
+                const size_t o1 = j * n, on = j * n * sph::NC;
+                sph::TapeOut t;
+                t.x = a->tx + o1; t.vel = a->tv + o1; t.angvel = a->tw + o1;
+                t.wnx = a->twn + 6 * j; t.wmvfd = a->twm + 6 * j;
+                t.id = a->tid + o1; t.pos_of = a->tpos + o1; t.tmask = a->tmask + o1; t.tfresh = a->tfresh + o1;
+                t.cslot = a->tslot + on;
+                t.cdt = a->tcdt ? a->tcdt + 3 * on : nullptr;
+                t.cdt32 = a->tcdt32 ? a->tcdt32 + 3 * on : nullptr;
+                sph::k_adj_tape<<<sph::nb(n), 256, 0, s>>>(d, t);
             };
             sph::run_one(h, &hooks);
         }
@@ -808,8 +842,6 @@ int sph_adjoint_backward(sph_t* h, double grad_out[4]) {
             ad.theta = a->scal + 1;
             ad.wmode = d_wmode;
             for (uint32_t w = 0; w < 6; ++w) ad.wmass[w] = w < h->hs.nw ? h->hs.wmvfd[4 * w] : 1.0;
-            sph::Lam& cu = a->lam[nxt ^ 1];
-            zero((double*)cu.x, 4 * n); zero((double*)cu.v, 4 * n); zero((double*)cu.w, 4 * n);
             sph::k_adj_walls<<<1, 64, 0, st>>>(h->d, ad, a->scal + 5);
             if (h->precision == SPH_PREC_MIXED) sph::k_adj_step<float><<<sph::nb(n), 256, 0, st>>>(h->d, ad);
             else sph::k_adj_step<double><<<sph::nb(n), 256, 0, st>>>(h->d, ad);

@@ -594,7 +594,9 @@ int make_events(sph_dem* h) {
 // are ordinary steps of the run (all kernels give the same bits).
 constexpr uint64_t TUNE_STEPS = 12;
 
-int tune_search(sph_dem* h) {
+}  // namespace
+
+int sph::tune_search(sph_dem* h) {
     int rc;
     if ((rc = make_events(h))) return rc;
     const int modes[3] = {2, 4, 5};
@@ -627,6 +629,8 @@ int tune_search(sph_dem* h) {
     return check_flags(h);
 }
 
+namespace {
+
 int run_steps(sph_dem* h, uint64_t n) {
     if (n == 0) return SPH_OK;
     int rc;
@@ -656,7 +660,7 @@ int run_steps(sph_dem* h, uint64_t n) {
         return check_flags(h);
     }
     if (h->search_auto && n >= 2 * TUNE_STEPS && (!h->tuned || h->steps_since_tune >= 4096)) {
-        if ((rc = tune_search(h))) return rc;
+        if ((rc = sph::tune_search(h))) return rc;
         n -= TUNE_STEPS;
     }
     h->steps_since_tune += n;

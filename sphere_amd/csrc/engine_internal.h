@@ -62,6 +62,10 @@ struct StepHooks {
 // update on the side stream first, `tail` ends with this step's.
 void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail, const StepHooks* hooks = nullptr);
 
+// Runs a few ordinary steps with each contact-search kernel and keeps the
+// fastest (search mode 0).
+int tune_search(sph_dem* h);
+
 }  // namespace sph
 
 #endif
