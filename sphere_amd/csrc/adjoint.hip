@@ -572,10 +572,9 @@ std::vector<Region> state_regions(sph_dem* h) {
         r.push_back({p->x, n * 32});
         r.push_back({p->vel, n * 32});
         r.push_back({p->angvel, n * 32});
-        r.push_back({p->vel0, n * 32});
-        r.push_back({p->angvel0, n * 32});
-        r.push_back({p->angpos, n * 32});
-        r.push_back({p->acc, n * 32});
+        r.push_back({p->pr0, n * 32});
+        r.push_back({p->pr1, n * 32});
+        r.push_back({p->pr2, n * 32});
         r.push_back({p->id, n * 4});
         r.push_back({p->cmask, n * 4});
     }

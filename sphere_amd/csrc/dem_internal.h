@@ -53,11 +53,11 @@ enum : uint32_t {
 struct PState {
     double4 *x;       // x, y, z, r
     double4 *vel;     // vx, vy, vz, fixvel
-    double4 *angvel;  // wx, wy, wz, 0
-    double4 *vel0;    // half-step velocity
-    double4 *angvel0;
-    double4 *angpos;
-    double4 *acc;     // xysum.x, xysum.y, es, ev
+    double4 *angvel;  // wx, wy, wz, ev
+    // records only the particle reads, 11 doubles packed without padding:
+    double4 *pr0;     // vel0.x, vel0.y, vel0.z, angvel0.x (half-step velocities)
+    double4 *pr1;     // angvel0.y, angvel0.z, angpos.x, angpos.y
+    double4 *pr2;     // angpos.z, xysum.x, xysum.y, es
     uint32_t *id;     // original index
     uint32_t *cmask;  // occupied contact slots
     uint32_t *lidx;   // search mode 5: the particle's neighbour list (its sorted position at the build)
@@ -65,7 +65,7 @@ struct PState {
 
 // The records of one buffer set, as a step reads (ro) or writes (wo) them.
 struct OwnSet {
-    double4 *x, *vel, *angvel, *vel0, *angvel0, *angpos, *acc;
+    double4 *x, *vel, *angvel, *pr0, *pr1, *pr2;
     uint32_t *id, *cmask, *lidx;
 };
 

@@ -65,7 +65,7 @@ void destroy_graph(sph_dem* h) {
 }
 
 sph::OwnSet own_set(const sph::PState& p) {
-    return {p.x, p.vel, p.angvel, p.vel0, p.angvel0, p.angpos, p.acc, p.id, p.cmask, p.lidx};
+    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx};
 }
 
 // the buffer set holding the current private records
@@ -208,7 +208,7 @@ int validate_for_device(sph_dem* h) {
 // Host-side view of the device state in original particle order.
 struct Snapshot {
     std::vector<uint32_t> id;       // storage position -> original index
-    std::vector<double> x, vel, angvel, vel0, angvel0, angpos, acc;  // 4 per particle, original order
+    std::vector<double> x, vel, angvel, pr0, pr1, pr2;  // 4 per particle, original order
 };
 
 int upload_all(sph_dem* h) {
@@ -217,18 +217,27 @@ int upload_all(sph_dem* h) {
     const size_t n = s.np;
     // storage starts in original order: b holds the state, id and pos_of are the identity
     h->own_cur = 0;
-    std::vector<double> v4(4 * n), w4(4 * n), p4(4 * n), acc(4 * n), f4(4 * n), t4(4 * n);
+    // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel; packed records
+    // pr0 = (vel0, angvel0.x), pr1 = (angvel0.yz, angpos.xy),
+    // pr2 = (angpos.z, xysum, es), angvel.w = ev
+    std::vector<double> w4(4 * n), r0(4 * n), r1(4 * n), r2(4 * n), f4(4 * n), t4(4 * n);
     for (size_t i = 0; i < n; ++i) {
         for (int a = 0; a < 3; ++a) {
             w4[4 * i + a] = s.angvel[3 * i + a];
-            p4[4 * i + a] = s.angpos[3 * i + a];
             f4[4 * i + a] = s.force[3 * i + a];
             t4[4 * i + a] = s.torque[3 * i + a];
+            r0[4 * i + a] = s.vel[4 * i + a];
         }
-        acc[4 * i + 0] = s.xysum[2 * i + 0];
-        acc[4 * i + 1] = s.xysum[2 * i + 1];
-        acc[4 * i + 2] = s.es[i];
-        acc[4 * i + 3] = s.ev[i];
+        w4[4 * i + 3] = s.ev[i];
+        r0[4 * i + 3] = s.angvel[3 * i + 0];
+        r1[4 * i + 0] = s.angvel[3 * i + 1];
+        r1[4 * i + 1] = s.angvel[3 * i + 2];
+        r1[4 * i + 2] = s.angpos[3 * i + 0];
+        r1[4 * i + 3] = s.angpos[3 * i + 1];
+        r2[4 * i + 0] = s.angpos[3 * i + 2];
+        r2[4 * i + 1] = s.xysum[2 * i + 0];
+        r2[4 * i + 2] = s.xysum[2 * i + 1];
+        r2[4 * i + 3] = s.es[i];
     }
     std::vector<uint32_t> iota(n);
     for (size_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
@@ -240,12 +249,10 @@ int upload_all(sph_dem* h) {
     }
     HIP_TRY(hipMemcpy(own(h).x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
-    // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel
-    HIP_TRY(hipMemcpy(own(h).vel0, s.vel.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).angvel, w4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(own(h).angvel0, w4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(own(h).angpos, p4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(own(h).acc, acc.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).pr0, r0.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).pr1, r1.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(own(h).pr2, r2.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.b.id, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.a.id, iota.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.sid, iota.data(), n * 4, hipMemcpyHostToDevice));
@@ -346,8 +353,8 @@ int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) {
 }
 
 int alloc_pstate(sph_dem* h, sph::PState& p, size_t n) {
-    ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.vel0, n); ALLOC(p.angvel0, n);
-    ALLOC(p.angpos, n); ALLOC(p.acc, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n);
+    ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.pr0, n); ALLOC(p.pr1, n);
+    ALLOC(p.pr2, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n);
     return SPH_OK;
 }
 
@@ -507,16 +514,19 @@ int download_all(sph_dem* h) {
     };
     if ((rc = fetch4(h, d.force, id, t))) return rc;
     unpad(s.force);
-    if ((rc = fetch4(h, own(h).angpos, id, t))) return rc;
-    unpad(s.angpos);
     if ((rc = fetch4(h, own(h).angvel, id, t))) return rc;
     unpad(s.angvel);
+    for (size_t i = 0; i < n; ++i) s.ev[i] = t[4 * i + 3];
     if ((rc = fetch4(h, d.torque, id, t))) return rc;
     unpad(s.torque);
-    if ((rc = fetch4(h, own(h).acc, id, t))) return rc;
+    std::vector<double> t1, t2;
+    if ((rc = fetch4(h, own(h).pr1, id, t1)) || (rc = fetch4(h, own(h).pr2, id, t2))) return rc;
     for (size_t i = 0; i < n; ++i) {
-        s.xysum[2 * i] = t[4 * i]; s.xysum[2 * i + 1] = t[4 * i + 1];
-        s.es[i] = t[4 * i + 2]; s.ev[i] = t[4 * i + 3];
+        s.angpos[3 * i + 0] = t1[4 * i + 2];
+        s.angpos[3 * i + 1] = t1[4 * i + 3];
+        s.angpos[3 * i + 2] = t2[4 * i + 0];
+        s.xysum[2 * i] = t2[4 * i + 1]; s.xysum[2 * i + 1] = t2[4 * i + 2];
+        s.es[i] = t2[4 * i + 3];
     }
     if ((rc = fetch4(h, d.rates, id, t))) return rc;
     for (size_t i = 0; i < n; ++i) { s.es_dot[i] = t[4 * i]; s.ev_dot[i] = t[4 * i + 1]; s.p[i] = t[4 * i + 2]; }
@@ -691,6 +701,57 @@ int run_steps(sph_dem* h, uint64_t n) {
 
 bool field_is_known(int f) { return f >= 0 && f < SPH_NFIELDS; }
 
+// A field component kept as component c of a per-particle double4 array.
+struct Comp {
+    double4* arr;
+    int c;
+};
+
+// Fields packed into the private records and the spare lane of angvel.
+std::vector<Comp> packed_comps(const sph::PState& p, int f) {
+    switch (f) {
+        case SPH_VEL0: return {{p.pr0, 0}, {p.pr0, 1}, {p.pr0, 2}};
+        case SPH_ANGVEL0: return {{p.pr0, 3}, {p.pr1, 0}, {p.pr1, 1}};
+        case SPH_ANGPOS: return {{p.pr1, 2}, {p.pr1, 3}, {p.pr2, 0}};
+        case SPH_XYSUM: return {{p.pr2, 1}, {p.pr2, 2}};
+        case SPH_ES: return {{p.pr2, 3}};
+        case SPH_EV: return {{p.angvel, 3}};
+        default: return {};
+    }
+}
+
+// storage-order arrays -> original-order values, comps.size() per particle
+int get_comps(sph_dem* h, const std::vector<Comp>& comps, double* out) {
+    const size_t n = h->hs.np, nc = comps.size();
+    std::vector<uint32_t> id;
+    int rc;
+    if ((rc = fetch_ids(h, id))) return rc;
+    std::vector<double> t(4 * n);
+    for (size_t a = 0; a < nc; ++a) {
+        if (a == 0 || comps[a].arr != comps[a - 1].arr)
+            HIP_TRY(hipMemcpy(t.data(), comps[a].arr, n * 32, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q) out[nc * id[q] + a] = t[4 * q + comps[a].c];
+    }
+    return SPH_OK;
+}
+
+// original-order values -> the given components, the rest of each record kept
+int set_comps(sph_dem* h, const std::vector<Comp>& comps, const double* in) {
+    const size_t n = h->hs.np, nc = comps.size();
+    std::vector<uint32_t> id;
+    int rc;
+    if ((rc = fetch_ids(h, id))) return rc;
+    std::vector<double> t(4 * n);
+    for (size_t a = 0; a < nc; ++a) {
+        const bool first = a == 0 || comps[a].arr != comps[a - 1].arr;
+        const bool last = a + 1 == nc || comps[a + 1].arr != comps[a].arr;
+        if (first) HIP_TRY(hipMemcpy(t.data(), comps[a].arr, n * 32, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q) t[4 * q + comps[a].c] = in[nc * id[q] + a];
+        if (last) HIP_TRY(hipMemcpy(comps[a].arr, t.data(), n * 32, hipMemcpyHostToDevice));
+    }
+    return SPH_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C-ABI
@@ -864,12 +925,8 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_ANGVEL: return gather(own(h).angvel, 0, 3);
         case SPH_FORCE: return gather(d.force, 0, 3);
         case SPH_TORQUE: return gather(d.torque, 0, 3);
-        case SPH_ANGPOS: return gather(own(h).angpos, 0, 3);
-        case SPH_VEL0: return gather(own(h).vel0, 0, 3);
-        case SPH_ANGVEL0: return gather(own(h).angvel0, 0, 3);
-        case SPH_XYSUM: return gather(own(h).acc, 0, 2);
-        case SPH_ES: return gather(own(h).acc, 2, 1);
-        case SPH_EV: return gather(own(h).acc, 3, 1);
+        case SPH_ANGPOS: case SPH_VEL0: case SPH_ANGVEL0: case SPH_XYSUM: case SPH_ES: case SPH_EV:
+            return get_comps(h, packed_comps(own(h), f), dd);
         case SPH_ES_DOT: return gather(d.rates, 0, 1);
         case SPH_EV_DOT: return gather(d.rates, 1, 1);
         case SPH_P: return gather(d.rates, 2, 1);
@@ -990,12 +1047,8 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
         case SPH_ANGVEL: return scatter(own(h).angvel, 0, 3);
         case SPH_FORCE: return scatter(d.force, 0, 3);
         case SPH_TORQUE: return scatter(d.torque, 0, 3);
-        case SPH_ANGPOS: return scatter(own(h).angpos, 0, 3);
-        case SPH_VEL0: return scatter(own(h).vel0, 0, 3);
-        case SPH_ANGVEL0: return scatter(own(h).angvel0, 0, 3);
-        case SPH_XYSUM: return scatter(own(h).acc, 0, 2);
-        case SPH_ES: return scatter(own(h).acc, 2, 1);
-        case SPH_EV: return scatter(own(h).acc, 3, 1);
+        case SPH_ANGPOS: case SPH_VEL0: case SPH_ANGVEL0: case SPH_XYSUM: case SPH_ES: case SPH_EV:
+            return set_comps(h, packed_comps(own(h), f), sd);
         case SPH_ES_DOT: return scatter(d.rates, 0, 1);
         case SPH_EV_DOT: return scatter(d.rates, 1, 1);
         case SPH_P: return scatter(d.rates, 2, 1);

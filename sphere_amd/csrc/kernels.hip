@@ -1378,8 +1378,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     const uint32_t* slot = d.cslot + (size_t)o * NC;
 
     // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
-    const double4 va4 = d.ro.vel[prev];
-    const V3<double> va = xyz(va4), wa = xyz(d.ro.angvel[prev]);
+    const double4 va4 = d.ro.vel[prev], wa4 = d.ro.angvel[prev];
+    const V3<double> va = xyz(va4), wa = xyz(wa4);
     Acc acc;
     acc.F = mk(0.0, 0.0, 0.0);
     acc.T = mk(0.0, 0.0, 0.0);
@@ -1435,7 +1435,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
 
     // ---- per-particle outputs (src/contactsearch.cuh:669-687)
     const double dt = d.dt;
-    double4 ea = d.ro.acc[prev];  // xysum.x, xysum.y, es, ev
+    const double4 q0 = d.ro.pr0[prev], q1 = d.ro.pr1[prev], q2 = d.ro.pr2[prev];
+    double4 ea = make_double4(q2.y, q2.z, q2.w, wa4.w);  // xysum.x, xysum.y, es, ev
     ea.z += acc.es_dot * dt;
     ea.w += acc.ev_dot * dt;
     d.rates[s] = make_double4(acc.es_dot, acc.ev_dot, acc.p, 0.0);
@@ -1475,7 +1476,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     }
     double px = xa.x, py = xa.y, pz = xa.z;
     (void)wrap_xy(d, px, py);
-    double4 v0 = d.ro.vel0[prev], w0 = d.ro.angvel0[prev], ap = d.ro.angpos[prev];
+    double4 v0 = make_double4(q0.x, q0.y, q0.z, 0.0), w0 = make_double4(q0.w, q1.x, q1.y, 0.0);
+    double4 ap = make_double4(q1.z, q1.w, q2.x, 0.0);
     v0.x += ax * dt; v0.y += ay * dt; v0.z += az * dt;
     w0.x += bx * dt; w0.y += by * dt; w0.z += bz * dt;
     px += v0.x * dt; py += v0.y * dt; pz += v0.z * dt;
@@ -1487,12 +1489,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     sy += vy * dt;
     ea.x += sx;
     ea.y += sy;
-    d.wo.acc[s] = ea;
-    d.wo.angvel[s] = make_double4(wx, wy, wz, 0.0);
-    d.wo.angvel0[s] = w0;
+    d.wo.pr0[s] = make_double4(v0.x, v0.y, v0.z, w0.x);
+    d.wo.pr1[s] = make_double4(w0.y, w0.z, ap.x, ap.y);
+    d.wo.pr2[s] = make_double4(ap.z, ea.x, ea.y, ea.z);
+    d.wo.angvel[s] = make_double4(wx, wy, wz, ea.w);
     d.wo.vel[s] = make_double4(vx, vy, vz, va4.w);
-    d.wo.vel0[s] = v0;
-    d.wo.angpos[s] = ap;
     d.wo.x[s] = make_double4(px, py, pz, radius);
     d.wo.id[s] = o;
     d.wo.cmask[s] = mask;
