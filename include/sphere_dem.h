@@ -177,7 +177,8 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y);
  * in LDS, 2 = one thread per sorted particle reading through L1/L2, 3 = one
  * wavefront per cell, one lane per candidate, 4 = half shell (each pair
  * tested once, hits replayed in the reference's order), 5 = neighbour lists
- * (pairs within r_a + r_b + skin, rebuilt when a sphere has moved 0.45 skin;
+ * (pairs within r_a + r_b + skin, rebuilt when a sphere has moved 0.45 skin
+ * or half of cell - 2 r_max, whichever is less;
  * overlaps replayed in the reference's order), 0 = automatic (the default):
  * every 4096 steps sph_step times a few steps with each of 2, 4 and 5 and
  * keeps the fastest.  All kernels give bit-identical results. */

@@ -85,10 +85,17 @@ void update_search_eps(sph_dem* h) {
     h->d.search_eps = (float)(16.0 * ulp + 2e-6 * rmax);
     // neighbour-list reach beyond r_a + r_b: a tenth of the smallest cell
     // (at least 2 r_max / 10), so lists hold the touching and nearly touching
-    // spheres and a rebuild is due after 0.45 of that displacement
+    // spheres.  A rebuild is due once a sphere has moved 0.45 skin, or half
+    // the room the cell leaves beyond the largest pair, less the prefilter
+    // margin for FP32 rounding: two spheres more than a cell apart at the
+    // build (outside its 27-cell scan) then cannot meet before the next one.
+    // A cell no wider than the largest pair gives 0: rebuild on any motion.
     double cmin = s.L[0] / s.num[0];
     for (int a = 1; a < 3; ++a) cmin = std::min(cmin, s.L[a] / s.num[a]);
     h->d.list_skin = (float)(0.1 * cmin);
+    const double lim = std::min(0.45 * (double)h->d.list_skin,
+                                0.5 * (cmin - 2.0 * rmax) - (double)h->d.search_eps);
+    h->d.list_lim = lim > 0.0 ? (float)lim : 0.0f;
     h->d.Lf[0] = (float)s.L[0];
     h->d.Lf[1] = (float)s.L[1];
 }

@@ -1034,9 +1034,13 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
 // Search mode 5.  k_topo_rows<true> rebuilds, per particle, the list of every
 // sphere within r_a + r_b + skin (original indices) and keeps the position of
 // the build; the force kernel requests a rebuild once any sphere has moved
-// more than 0.45 skin since.  Until then no pair outside the lists can
-// overlap (both spheres would have had to close a gap of more than skin), so
-// testing the list pairs finds exactly the overlaps the full search finds.
+// more than list_lim since.  list_lim is at most 0.45 skin, so no pair
+// outside the reach can overlap before the rebuild (both spheres would have
+// had to close a gap of more than skin), and at most half the room a cell
+// leaves beyond the largest pair (cell - 2 r_max), so a pair that the 27-cell
+// scan of the build did not see (more than a cell apart) cannot overlap
+// either.  Testing the list pairs then finds exactly the overlaps the full
+// search finds.
 // The overlaps are keyed by the reference's visiting order -- the offset of
 // the partner's cell (periodic wrap included, as findDistMod resolves it,
 // src/contactsearch.cuh:12-74) and its sorted position -- and replayed
@@ -1504,7 +1508,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     if (cerr) atomicOr(d.flags_next, cerr);
 
     // neighbour lists (search mode 5): the list index travels with the
-    // particle; request a rebuild once it moved 0.45 skin since the build
+    // particle; request a rebuild once it moved list_lim since the build
     if (d.search_mode == 5) {
         const uint32_t li = d.nstate[0] ? s : d.slidx[s];
         d.wo.lidx[s] = li;
@@ -1513,8 +1517,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         const float ddz = (float)pz - xr.z;
         if (d.periodic == 1 || d.periodic == 2) ddx -= d.Lf[0] * rintf(ddx / d.Lf[0]);
         if (d.periodic == 1) ddy -= d.Lf[1] * rintf(ddy / d.Lf[1]);
-        const float lim = 0.45f * d.list_skin;
-        if (ddx * ddx + ddy * ddy + ddz * ddz > lim * lim) atomicOr(&d.nstate[1], 1u);
+        if (ddx * ddx + ddy * ddy + ddz * ddz > d.list_lim * d.list_lim) atomicOr(&d.nstate[1], 1u);
     }
 }
 

@@ -51,6 +51,27 @@ def test_c0_settle_parity(tmp_path, search):
     assert not errs, errs
 
 
+def test_lists_cover_pairs_beyond_the_build_scan(tmp_path):
+    """Neighbour lists when a cell is barely wider than the largest pair: two
+    spheres two cells apart at the list build (outside its 27-cell scan) but
+    within r_a + r_b + 0.9 skin close in on each other; the lists must be
+    rebuilt before they can overlap."""
+    _say("lists: pair two cells apart at the build")
+    sb = configs.two_spheres(k=1e7, periodic=0, sid="list-gap")
+    cell = sb.L[0] / sb.num[0]
+    # a at the top of its cell, b 1.08 (r_a + r_b = 1, skin = cell / 10) away,
+    # in the cell after next; they approach at 10 m/s each
+    xa = (np.floor(10.0 / cell) + 1.0) * cell - 0.02
+    sb.x[0] = (xa, 10.0, 10.0)
+    sb.x[1] = (xa + 1.08, 10.0, 10.0)
+    assert int(sb.x[1, 0] // cell) - int(sb.x[0, 0] // cell) == 2
+    sb.vel[0] = (10.0, 0.0, 0.0)
+    sb.vel[1] = (-10.0, 0.0, 0.0)
+    steps = [1] + [5] * 20
+    errs = run_pair(sb, tmp_path, steps, setup=lambda d: d.set_search(5))
+    assert not errs, errs
+
+
 @pytest.mark.parametrize("search", [2, 4, 5])
 def test_position_update_midway(tmp_path, search):
     """sph_set of positions between steps (cell ids and neighbour lists must
