@@ -12,7 +12,10 @@ resident in HBM.  Prints ONE JSON line on stdout.
 
 Also measured: the mixed-precision mode on the same input, and both modes on
 a dense 1,048,576-sphere FCC crystal (twelve contacts per sphere), with the
-mixed mode's error against the FP64 path per output.  Per-kernel time comes
+mixed mode's error against the FP64 path per output; the adjoint on a
+65,536-sphere shear cell; the 262,144-sphere bonded solid (about 1.5M
+bonds) over 5,000 steps; and the 512-member calibration ensemble stepped in
+one handle against the same members one at a time.  Per-kernel time comes
 from hipEvents recorded on the simulation's own stream around each stage,
 over a separate run of the same steps; the roofline entry is for the stage
 with the largest share of the step.
@@ -42,10 +45,12 @@ STAGE_KERNELS = {
     "bonds": "k_bonds", "cellid": "k_fill_u32/k_count (ids from k_force)", "sort": "k_scan_*/k_scatter/k_cellsort",
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
-PROFILE_KERNEL = {"reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force<double>", "cellid": "k_count"}
+PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>", "force": "k_force<double, false>",
+                  "cellid": "k_count"}
 SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply",
                   5: "k_list_search (k_topo_rows<true> on rebuild steps)"}
-SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false>", 3: "k_topo_wave", 4: "k_topo_half", 5: "k_list_search"}
+SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false, false>", 3: "k_topo_wave", 4: "k_topo_half<false>",
+                  5: "k_list_search<false>"}
 
 
 def log(msg):
@@ -192,11 +197,96 @@ def adjoint_run(tmp, nsteps, seg=100):
             "backward_over_forward": (t2 - t1) / t_plain, "total_over_forward": (t2 - t0) / t_plain}
 
 
+def bonded_run(tmp, steps, warmup, prof_steps):
+    """The 262,144-sphere bonded FCC solid (about 1.5M bonds, strengths 1e5 Pa
+    so that bonds break during the run) pulled apart for `steps` FP64 steps."""
+    from sphere_amd import configs
+    from sphere_amd.dem import DEM
+
+    sb = configs.fcc_bonded(cells=(32, 32, 64), sigma_b=1e5, tau_b=1e5, steps=steps)
+    path = os.path.join(tmp, "bonded.bin")
+    sb.writebin(path)
+    n, nb = sb.np, sb.nb0
+    dem = DEM(path, verbosity=0)
+    dem.step(warmup)
+    t0 = time.perf_counter()
+    dem.step(steps)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    broken = int((dem.get("bonds")[:, 0] >= n).sum())
+    dem.set_profiling(True)
+    dem.step(prof_steps)
+    tim = dem.timing()
+    dem.close()
+    stage_ms = {k: tim[f"{k}_ms"] / max(1, tim["steps"]) for k in STAGES}
+    bonds_bytes = 136 * nb
+    log(f"bonded: {ms:.4f} ms/step, {nb} bonds, {broken} broken after {warmup + steps} steps, stages {stage_ms}")
+    return {"workload": f"{n:,}-sphere jittered FCC solid, {nb:,} bonds (d <= 1.05 (r_i + r_j)), k = 1e7, "
+                        f"sigma_b = tau_b = 1e5 Pa, top and bottom layers pulled apart at 0.01 m/s, FP64",
+            "n": n, "bonds": nb, "steps": steps, "warmup": warmup, "value": n / (ms * 1e-3),
+            "unit": "particle-steps/s", "ms_per_step": ms, "broken_bonds": broken, "stage_ms": stage_ms,
+            "k_bonds": {"algorithmic_bytes": bonds_bytes, "GBs": bonds_bytes / (stage_ms["bonds"] * 1e-3) / 1e9
+                        if stage_ms["bonds"] > 0 else None}}
+
+
+def ensemble_run(members, steps, warmup, sample_every=32, sample_steps=100):
+    """The calibration ensemble in one handle (one launch of each kernel per
+    step for every member, graph-replayed), mixed precision, grid and block
+    sizes from the autotuner; against the same members run one at a time,
+    each in its own handle (a sample of every `sample_every`-th member,
+    extrapolated by member count)."""
+    from sphere_amd.dem import DEM, Ensemble
+
+    n_total = sum(m.np for m in members)
+    t0 = time.perf_counter()
+    ens = Ensemble(members, cell_factor=0.0)
+    t_open = time.perf_counter() - t0
+    ens.set_precision("mixed")
+    ens.step(warmup)  # the first call also tunes the search kernel and the block sizes
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    info = ens.info()
+    ens.set_profiling(True)
+    ens.step(20)
+    tim = ens.timing()
+    ens.close()
+    stage_ms = {k: tim[f"{k}_ms"] / max(1, tim["steps"]) for k in STAGES}
+    log(f"ensemble: {len(members)} members, {n_total} spheres, {ms:.4f} ms/step, cell factor {info.cell_factor}, "
+        f"search {info.search_mode}, blocks force {info.force_block} search {info.search_block}, open {t_open:.1f}s")
+    sample = members[::sample_every]
+    per = []
+    for sb in sample:
+        d = DEM(sb, verbosity=0)
+        d.set_precision("mixed")
+        d.step(warmup)
+        t0 = time.perf_counter()
+        d.step(sample_steps)
+        per.append((time.perf_counter() - t0) * 1e3 / sample_steps)
+        d.close()
+    ms_one = float(np.mean(per)) * len(members)
+    log(f"one at a time: {np.mean(per):.4f} ms/step per member (sample of {len(sample)}), {ms_one:.2f} ms per "
+        f"ensemble step")
+    return {"workload": f"{len(members)} packings, {n_total:,} spheres (counts uniform in [1024, 32768]), k_n uniform "
+                        "in [1e5, 1e7], mu uniform in [0.2, 0.8], C1 radii, settling under a 10 kPa top wall, mixed "
+                        "precision",
+            "members": len(members), "n": n_total, "steps": steps, "warmup": warmup,
+            "value": n_total / (ms * 1e-3), "unit": "particle-steps/s", "ms_per_step": ms, "stage_ms": stage_ms,
+            "autotuned": {"cell_factor": info.cell_factor, "trial_ms_per_step": list(info.tuned_ms),
+                          "search_kernel": SEARCH_KERNELS.get(info.search_mode),
+                          "force_block": info.force_block, "search_block": info.search_block},
+            "one_at_a_time": {"value": n_total / (ms_one * 1e-3), "ms_per_ensemble_step": ms_one,
+                              "ms_per_member_step": float(np.mean(per)), "sample_members": len(sample),
+                              "sample_steps": sample_steps},
+            "speedup_vs_one_at_a_time": ms_one / ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    # (the first sph_step of at least 24 steps, 42 for the ensemble, times the
+    # contact-search kernels; a warm-up of 50 keeps that out of the timed steps)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-steps", type=int, default=4)
@@ -205,6 +295,11 @@ def main():
     ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-bonded", action="store_true")
+    ap.add_argument("--bonded-steps", type=int, default=5000)
+    ap.add_argument("--ensemble-members", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the copy-kernel bandwidth probe (counter calibration)")
     args = ap.parse_args()
@@ -261,7 +356,7 @@ def main():
         "data": "synthetic (seeded lognormal random packing)",
         "config": {"workload": "1,048,576 lognormal spheres, random packing, 10 kPa top wall, 2.07x2.07x4.13 m x/y-periodic box",
                    "n": n, "cells": K, "contacts_per_particle": c / n, "precision": args.precision,
-                   "search_kernel": stage_kernels["search"]},
+                   "search_kernel": stage_kernels["search"], "search_mode": main_run["search_mode"]},
         "per_kernel": per_kernel,
         "roofline": {"bound": "hbm", "kernel": stage_kernels[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -300,6 +395,14 @@ def main():
     if not args.no_adjoint:
         result["adjoint"] = adjoint_run(tmp, args.adjoint_steps)
         result["adjoint_forward_ratio"] = result["adjoint"]["backward_over_forward"]
+    if not args.no_bonded and args.precision == "fp64":
+        result["bonded"] = bonded_run(tmp, args.bonded_steps, args.warmup, prof_steps)
+    if not args.no_ensemble:
+        t0 = time.perf_counter()
+        members = configs.calibration_ensemble(members=args.ensemble_members, steps=args.ensemble_steps)
+        log(f"generated the ensemble in {time.perf_counter() - t0:.1f}s")
+        result["ensemble"] = ensemble_run(members, args.ensemble_steps, max(args.warmup, 50))
+        del members
     if not args.no_cpu:
         rate, secs = cpu_baseline(bin_path, n, args.cpu_steps)
         result["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",

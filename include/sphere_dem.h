@@ -214,6 +214,63 @@ int sph_adjoint_backward(sph_t* h, double grad[4]);
 int sph_pack_rsa(uint64_t n, const double* radii, const double* box, int periodic, uint64_t seed,
                  uint64_t max_tries, double* xyz);
 
+/* ------------------------------------------------------------ ensembles
+ * No counterpart in the reference, which steps one DEM per process: an
+ * ensemble steps many independent systems (each a .bin image with its own
+ * particle count, box, grid, parameters, time step and walls) together, one
+ * launch of each kernel for all of them per step, the step captured in a HIP
+ * graph.  Every member evolves exactly as it would in its own handle on the
+ * same grid (bit for bit in FP64).  Members may not have bonds
+ * (SPH_E_UNSUPPORTED); at most 65535 members. */
+typedef struct sph_ens sph_ens_t;
+
+/* Opens an ensemble from n in-memory .bin images (sph_ens_open: files).
+ * checkValues is applied to every member.  cell_factor < 0 keeps each
+ * image's grid; cell_factor >= 1 regrids every member to cells of
+ * cell_factor x 2 r_max (r_max of the member; an axis keeps at least one
+ * cell, a periodic one at least three); cell_factor = 0 lets the autotuner
+ * choose the factor among 1, 1.25, 1.5 and 2 by timing a few steps of
+ * trial ensembles (discarded) built from the same images. */
+int sph_ens_open_mem(const void* const* images, const size_t* bytes, uint32_t n, double cell_factor,
+                     int verbosity, sph_ens_t** out);
+int sph_ens_open(const char* const* paths, uint32_t n, double cell_factor, int verbosity, sph_ens_t** out);
+void sph_ens_close(sph_ens_t* e);
+const char* sph_ens_last_error(const sph_ens_t* e);
+
+/* nsteps steps of every member (sph_step for each). */
+int sph_ens_step(sph_ens_t* e, uint64_t nsteps);
+
+/* Fields of member k in that member's own numbering and sizes, as sph_get
+ * returns them for a handle opened on the member alone; the sort fields and
+ * SPH_DISTMOD are not available (SPH_E_UNSUPPORTED). */
+size_t sph_ens_field_bytes(const sph_ens_t* e, uint32_t k, int field);
+int sph_ens_get(sph_ens_t* e, uint32_t k, int field, void* dst, size_t bytes);
+/* Sizes of member k (grid after any regridding). */
+int sph_ens_member_info(const sph_ens_t* e, uint32_t k, sph_info_t* out);
+
+typedef struct sph_ens_info {
+    uint32_t members;
+    uint64_t np, ncells, steps_done;
+    double cell_factor;          /* in use; < 0: the images' own grids */
+    double tuned_ms[4];          /* autotuner: ms per step of each trial factor (0: not tried) */
+    int32_t search_mode;         /* contact-search kernel in use (2, 4 or 5) */
+    uint32_t force_block, search_block;  /* threads per block of k_force and of the search kernel */
+    int32_t precision;
+} sph_ens_info_t;
+int sph_ens_info(const sph_ens_t* e, sph_ens_info_t* out);
+
+/* Kernel choices, as for one handle: precision; search mode 0 (automatic: on
+ * the next sph_ens_step the autotuner times the row, half-shell and list
+ * kernels and then 64, 128 and 256 threads per block for the search and
+ * force kernels, on ordinary steps, and keeps the fastest), 2, 4 or 5 (1 and
+ * 3 walk one grid and are refused); graph capture; per-kernel profiling. */
+int sph_ens_set_precision(sph_ens_t* e, int precision);
+int sph_ens_set_search(sph_ens_t* e, int mode);
+int sph_ens_set_blocks(sph_ens_t* e, uint32_t force_block, uint32_t search_block);
+int sph_ens_set_graph(sph_ens_t* e, int on);
+int sph_ens_set_profiling(sph_ens_t* e, int on);
+int sph_ens_get_timing(const sph_ens_t* e, sph_timing_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,6 +120,55 @@ def random_compression(n=1048576, box=(2.07, 2.07, 4.13), periodic=1, stress=10e
     return sb
 
 
+def ensemble_member(n=4096, k_n=1e6, mu=0.5, stress=10e3, gravity=-9.81, steps=1000, packing=0.3,
+                    seed=BASE_SEED + 4, sid="c4-member") -> Spherebin:
+    """One packing of the calibration ensemble: ``n`` lognormal spheres (as
+    :func:`random_compression`) placed at random without overlap in an
+    x/y-periodic box of aspect 1:1:2 sized for the packing fraction, settling
+    under gravity below a top wall held at ``stress`` (10 kPa); contact
+    stiffness ``k_n`` (= k_t) and friction ``mu`` per member, dt = 0.17
+    sqrt(m_min / k_n)."""
+    from .dem import pack_rsa
+
+    rng = np.random.default_rng(seed)
+    sb = Spherebin(n, nw=1, sid=sid)
+    sb.radius = lognormal_radii(rng, n)
+    _contact_params(sb, k_n=k_n, k_t=k_n, mu=mu)
+    solid = float(np.sum(4.0 / 3.0 * math.pi * sb.radius ** 3))
+    side = (solid / packing / 2.0) ** (1.0 / 3.0)
+    r_max = float(np.max(sb.radius))
+    side = max(side, 3.0 * 2.0 * r_max)  # at least three cells across the periodic axes
+    box = np.array([side, side, 2.0 * side])
+    sb.x = pack_rsa(sb.radius, box, 1, seed)
+    L = np.array([box[0], box[1], box[2] * 1.1])
+    _grid(sb, L, 2.0 * r_max)
+    sb.periodic = 1
+    sb.g = np.array([0.0, 0.0, gravity])
+    sb.set_walls(1)
+    sb.wmode = np.array([1], dtype=np.int32)
+    sb.w_x[0] = float(np.max(sb.x[:, 2] + sb.radius))
+    cell = L[0] / sb.num[0]
+    sb.w_m[0] = sb.rho * n * math.pi * (cell / 2.0) ** 3
+    sb.w_devs[0] = float(stress)
+    sb.time_dt = _dt(sb)
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    return sb
+
+
+def calibration_ensemble(members=512, n_lo=1024, n_hi=32768, k_lo=1e5, k_hi=1e7, mu_lo=0.2, mu_hi=0.8,
+                         steps=1000, seed=BASE_SEED + 4):
+    """The calibration ensemble: ``members`` packings with seeded particle
+    counts uniform in [n_lo, n_hi], k_n uniform in [k_lo, k_hi] and mu
+    uniform in [mu_lo, mu_hi] (512 members hold about 8.6M spheres)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_lo, n_hi + 1, size=members)
+    ks = rng.uniform(k_lo, k_hi, size=members)
+    mus = rng.uniform(mu_lo, mu_hi, size=members)
+    return [ensemble_member(n=int(ns[i]), k_n=float(ks[i]), mu=float(mus[i]), steps=steps,
+                            seed=seed + 1000 + i, sid=f"c4-{i:03d}") for i in range(members)]
+
+
 def fcc_bonded(cells=(4, 4, 8), radius=1e-2, jitter_frac=0.01, bond_reach=1.05, k_bond=1e7,
                lambda_bar=0.5, sigma_b=math.inf, tau_b=math.inf, pull_velocity=0.01,
                gravity=0.0, steps=5000, seed=BASE_SEED + 2, sid="c2-bonded") -> Spherebin:

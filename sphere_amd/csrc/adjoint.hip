@@ -102,8 +102,8 @@ template <class T>
 __device__ __forceinline__ void contact_linear_vjp(const DemDev& d, V3<T> va, V3<T> wa, V3<T> vb, V3<T> wb, T ra,
                                                    T rb, V3<T> x, T len, T delta, V3<T> dt0u, V3<T> gF, V3<T> gT,
                                                    V3<T> gD, PairGrad<T>& g) {
-    const T k_n = (T)d.k_n, k_t = (T)d.k_t, gamma_n = (T)d.gamma_n, gamma_t = (T)d.gamma_t;
-    const T mu_s = (T)d.mu_s, mu_d = (T)d.mu_d, kappa = (T)d.kappa, dt = (T)d.dt;
+    const T k_n = (T)d.sys.k_n, k_t = (T)d.sys.k_t, gamma_n = (T)d.sys.gamma_n, gamma_t = (T)d.sys.gamma_t;
+    const T mu_s = (T)d.sys.mu_s, mu_d = (T)d.sys.mu_d, kappa = (T)d.sys.kappa, dt = (T)d.sys.dt;
     const T zero = (T)0;
     // ---- forward recompute
     const V3<T> n = x / len;
@@ -245,7 +245,7 @@ __device__ __forceinline__ void contact_wall_vjp(const DemDev& d, V3<double> lin
     const double vel_n = dot(vel, n);
     const V3<double> vel_t = vel - n * dot(vel, n);
     const double vtl = vlen(vel_t);
-    const double sn = -d.k_n * delta - d.gamma_wn * vel_n;
+    const double sn = -d.sys.k_n * delta - d.sys.gamma_wn * vel_n;
     V3<double> fn = sn * n;
     const bool clamped = dot(fn, n) < 0.0;
     if (clamped) fn = mk(0.0, 0.0, 0.0);
@@ -254,8 +254,8 @@ __device__ __forceinline__ void contact_wall_vjp(const DemDev& d, V3<double> lin
     bool visc_branch = false;
     double lim = 0.0, mu = 0.0;
     if (vtl > 0.0) {
-        const double visc = d.gamma_wt * vtl;
-        mu = vtl > (double)0.001f ? d.mu_wd : d.mu_ws;
+        const double visc = d.sys.gamma_wt * vtl;
+        mu = vtl > (double)0.001f ? d.sys.mu_wd : d.sys.mu_ws;
         lim = mu * fnl;
         visc_branch = visc < lim;
         ft = (-1.0 * (visc_branch ? visc : lim)) * vel_t / vtl;
@@ -267,7 +267,7 @@ __device__ __forceinline__ void contact_wall_vjp(const DemDev& d, V3<double> lin
     double g_fnl = 0.0;
     if (vtl > 0.0) {
         if (visc_branch) {
-            g_vt = g_vt - d.gamma_wt * g_ft;  // ft = -gamma_wt vel_t
+            g_vt = g_vt - d.sys.gamma_wt * g_ft;  // ft = -gamma_wt vel_t
         } else {
             const V3<double> e = vel_t / vtl;
             const double g_lim = -dot(e, g_ft);
@@ -283,8 +283,8 @@ __device__ __forceinline__ void contact_wall_vjp(const DemDev& d, V3<double> lin
         if (fnl > 0.0) g_fn = g_fn + (g_fnl / fnl) * fn;
         const double g_sn = dot(n, g_fn);
         g.gkn = -delta * g_sn;
-        g.gdelta = -d.k_n * g_sn;
-        g_veln = -d.gamma_wn * g_sn;
+        g.gdelta = -d.sys.k_n * g_sn;
+        g_veln = -d.sys.gamma_wn * g_sn;
     }
     const V3<double> g_vel = (g_vt - n * dot(n, g_vt)) + g_veln * n;
     g.gwvel = g_vel.x + g_vel.y + g_vel.z;
@@ -300,7 +300,7 @@ __global__ void k_adj_walls(DemDev d, AdjDev a, double* lfw) {
     if (w >= NWALL) return;
     const double* nx = a.next.wall + 3 * w;  // pos, vel, vel0
     double* cu = a.cur.wall + 3 * w;
-    const double dt = d.dt;
+    const double dt = d.sys.dt;
     const int mode = w < (int)d.nw ? a.wmode[w] : 0;
     if (mode == 0) {  // the wall does not move
         cu[0] = nx[0];
@@ -351,8 +351,8 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
         const double4 xa4 = t.x[s], va4 = t.vel[s], wa4 = t.angvel[s];
         const double ra = xa4.w;
         const bool fix = va4.w > 0.0;
-        const double dt = d.dt;
-        const double m = 4.0 / 3.0 * PI_REF * ra * ra * ra * d.rho;
+        const double dt = d.sys.dt;
+        const double m = 4.0 / 3.0 * PI_REF * ra * ra * ra * d.sys.rho;
         const double inertia = 2.0 / 5.0 * m * ra * ra;
         // 1. leapfrog backward
         const V3<double> Lx = xyz(a.next.x[o]), Lv0 = xyz(a.next.v0[o]), Lv = xyz(a.next.v[o]);
@@ -390,8 +390,8 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
             const uint32_t sb = t.pos_of[bo];
             const double4 xb = t.x[sb];
             const uint32_t code = w >> 28, cx = code & 3u, cy = (code >> 2) & 3u;
-            const double dmx = cx == 1u ? d.L[0] : (cx == 2u ? -d.L[0] : 0.0);
-            const double dmy = cy == 1u ? d.L[1] : (cy == 2u ? -d.L[1] : 0.0);
+            const double dmx = cx == 1u ? d.sys.L[0] : (cx == 2u ? -d.sys.L[0] : 0.0);
+            const double dmy = cy == 1u ? d.sys.L[1] : (cy == 2u ? -d.sys.L[1] : 0.0);
             const V3<double> xab = mk((xa4.x - xb.x) + dmx, (xa4.y - xb.y) + dmy, (xa4.z - xb.z) + 0.0);
             const double len = vlen(xab);
             const double delta_n = len - (ra + xb.w);
@@ -440,14 +440,14 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
             }
         }
         {
-            const double dw = xa4.z - ra - d.origo[2];
+            const double dw = xa4.z - ra - d.sys.origo[2];
             if (dw < 0.0) {
                 contact_wall_vjp(d, va, wa, ra, mk(0.0, 0.0, 1.0), dw, 0.0, LF, LT, 0.0, g);
                 gv = gv + g.gv; gw = gw + g.gw; th[0] += g.gkn;
                 gx.z += g.gdelta;
             }
         }
-        if (d.periodic == 0) {
+        if (d.sys.periodic == 0) {
             const double4 w1 = t.wnx[1], w2 = t.wnx[2];
             double dw = w1.w - (xa4.x + ra);
             if (dw < 0.0) {
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
                 gx.x += g.gdelta; gwp[2] -= g.gdelta; gwv[2] += g.gwvel;
             }
         }
-        if (d.periodic == 0 || d.periodic == 2) {
+        if (d.sys.periodic == 0 || d.sys.periodic == 2) {
             const double4 w3 = t.wnx[3], w4 = t.wnx[4];
             double dw = w3.w - (xa4.y + ra);
             if (dw < 0.0) {
@@ -520,13 +520,13 @@ __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
     if (q < d.np) {
         const uint32_t p = d.perm[q];
         double4 x = d.ro.x[p];
-        if (d.periodic == 1 || d.periodic == 2) {
-            if (x.x < d.origo[0]) x.x += d.L[0];
-            if (x.x > d.L[0]) x.x -= d.L[0];
+        if (d.sys.periodic == 1 || d.sys.periodic == 2) {
+            if (x.x < d.sys.origo[0]) x.x += d.sys.L[0];
+            if (x.x > d.sys.L[0]) x.x -= d.sys.L[0];
         }
-        if (d.periodic == 1) {
-            if (x.y < d.origo[1]) x.y += d.L[1];
-            if (x.y > d.L[1]) x.y -= d.L[1];
+        if (d.sys.periodic == 1) {
+            if (x.y < d.sys.origo[1]) x.y += d.sys.L[1];
+            if (x.y > d.sys.L[1]) x.y -= d.sys.L[1];
         }
         t.x[q] = x;
         t.vel[q] = d.ro.vel[p];

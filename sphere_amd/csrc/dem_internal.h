@@ -69,19 +69,39 @@ struct OwnSet {
     uint32_t *id, *cmask, *lidx;
 };
 
-// Everything a step kernel needs, passed by value as one kernel argument.
-struct DemDev {
-    uint32_t np, nb0, nw, K;
+// Box, cell grid, parameters and walls of one system.  A handle steps either
+// one system (DemDev::sys) or an ensemble of independent systems stored back
+// to back (DemDev::systems, system of index i = sys_of[i]): system k owns the
+// original indices and the sorted positions [pbase, pbase + np) -- the cell
+// ids of different systems never interleave, so the sort keeps every system
+// in its own range -- and the cell ids [cbase, cbase + K).
+struct Sys {
+    double origo[3], L[3], cell_len[3];
     uint32_t num[3];
     int periodic;
-    uint32_t bond_broken;  // sentinel written into bonds[k].x on breakage
-    uint32_t nwall_blocks; // ceil(np/256), partials per wall
-    double origo[3], L[3], cell_len[3];
+    float Lf[2];        // L[0], L[1] in FP32 (periodic images in the prefilter)
+    uint32_t cbase, K;  // first cell id, cells
+    uint32_t pbase, np; // first index, particles
+    uint32_t nw, wbase; // walls; wall w of the system is entry wbase + w of the wall arrays
+    uint32_t bbase, nblocks;  // wall-sum blocks (256 original indices each) of the system
     double dt;
     double g[3];
     double k_n, k_t, gamma_n, gamma_t, mu_s, mu_d;
     double gamma_wn, gamma_wt, mu_ws, mu_wd, rho, kappa;
-    double lambda_bar, sigma_b, tau_b, devs_A, devs_f;
+    double devs_A, devs_f;
+};
+
+// Everything a step kernel needs, passed by value as one kernel argument.
+struct DemDev {
+    uint32_t np, nb0, nw, K;  // totals over the systems (nw: the most walls of any system)
+    uint32_t bond_broken;  // sentinel written into bonds[k].x on breakage
+    uint32_t nwall_blocks; // wall-sum blocks over all systems, partials per wall
+    double lambda_bar, sigma_b, tau_b;
+    Sys sys;               // the system of a one-system handle
+    const Sys* systems;    // ensemble: one entry per system (nullptr for one system)
+    const uint16_t* sys_of;    // ensemble: system of each index / sorted position
+    const uint16_t* sys_of_block;  // ensemble: system of each wall-sum block
+    uint32_t nsys;
 
     PState a, b;        // storage of the two buffer sets
     OwnSet ro, wo;      // this step's input (at perm[s]) and output (at s)
@@ -104,10 +124,10 @@ struct DemDev {
     // contact search tiles: TX x TY cells of one z plane per block
     uint32_t tile_x, tile_y, tiles_x, tiles_y;
     int search_mode;    // 1: LDS tiles, 2: rows per particle, 3: wave per cell, 4: half shell, 5: lists
+    uint32_t tpb_search, tpb_force;  // threads per block of k_topo_rows / k_topo_half and of k_force (<= BLOCK)
     uint32_t* skey;     // cell id per sorted position
     float4* xf;         // FP32 copy of a.x for the search's distance prefilter
     float search_eps;   // prefilter margin (m), above the FP32 rounding of a distance
-    float Lf[2];        // L[0], L[1] in FP32 (periodic images in the prefilter)
     uint32_t* nlist;    // mode 5: np * NLIST partners (their list index), list index = sorted position at the build
     uint32_t* ncnt;     // mode 5: list length per list index
     float4* xref;       // mode 5: position at the last list build, per list index
@@ -121,13 +141,13 @@ struct DemDev {
     uint32_t* tmask;    // slot occupancy after the search, sorted order
     uint32_t* tfresh;   // slots taken from empty by the search, sorted order
 
-    // walls (6 entries; entries >= nw hold the reference's defaults)
+    // walls (6 entries per system; entries >= its nw hold the reference's defaults)
     double4 *wnx, *wmvfd;
     double* wvel0;
     int* wmode;
     double* wall_pp;       // nw * np, original order
     double* wall_partial;  // nw * nwall_blocks
-    double* t_cur;         // device copy of time.current
+    double* t_cur;         // device copy of time.current, one per system
 
     // bonds
     uint2* bonds;

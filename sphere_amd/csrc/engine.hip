@@ -76,11 +76,26 @@ const sph::PState& own(const sph_dem* h) { return h->own_cur ? h->d.a : h->d.b; 
 // by a periodic image errs by at most 3.5 ulp(M) per axis and 6.1 ulp(M) in
 // the distance; 16 ulp(M) plus 1e-6 of the largest radius sum also covers
 // the FP32 rounding of the squares.
-void update_search_eps(sph_dem* h) {
-    const HostState& s = h->hs;
-    double M = 0.0, rmax = 0.0;
-    for (int a = 0; a < 3; ++a) M = std::max(M, std::fabs(s.origo[a]) + 2.0 * std::fabs(s.L[a]));
+// The systems of a handle: its own, or an ensemble's members.
+std::vector<const HostState*> systems_of(const sph_dem* h) {
+    std::vector<const HostState*> v;
+    if (h->members.empty()) v.push_back(&h->hs);
+    for (const HostState& m : h->members) v.push_back(&m);
+    return v;
+}
+
+double max_radius(const HostState& s) {
+    double rmax = 0.0;
     for (uint32_t i = 0; i < s.np; ++i) rmax = std::max(rmax, s.x[4 * (size_t)i + 3]);
+    return rmax;
+}
+
+void update_search_eps(sph_dem* h) {
+    double M = 0.0, rmax = 0.0;
+    for (const HostState* m : systems_of(h)) {
+        for (int a = 0; a < 3; ++a) M = std::max(M, std::fabs(m->origo[a]) + 2.0 * std::fabs(m->L[a]));
+        rmax = std::max(rmax, max_radius(*m));
+    }
     const double ulp = std::ldexp(1.0, std::ilogb(std::max(M, 1e-30)) - 23);
     h->d.search_eps = (float)(16.0 * ulp + 2e-6 * rmax);
     // neighbour-list reach beyond r_a + r_b: a tenth of the smallest cell
@@ -90,36 +105,71 @@ void update_search_eps(sph_dem* h) {
     // margin for FP32 rounding: two spheres more than a cell apart at the
     // build (outside its 27-cell scan) then cannot meet before the next one.
     // A cell no wider than the largest pair gives 0: rebuild on any motion.
-    double cmin = s.L[0] / s.num[0];
-    for (int a = 1; a < 3; ++a) cmin = std::min(cmin, s.L[a] / s.num[a]);
+    // (an ensemble takes the smallest cell and the least room of its members)
+    double cmin = HUGE_VAL, room = HUGE_VAL;
+    for (const HostState* m : systems_of(h)) {
+        double c = m->L[0] / m->num[0];
+        for (int a = 1; a < 3; ++a) c = std::min(c, m->L[a] / m->num[a]);
+        cmin = std::min(cmin, c);
+        room = std::min(room, c - 2.0 * (h->members.empty() ? rmax : max_radius(*m)));
+    }
     h->d.list_skin = (float)(0.1 * cmin);
-    const double lim = std::min(0.45 * (double)h->d.list_skin,
-                                0.5 * (cmin - 2.0 * rmax) - (double)h->d.search_eps);
+    const double lim = std::min(0.45 * (double)h->d.list_skin, 0.5 * room - (double)h->d.search_eps);
     h->d.list_lim = lim > 0.0 ? (float)lim : 0.0f;
-    h->d.Lf[0] = (float)s.L[0];
-    h->d.Lf[1] = (float)s.L[1];
+}
+
+// Box, grid, parameters and walls of system s as the kernels read them; the
+// caller places it (cbase, pbase, wbase, bbase).
+sph::Sys make_sys(const HostState& s) {
+    sph::Sys S{};
+    for (int a = 0; a < 3; ++a) {
+        S.num[a] = s.num[a]; S.origo[a] = s.origo[a]; S.L[a] = s.L[a];
+        S.cell_len[a] = s.L[a] / (double)s.num[a];
+        S.g[a] = s.g[a];
+    }
+    S.periodic = s.periodic;
+    S.Lf[0] = (float)s.L[0];
+    S.Lf[1] = (float)s.L[1];
+    S.K = s.num[0] * s.num[1] * s.num[2];
+    S.np = s.np;
+    S.nw = s.nw;
+    S.nblocks = (s.np + 255) / 256;
+    S.dt = s.dt;
+    S.k_n = s.k_n; S.k_t = s.k_t; S.gamma_n = s.gamma_n; S.gamma_t = s.gamma_t;
+    S.mu_s = s.mu_s; S.mu_d = s.mu_d; S.gamma_wn = s.gamma_wn; S.gamma_wt = s.gamma_wt;
+    S.mu_ws = s.mu_ws; S.mu_wd = s.mu_wd; S.rho = s.rho; S.kappa = s.kappa;
+    S.devs_A = s.devs_A; S.devs_f = s.devs_f;
+    return S;
 }
 
 void fill_params(sph_dem* h) {
     const HostState& s = h->hs;
     DemDev& d = h->d;
-    d.np = s.np; d.nb0 = s.nb0; d.nw = s.nw;
-    for (int a = 0; a < 3; ++a) {
-        d.num[a] = s.num[a]; d.origo[a] = s.origo[a]; d.L[a] = s.L[a];
-        d.cell_len[a] = s.L[a] / (double)s.num[a];
-        d.g[a] = s.g[a];
-    }
-    d.K = s.num[0] * s.num[1] * s.num[2];
-    d.periodic = s.periodic;
+    d.nb0 = s.nb0;
     d.bond_broken = s.nb0 >= s.np ? s.nb0 : s.np;
-    d.nwall_blocks = (s.np + 255) / 256;
-    d.dt = s.dt;
-    d.k_n = s.k_n; d.k_t = s.k_t; d.gamma_n = s.gamma_n; d.gamma_t = s.gamma_t;
-    d.mu_s = s.mu_s; d.mu_d = s.mu_d; d.gamma_wn = s.gamma_wn; d.gamma_wt = s.gamma_wt;
-    d.mu_ws = s.mu_ws; d.mu_wd = s.mu_wd; d.rho = s.rho; d.kappa = s.kappa;
     d.lambda_bar = s.lambda_bar; d.sigma_b = s.sigma_b; d.tau_b = s.tau_b;
-    d.devs_A = s.devs_A; d.devs_f = s.devs_f;
+    if (h->members.empty()) {
+        d.sys = make_sys(s);
+        d.np = s.np; d.nw = s.nw; d.K = d.sys.K; d.nwall_blocks = d.sys.nblocks;
+        d.systems = nullptr; d.sys_of = nullptr; d.sys_of_block = nullptr; d.nsys = 1;
+    } else {
+        // members back to back: indices, cells, wall entries (6 each) and
+        // wall-sum blocks; the device copy of the table is made at upload
+        h->sys_host.clear();
+        uint32_t pb = 0, cb = 0, bb = 0, nw = 0;
+        for (size_t k = 0; k < h->members.size(); ++k) {
+            sph::Sys S = make_sys(h->members[k]);
+            S.pbase = pb; S.cbase = cb; S.wbase = (uint32_t)(6 * k); S.bbase = bb;
+            pb += S.np; cb += S.K; bb += S.nblocks; nw = std::max(nw, S.nw);
+            h->sys_host.push_back(S);
+        }
+        d.sys = h->sys_host[0];
+        d.np = pb; d.nw = nw; d.K = cb; d.nwall_blocks = bb;
+        d.nsys = (uint32_t)h->members.size();
+    }
     d.scan_blocks = (d.K + sph::SCAN_TILE - 1) / sph::SCAN_TILE;
+    if (!d.tpb_force) d.tpb_force = sph::BLOCK;
+    if (!d.tpb_search) d.tpb_search = sph::BLOCK;
 }
 
 // Contact-search tile: tile_x x tile_y cells of one z plane per block.  Aim
@@ -129,6 +179,11 @@ void fill_params(sph_dem* h) {
 void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
     const HostState& s = h->hs;
     DemDev& d = h->d;
+    if (!h->members.empty()) {
+        d.tile_x = d.tile_y = d.tiles_x = d.tiles_y = 1;
+        if (d.search_mode == 0) d.search_mode = 2;
+        return;
+    }
     const uint32_t nx = s.num[0], ny = s.num[1];
     uint32_t bx = 1, by = 1;
     if (want_x && want_y) {
@@ -139,7 +194,7 @@ void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
         for (uint32_t i = 0; i < s.np; ++i) {
             uint32_t c[3];
             for (int a = 0; a < 3; ++a) {
-                double f = std::floor((s.x[4 * (size_t)i + a] - s.origo[a]) / d.cell_len[a]);
+                double f = std::floor((s.x[4 * (size_t)i + a] - s.origo[a]) / d.sys.cell_len[a]);
                 if (!(f >= 0.0)) f = 0.0;
                 if (f > (double)(s.num[a] - 1)) f = (double)(s.num[a] - 1);
                 c[a] = (uint32_t)f;
@@ -174,8 +229,7 @@ void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
     d.tiles_y = (ny + by - 1) / by;
 }
 
-int validate_for_device(sph_dem* h) {
-    const HostState& s = h->hs;
+int validate_system(sph_dem* h, const HostState& s) {
     if (s.contactmodel != 2) {
         h->err = "contactmodel " + std::to_string(s.contactmodel) +
                  " is not on this path (only the linear visco-elastic model 2)";
@@ -208,6 +262,30 @@ int validate_for_device(sph_dem* h) {
                      " >= np";
             return SPH_E_INVALID;
         }
+    }
+    return SPH_OK;
+}
+
+int validate_for_device(sph_dem* h) {
+    if (h->members.empty()) return validate_system(h, h->hs);
+    double K = 0.0, np = 0.0;
+    for (size_t k = 0; k < h->members.size(); ++k) {
+        const HostState& m = h->members[k];
+        int rc;
+        if ((rc = validate_system(h, m))) {
+            h->err = "member " + std::to_string(k) + ": " + h->err;
+            return rc;
+        }
+        if (m.nb0) {
+            h->err = "member " + std::to_string(k) + " has bonds; ensembles step unbonded systems";
+            return SPH_E_UNSUPPORTED;
+        }
+        K += (double)m.num[0] * m.num[1] * m.num[2];
+        np += m.np;
+    }
+    if (K >= 4294967295.0 || np > (double)sph::MAX_PARTICLES) {
+        h->err = "ensemble has too many cells or particles for 32-bit cell ids and 28-bit indices";
+        return SPH_E_INVALID;
     }
     return SPH_OK;
 }
@@ -286,23 +364,46 @@ int upload_all(sph_dem* h) {
     }
     HIP_TRY(hipMemcpy(d.a.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
-    // walls: entries past nw keep the defaults of src/contactsearch.cuh:416-427
-    double wnx[24] = {0.0, 0.0, -1.0, s.L[2], -1.0, 0.0, 0.0, s.L[0], 1.0, 0.0, 0.0, 0.0,
-                      0.0, -1.0, 0.0, s.L[1], 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double wmv[24] = {0};
-    double wv0[6] = {0};
-    int wmode[6] = {0};
-    for (uint32_t w = 0; w < s.nw; ++w) {
-        for (int a = 0; a < 4; ++a) { wnx[4 * w + a] = s.wnx[4 * w + a]; wmv[4 * w + a] = s.wmvfd[4 * w + a]; }
-        wv0[w] = s.wmvfd[4 * w + 1];
-        wmode[w] = s.wmode[w];
+    // walls, 6 entries per system: entries past its nw keep the defaults of
+    // src/contactsearch.cuh:416-427
+    const std::vector<const HostState*> sys = systems_of(h);
+    const size_t ns = sys.size();
+    std::vector<double> wnx(24 * ns), wmv(24 * ns, 0.0), wv0(6 * ns, 0.0), tcur(ns);
+    std::vector<int> wmode(6 * ns, 0);
+    for (size_t k = 0; k < ns; ++k) {
+        const HostState& m = *sys[k];
+        const double def[24] = {0.0, 0.0, -1.0, m.L[2], -1.0, 0.0, 0.0, m.L[0], 1.0, 0.0, 0.0, 0.0,
+                                0.0, -1.0, 0.0, m.L[1], 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        std::copy(def, def + 24, wnx.begin() + 24 * k);
+        for (uint32_t w = 0; w < m.nw; ++w) {
+            for (int a = 0; a < 4; ++a) {
+                wnx[24 * k + 4 * w + a] = m.wnx[4 * w + a];
+                wmv[24 * k + 4 * w + a] = m.wmvfd[4 * w + a];
+            }
+            wv0[6 * k + w] = m.wmvfd[4 * w + 1];
+            wmode[6 * k + w] = m.wmode[w];
+        }
+        tcur[k] = m.t_cur;
     }
-    HIP_TRY(hipMemcpy(d.wnx, wnx, sizeof(wnx), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.wmvfd, wmv, sizeof(wmv), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.wvel0, wv0, sizeof(wv0), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.wmode, wmode, sizeof(wmode), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.t_cur, &s.t_cur, 8, hipMemcpyHostToDevice));
-    if (s.nw) HIP_TRY(hipMemset(d.wall_pp, 0, (size_t)s.nw * n * 8));
+    HIP_TRY(hipMemcpy(d.wnx, wnx.data(), wnx.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wmvfd, wmv.data(), wmv.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wvel0, wv0.data(), wv0.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.wmode, wmode.data(), wmode.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.t_cur, tcur.data(), ns * 8, hipMemcpyHostToDevice));
+    if (d.nw) HIP_TRY(hipMemset(d.wall_pp, 0, (size_t)d.nw * n * 8));
+    if (!h->members.empty()) {
+        // system table, system of each index and of each wall-sum block
+        std::vector<uint16_t> of(n), ofb(d.nwall_blocks);
+        for (size_t k = 0; k < ns; ++k) {
+            const sph::Sys& S = h->sys_host[k];
+            std::fill(of.begin() + S.pbase, of.begin() + S.pbase + S.np, (uint16_t)k);
+            std::fill(ofb.begin() + S.bbase, ofb.begin() + S.bbase + S.nblocks, (uint16_t)k);
+        }
+        HIP_TRY(hipMemcpy(const_cast<sph::Sys*>(d.systems), h->sys_host.data(), ns * sizeof(sph::Sys),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(const_cast<uint16_t*>(d.sys_of), of.data(), n * 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(const_cast<uint16_t*>(d.sys_of_block), ofb.data(), ofb.size() * 2, hipMemcpyHostToDevice));
+    }
     // bonds
     const size_t nb = s.nb0;
     if (nb) {
@@ -379,9 +480,16 @@ int allocate(sph_dem* h) {
     ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
     ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
-    ALLOC(d.wnx, 6); ALLOC(d.wmvfd, 6); ALLOC(d.wvel0, 6); ALLOC(d.wmode, 6);
-    ALLOC(d.wall_pp, (size_t)h->hs.nw * n); ALLOC(d.wall_partial, (size_t)h->hs.nw * d.nwall_blocks);
-    ALLOC(d.t_cur, 1);
+    const size_t ns = h->members.empty() ? 1 : h->members.size();
+    ALLOC(d.wnx, 6 * ns); ALLOC(d.wmvfd, 6 * ns); ALLOC(d.wvel0, 6 * ns); ALLOC(d.wmode, 6 * ns);
+    ALLOC(d.wall_pp, (size_t)d.nw * n); ALLOC(d.wall_partial, (size_t)d.nw * d.nwall_blocks);
+    ALLOC(d.t_cur, ns);
+    if (!h->members.empty()) {
+        sph::Sys* tbl;
+        uint16_t *of, *ofb;
+        ALLOC(tbl, ns); ALLOC(of, n); ALLOC(ofb, d.nwall_blocks);
+        d.systems = tbl; d.sys_of = of; d.sys_of_block = ofb;
+    }
     const size_t nb = h->hs.nb0;
     ALLOC(d.bonds, nb); ALLOC(d.bdelta, nb); ALLOC(d.bomega, nb); ALLOC(d.bond_f, nb); ALLOC(d.bond_t, nb);
     ALLOC(d.bcsr_off, n + 1);
@@ -611,25 +719,37 @@ int make_events(sph_dem* h) {
 // are ordinary steps of the run (all kernels give the same bits).
 constexpr uint64_t TUNE_STEPS = 12;
 
+// One ordinary step, timed per stage: ms[0] search, ms[1] force.
+int timed_step(sph_dem* h, float ms[2]) {
+    sph::enqueue_step(h, true, h->own_cur, false, true);
+    h->own_cur ^= 1;
+    h->hs.t_cur += h->hs.dt;
+    ++h->steps_done;
+    HIP_TRY(hipEventSynchronize(h->ev[7]));
+    HIP_TRY(hipEventElapsedTime(&ms[0], h->ev[4], h->ev[5]));
+    HIP_TRY(hipEventElapsedTime(&ms[1], h->ev[5], h->ev[6]));
+    return SPH_OK;
+}
+
+// Threads per block tried for the search and force kernels.
+constexpr uint32_t TUNE_BLOCKS[3] = {64, 128, 256};
+
 }  // namespace
 
-int sph::tune_search(sph_dem* h) {
+int sph::tune_search(sph_dem* h, uint64_t* ran) {
     int rc;
     if ((rc = make_events(h))) return rc;
     const int modes[3] = {2, 4, 5};
     float mean[3] = {0.0f, 0.0f, 0.0f};
+    uint64_t steps = 0;
     for (int k = 0; k < 3; ++k) {
         h->d.search_mode = modes[k];
         if (modes[k] == 5) request_list_rebuild(h);
         for (int rep = 0; rep < 4; ++rep) {
-            sph::enqueue_step(h, true, h->own_cur, false, true);
-            h->own_cur ^= 1;
-            h->hs.t_cur += h->hs.dt;
-            ++h->steps_done;
-            HIP_TRY(hipEventSynchronize(h->ev[7]));
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
-            if (rep > 0) mean[k] += ms / 3.0f;
+            float ms[2];
+            if ((rc = timed_step(h, ms))) return rc;
+            ++steps;
+            if (rep > 0) mean[k] += ms[0] / 3.0f;
         }
     }
     int best = 0;
@@ -637,6 +757,28 @@ int sph::tune_search(sph_dem* h) {
         if (mean[k] < mean[best]) best = k;
     h->d.search_mode = modes[best];
     if (modes[best] == 5) request_list_rebuild(h);
+    if (h->blocks_auto) {
+        // threads per block of the chosen search kernel and of k_force, on
+        // the same steps (mean of the last two of three per size)
+        float tb[3][2] = {};
+        for (int k = 0; k < 3; ++k) {
+            h->d.tpb_search = h->d.tpb_force = TUNE_BLOCKS[k];
+            for (int rep = 0; rep < 3; ++rep) {
+                float ms[2];
+                if ((rc = timed_step(h, ms))) return rc;
+                ++steps;
+                if (rep > 0) { tb[k][0] += ms[0] / 2.0f; tb[k][1] += ms[1] / 2.0f; }
+            }
+        }
+        int bs = 0, bf = 0;
+        for (int k = 1; k < 3; ++k) {
+            if (tb[k][0] < tb[bs][0]) bs = k;
+            if (tb[k][1] < tb[bf][1]) bf = k;
+        }
+        h->d.tpb_search = TUNE_BLOCKS[bs];
+        h->d.tpb_force = TUNE_BLOCKS[bf];
+    }
+    if (ran) *ran = steps;
     destroy_graph(h);
     h->tuned = true;
     h->steps_since_tune = 0;
@@ -676,9 +818,11 @@ int run_steps(sph_dem* h, uint64_t n) {
         h->timing = t;
         return check_flags(h);
     }
-    if (h->search_auto && n >= 2 * TUNE_STEPS && (!h->tuned || h->steps_since_tune >= 4096)) {
-        if ((rc = sph::tune_search(h))) return rc;
-        n -= TUNE_STEPS;
+    const uint64_t tune_steps = TUNE_STEPS + (h->blocks_auto ? 9 : 0);
+    if (h->search_auto && n >= 2 * tune_steps && (!h->tuned || h->steps_since_tune >= 4096)) {
+        uint64_t ran = 0;
+        if ((rc = sph::tune_search(h, &ran))) return rc;
+        n -= ran;
     }
     h->steps_since_tune += n;
     if (h->use_graph && (rc = ensure_graph(h))) return rc;
@@ -1070,7 +1214,7 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
             std::memcpy(tm, src, 32);
             h->hs.dt = tm[0]; h->hs.t_cur = tm[1]; h->hs.t_total = tm[2]; h->hs.file_dt = tm[3];
             HIP_TRY(hipMemcpy(d.t_cur, &h->hs.t_cur, 8, hipMemcpyHostToDevice));
-            if (d.dt != h->hs.dt) { d.dt = h->hs.dt; destroy_graph(h); }
+            if (d.sys.dt != h->hs.dt) { d.sys.dt = h->hs.dt; destroy_graph(h); }
             return SPH_OK;
         }
         default: break;
@@ -1113,7 +1257,10 @@ int sph_set_precision(sph_t* h, int precision) {
     int rc;
     const size_t cnt = (size_t)h->hs.np * sph::NC * 3;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    if (precision == SPH_PREC_MIXED) {
+    if (precision == SPH_PREC_MIXED && h->steps_done == 0) {
+        // no step yet: every spring is zero, as the fresh FP32 buffer is
+        if ((rc = ensure_mixed_buffers(h))) return rc;
+    } else if (precision == SPH_PREC_MIXED) {
         if ((rc = ensure_mixed_buffers(h))) return rc;
         std::vector<double> a(cnt);
         std::vector<float> b(cnt);
@@ -1195,5 +1342,347 @@ int sph_bandwidth_probe(uint64_t bytes, int reps, double* gbs) {
     if (b) (void)hipFree(b);
     return rc;
 }
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- ensembles
+
+struct sph_ens {
+    sph_dem* h = nullptr;
+    double cell_factor = -1.0;
+    double tuned_ms[4] = {0.0, 0.0, 0.0, 0.0};
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_ens_error;
+constexpr double CELL_FACTORS[4] = {1.0, 1.25, 1.5, 2.0};
+
+// Cells of factor x 2 r_max on every axis (at least one cell, three on a
+// periodic axis).
+void regrid(HostState& m, double factor) {
+    const double edge = factor * 2.0 * max_radius(m);
+    for (int a = 0; a < 3; ++a) {
+        double c = edge > 0.0 ? std::floor(m.L[a] / edge) : 1.0;
+        const bool per = (a == 0 && (m.periodic == 1 || m.periodic == 2)) || (a == 1 && m.periodic == 1);
+        c = std::max(c, per ? 3.0 : 1.0);
+        m.num[a] = (uint32_t)std::min(c, 4.0e9);
+    }
+}
+
+// The members' particles back to back, in member order.
+void concatenate(const std::vector<HostState>& ms, HostState& out) {
+    out = HostState{};
+    const HostState& f = ms[0];
+    out.dt = f.dt; out.t_cur = f.t_cur; out.t_total = f.t_total; out.file_dt = f.file_dt;
+    for (int a = 0; a < 3; ++a) { out.origo[a] = f.origo[a]; out.L[a] = f.L[a]; out.num[a] = f.num[a]; out.g[a] = f.g[a]; }
+    out.periodic = f.periodic;
+    out.k_n = f.k_n; out.k_t = f.k_t; out.gamma_n = f.gamma_n; out.gamma_t = f.gamma_t;
+    out.mu_s = f.mu_s; out.mu_d = f.mu_d; out.gamma_wn = f.gamma_wn; out.gamma_wt = f.gamma_wt;
+    out.mu_ws = f.mu_ws; out.mu_wd = f.mu_wd; out.rho = f.rho; out.kappa = f.kappa;
+    for (const HostState& m : ms) {
+        out.np += m.np;
+        out.nw = std::max(out.nw, m.nw);
+        auto app = [](std::vector<double>& dst, const std::vector<double>& src) {
+            dst.insert(dst.end(), src.begin(), src.end());
+        };
+        app(out.x, m.x); app(out.xysum, m.xysum); app(out.vel, m.vel); app(out.force, m.force);
+        app(out.angpos, m.angpos); app(out.angvel, m.angvel); app(out.torque, m.torque);
+        app(out.es_dot, m.es_dot); app(out.es, m.es); app(out.ev_dot, m.ev_dot); app(out.ev, m.ev); app(out.p, m.p);
+    }
+}
+
+void drop_particles(HostState& m) {
+    for (std::vector<double>* v : {&m.x, &m.xysum, &m.vel, &m.force, &m.angpos, &m.angvel, &m.torque, &m.es_dot,
+                                   &m.es, &m.ev_dot, &m.ev, &m.p}) {
+        v->clear();
+        v->shrink_to_fit();
+    }
+}
+
+// Opens an ensemble on `ms` (already on their final grids).
+int ens_open(std::vector<HostState> ms, int verbosity, sph_ens** out, std::string& err) {
+    *out = nullptr;
+    auto* h = new sph_dem();
+    h->members = std::move(ms);
+    for (size_t k = 0; k < h->members.size(); ++k) {
+        std::string e;
+        int rc = sph::check_values(h->members[k], e);
+        if (rc) {
+            err = "member " + std::to_string(k) + ": " + e;
+            delete h;
+            return rc;
+        }
+    }
+    concatenate(h->members, h->hs);
+    h->blocks_auto = true;
+    int rc = open_common(h, verbosity, 0);
+    if (rc) {
+        err = h->err;
+        free_all(h);
+        delete h;
+        return rc;
+    }
+    for (HostState& m : h->members) drop_particles(m);
+    drop_particles(h->hs);
+    auto* e = new sph_ens();
+    e->h = h;
+    *out = e;
+    return SPH_OK;
+}
+
+// ms per step of an ensemble on `ms` regridded by `factor`: the autotuner's
+// steps (search kernel and block sizes) and then ten graph-replayed steps.
+int trial_ms(const std::vector<HostState>& ms, double factor, double& ms_step, std::string& err) {
+    std::vector<HostState> t = ms;
+    for (HostState& m : t) regrid(m, factor);
+    sph_ens* e = nullptr;
+    int rc = ens_open(std::move(t), 0, &e, err);
+    if (rc) return rc;
+    sph_dem* h = e->h;
+    rc = run_steps(h, 2 * (TUNE_STEPS + 9));
+    hipEvent_t a = nullptr, b = nullptr;
+    if (!rc && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) rc = SPH_E_HIP;
+    float ms10 = 0.0f;
+    if (!rc) {
+        (void)hipEventRecord(a, h->stream);
+        rc = run_steps(h, 10);
+        (void)hipEventRecord(b, h->stream);
+        if (!rc && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms10, a, b) != hipSuccess))
+            rc = SPH_E_HIP;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc) err = h->err;
+    ms_step = ms10 / 10.0;
+    sph_ens_close(e);
+    return rc;
+}
+
+int ens_open_states(std::vector<HostState> ms, double cell_factor, int verbosity, sph_ens_t** out) {
+    *out = nullptr;
+    std::string err;
+    if (ms.empty() || ms.size() > 65535) {
+        g_ens_error = "an ensemble has 1 to 65535 members";
+        return SPH_E_INVALID;
+    }
+    if (cell_factor > 0.0 && cell_factor < 1.0) {
+        g_ens_error = "cell_factor below 1 would make cells narrower than the largest pair";
+        return SPH_E_INVALID;
+    }
+    double tuned[4] = {0.0, 0.0, 0.0, 0.0};
+    if (cell_factor == 0.0) {
+        int best = -1;
+        for (int k = 0; k < 4; ++k) {
+            double t = 0.0;
+            const int rc = trial_ms(ms, CELL_FACTORS[k], t, err);
+            if (rc) continue;  // e.g. too few cells on a periodic axis at this size
+            tuned[k] = t;
+            if (best < 0 || t < tuned[best]) best = k;
+        }
+        if (best < 0) {
+            g_ens_error = "autotuner: no cell factor gives a valid ensemble: " + err;
+            return SPH_E_INVALID;
+        }
+        cell_factor = CELL_FACTORS[best];
+        if (verbosity > 0)
+            std::printf("sphere_dem: ensemble cell factor %.2f (ms/step %.4f %.4f %.4f %.4f)\n", cell_factor,
+                        tuned[0], tuned[1], tuned[2], tuned[3]);
+    }
+    if (cell_factor >= 1.0)
+        for (HostState& m : ms) regrid(m, cell_factor);
+    const int rc = ens_open(std::move(ms), verbosity, out, err);
+    if (rc) {
+        g_ens_error = err;
+        return rc;
+    }
+    (*out)->cell_factor = cell_factor;
+    std::copy(tuned, tuned + 4, (*out)->tuned_ms);
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_ens_open_mem(const void* const* images, const size_t* bytes, uint32_t n, double cell_factor, int verbosity,
+                     sph_ens_t** out) {
+    if (!out) return SPH_E_INVALID;
+    *out = nullptr;
+    if (!images || !bytes) return SPH_E_INVALID;
+    std::vector<HostState> ms(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        std::string e;
+        const int rc = sph::parse_bin(images[k], bytes[k], ms[k], e);
+        if (rc) {
+            g_ens_error = "member " + std::to_string(k) + ": " + e;
+            return rc;
+        }
+    }
+    return ens_open_states(std::move(ms), cell_factor, verbosity, out);
+}
+
+int sph_ens_open(const char* const* paths, uint32_t n, double cell_factor, int verbosity, sph_ens_t** out) {
+    if (!out) return SPH_E_INVALID;
+    *out = nullptr;
+    if (!paths) return SPH_E_INVALID;
+    std::vector<HostState> ms(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        std::string e;
+        const int rc = sph::read_bin_file(paths[k], ms[k], e);
+        if (rc) {
+            g_ens_error = "member " + std::to_string(k) + ": " + e;
+            return rc;
+        }
+    }
+    return ens_open_states(std::move(ms), cell_factor, verbosity, out);
+}
+
+void sph_ens_close(sph_ens_t* e) {
+    if (!e) return;
+    if (e->h) {
+        sph_close(e->h);
+        e->h = nullptr;
+    }
+    delete e;
+}
+
+const char* sph_ens_last_error(const sph_ens_t* e) { return e ? e->h->err.c_str() : g_ens_error.c_str(); }
+
+int sph_ens_step(sph_ens_t* e, uint64_t nsteps) {
+    if (!e) return SPH_E_INVALID;
+    return run_steps(e->h, nsteps);
+}
+
+size_t sph_ens_field_bytes(const sph_ens_t* e, uint32_t k, int f) {
+    if (!e || k >= e->h->sys_host.size() || !field_is_known(f)) return 0;
+    const sph::Sys& S = e->h->sys_host[k];
+    const size_t n = S.np, nw = S.nw, nc = sph::NC;
+    switch (f) {
+        case SPH_X: case SPH_VEL: return n * 32;
+        case SPH_ANGVEL: case SPH_FORCE: case SPH_TORQUE: case SPH_ANGPOS: case SPH_VEL0:
+        case SPH_ANGVEL0: return n * 24;
+        case SPH_XYSUM: return n * 16;
+        case SPH_ES_DOT: case SPH_ES: case SPH_EV_DOT: case SPH_EV: case SPH_P: return n * 8;
+        case SPH_CONTACTS: return n * nc * 4;
+        case SPH_DELTA_T: return n * nc * 24;
+        case SPH_WALLS_NX: case SPH_WALLS_MVFD: return nw * 32;
+        case SPH_WALLS_VEL0: return nw * 8;
+        case SPH_WALL_PP: return nw * n * 8;
+        case SPH_TIME: return 32;
+        default: return 0;
+    }
+}
+
+int sph_ens_get(sph_ens_t* e, uint32_t k, int f, void* dst, size_t bytes) {
+    if (!e || !dst) return SPH_E_INVALID;
+    sph_dem* h = e->h;
+    if (k >= h->sys_host.size()) {
+        h->err = "sph_ens_get: no member " + std::to_string(k);
+        return SPH_E_INVALID;
+    }
+    const size_t need = sph_ens_field_bytes(e, k, f);
+    if (need == 0) {
+        h->err = "sph_ens_get: field " + std::to_string(f) + " is not available per member";
+        return SPH_E_UNSUPPORTED;
+    }
+    if (bytes != need) {
+        h->err = "sph_ens_get: field " + std::to_string(f) + " needs " + std::to_string(need) + " bytes";
+        return SPH_E_INVALID;
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const sph::Sys& S = h->sys_host[k];
+    const size_t n = S.np, pb = S.pbase, N = h->d.np, nc = sph::NC;
+    char* out = static_cast<char*>(dst);
+    switch (f) {
+        case SPH_WALLS_NX: HIP_TRY(hipMemcpy(dst, h->d.wnx + S.wbase, need, hipMemcpyDeviceToHost)); return SPH_OK;
+        case SPH_WALLS_MVFD: HIP_TRY(hipMemcpy(dst, h->d.wmvfd + S.wbase, need, hipMemcpyDeviceToHost)); return SPH_OK;
+        case SPH_WALLS_VEL0: HIP_TRY(hipMemcpy(dst, h->d.wvel0 + S.wbase, need, hipMemcpyDeviceToHost)); return SPH_OK;
+        case SPH_WALL_PP:
+            for (uint32_t w = 0; w < S.nw; ++w)
+                HIP_TRY(hipMemcpy(out + (size_t)w * n * 8, h->d.wall_pp + (size_t)w * N + pb, n * 8,
+                                  hipMemcpyDeviceToHost));
+            return SPH_OK;
+        case SPH_TIME: {
+            const HostState& m = h->members[k];
+            double tm[4] = {m.dt, 0.0, m.t_total, m.file_dt};
+            HIP_TRY(hipMemcpy(&tm[1], h->d.t_cur + k, 8, hipMemcpyDeviceToHost));
+            std::memcpy(dst, tm, 32);
+            return SPH_OK;
+        }
+        default: break;
+    }
+    // per-particle fields: the whole ensemble's array, then the member's rows
+    const size_t all = sph_field_bytes(h, f), row = all / N;
+    std::vector<char> tmp(all);
+    int rc = sph_get(h, f, tmp.data(), all);
+    if (rc) return rc;
+    std::memcpy(out, tmp.data() + pb * row, n * row);
+    if (f == SPH_CONTACTS) {  // partners in the member's numbering, empty = its np
+        uint32_t* c = reinterpret_cast<uint32_t*>(out);
+        for (size_t i = 0; i < n * nc; ++i) c[i] = c[i] == (uint32_t)N ? (uint32_t)n : c[i] - (uint32_t)pb;
+    }
+    return SPH_OK;
+}
+
+int sph_ens_member_info(const sph_ens_t* e, uint32_t k, sph_info_t* out) {
+    if (!e || !out || k >= e->h->sys_host.size()) return SPH_E_INVALID;
+    const sph_dem* h = e->h;
+    const sph::Sys& S = h->sys_host[k];
+    const HostState& m = h->members[k];
+    out->np = S.np; out->nb0 = 0; out->nw = S.nw; out->nc = sph::NC; out->ncells = S.K;
+    for (int a = 0; a < 3; ++a) out->num[a] = S.num[a];
+    out->periodic = S.periodic;
+    out->dt = m.dt;
+    out->t_current = m.t_cur + (double)h->steps_done * m.dt;  // (sph_ens_get(SPH_TIME) has the exact sum)
+    out->t_total = m.t_total; out->file_dt = m.file_dt;
+    out->steps_done = h->steps_done;
+    out->precision = h->precision;
+    return SPH_OK;
+}
+
+int sph_ens_info(const sph_ens_t* e, sph_ens_info_t* out) {
+    if (!e || !out) return SPH_E_INVALID;
+    const sph_dem* h = e->h;
+    out->members = (uint32_t)h->sys_host.size();
+    out->np = h->d.np; out->ncells = h->d.K; out->steps_done = h->steps_done;
+    out->cell_factor = e->cell_factor;
+    for (int k = 0; k < 4; ++k) out->tuned_ms[k] = e->tuned_ms[k];
+    out->search_mode = h->d.search_mode;
+    out->force_block = h->d.tpb_force; out->search_block = h->d.tpb_search;
+    out->precision = h->precision;
+    return SPH_OK;
+}
+
+int sph_ens_set_precision(sph_ens_t* e, int precision) { return e ? sph_set_precision(e->h, precision) : SPH_E_INVALID; }
+
+int sph_ens_set_search(sph_ens_t* e, int mode) {
+    if (!e) return SPH_E_INVALID;
+    if (mode != 0 && mode != 2 && mode != 4 && mode != 5) {
+        e->h->err = "ensembles use search mode 0, 2, 4 or 5";
+        return SPH_E_INVALID;
+    }
+    e->h->blocks_auto = mode == 0;
+    return sph_set_search(e->h, mode);
+}
+
+int sph_ens_set_blocks(sph_ens_t* e, uint32_t force_block, uint32_t search_block) {
+    if (!e) return SPH_E_INVALID;
+    auto ok = [](uint32_t b) { return b >= 64 && b <= (uint32_t)sph::BLOCK && b % 64 == 0; };
+    if (!ok(force_block) || !ok(search_block)) {
+        e->h->err = "threads per block: a multiple of 64 up to 256";
+        return SPH_E_INVALID;
+    }
+    e->h->blocks_auto = false;
+    e->h->d.tpb_force = force_block;
+    e->h->d.tpb_search = search_block;
+    destroy_graph(e->h);
+    return SPH_OK;
+}
+
+int sph_ens_set_graph(sph_ens_t* e, int on) { return e ? sph_set_graph(e->h, on) : SPH_E_INVALID; }
+int sph_ens_set_profiling(sph_ens_t* e, int on) { return e ? sph_set_profiling(e->h, on) : SPH_E_INVALID; }
+int sph_ens_get_timing(const sph_ens_t* e, sph_timing_t* out) { return e ? sph_get_timing(e->h, out) : SPH_E_INVALID; }
 
 }  // extern "C"

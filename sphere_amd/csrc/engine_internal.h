@@ -46,6 +46,12 @@ struct sph_dem {
     bool tuned = false;
     // checkpoints, tape and adjoint buffers of sph_adjoint_* (adjoint.hip)
     std::unique_ptr<sph::AdjointState, void (*)(sph::AdjointState*)> adj{nullptr, nullptr};
+    // ensemble (sph_ens_*): the members' host states (scalars, walls and,
+    // until upload, particles); hs then holds their concatenation and
+    // sys_host the table behind d.systems
+    std::vector<sph::HostState> members;
+    std::vector<sph::Sys> sys_host;
+    bool blocks_auto = false;  // tune threads per block with the search kernel
 };
 
 namespace sph {
@@ -63,8 +69,9 @@ struct StepHooks {
 void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail, const StepHooks* hooks = nullptr);
 
 // Runs a few ordinary steps with each contact-search kernel and keeps the
-// fastest (search mode 0).
-int tune_search(sph_dem* h);
+// fastest (search mode 0); with blocks_auto then also the threads per block
+// of the search and force kernels.  *ran receives the number of steps run.
+int tune_search(sph_dem* h, uint64_t* ran = nullptr);
 
 }  // namespace sph
 

@@ -48,26 +48,34 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
+// The system of index / sorted position i: the handle's only one, or its
+// entry of the ensemble table.
+template <bool B>
+__device__ __forceinline__ const Sys& sys_at(const DemDev& d, uint32_t i) {
+    if constexpr (B) return d.systems[d.sys_of[i]];
+    else return d.sys;
+}
+
 // ------------------------------------------------------------ cell hashing
 // Periodic wrap with the rule of src/integration.cuh:80-95.
-__device__ __forceinline__ bool wrap_xy(const DemDev& d, double& x, double& y) {
+__device__ __forceinline__ bool wrap_xy(const Sys& S, double& x, double& y) {
     bool changed = false;
-    if (d.periodic == 1 || d.periodic == 2) {
-        if (x < d.origo[0]) { x += d.L[0]; changed = true; }
-        if (x > d.L[0]) { x -= d.L[0]; changed = true; }
+    if (S.periodic == 1 || S.periodic == 2) {
+        if (x < S.origo[0]) { x += S.L[0]; changed = true; }
+        if (x > S.L[0]) { x -= S.L[0]; changed = true; }
     }
-    if (d.periodic == 1) {
-        if (y < d.origo[1]) { y += d.L[1]; changed = true; }
-        if (y > d.L[1]) { y -= d.L[1]; changed = true; }
+    if (S.periodic == 1) {
+        if (y < S.origo[1]) { y += S.L[1]; changed = true; }
+        if (y > S.L[1]) { y -= S.L[1]; changed = true; }
     }
     return changed;
 }
 
 // Start-of-step position of sorted particle q: its stored record at the
 // previous position perm[q], wrapped as the step's integrate wraps it.
-__device__ __forceinline__ double4 spos(const DemDev& d, uint32_t q) {
+__device__ __forceinline__ double4 spos(const DemDev& d, const Sys& S, uint32_t q) {
     double4 x = d.ro.x[d.perm[q]];
-    (void)wrap_xy(d, x.x, x.y);
+    (void)wrap_xy(S, x.x, x.y);
     return x;
 }
 
@@ -80,11 +88,11 @@ struct PosView {
     int periodic;
 };
 
-__device__ __forceinline__ PosView pos_view(const DemDev& d) {
+__device__ __forceinline__ PosView pos_view(const DemDev& d, const Sys& S) {
     PosView v;
     v.x = d.ro.x; v.perm = d.perm;
-    v.ox = d.origo[0]; v.oy = d.origo[1]; v.Lx = d.L[0]; v.Ly = d.L[1];
-    v.periodic = d.periodic;
+    v.ox = S.origo[0]; v.oy = S.origo[1]; v.Lx = S.L[0]; v.Ly = S.L[1];
+    v.periodic = S.periodic;
     return v;
 }
 
@@ -102,12 +110,12 @@ __device__ __forceinline__ double4 pv_pos(const PosView& v, uint32_t q) {
 }
 
 // floor((v - origo)/(L/num)) (src/sorting.cuh:12-14), clamped to the grid.
-__device__ __forceinline__ int cell_coord(const DemDev& d, double v, int a, uint32_t& err) {
-    const double f = floor((v - d.origo[a]) / d.cell_len[a]);
+__device__ __forceinline__ int cell_coord(const Sys& S, double v, int a, uint32_t& err) {
+    const double f = floor((v - S.origo[a]) / S.cell_len[a]);
     if (f != f) { err |= FLAG_NAN; return 0; }
-    if (f < -1.0 || f > (double)d.num[a]) err |= FLAG_OUT_OF_GRID;
+    if (f < -1.0 || f > (double)S.num[a]) err |= FLAG_OUT_OF_GRID;
     if (f < 0.0) return 0;
-    if (f > (double)(d.num[a] - 1)) return (int)(d.num[a] - 1);
+    if (f > (double)(S.num[a] - 1)) return (int)(S.num[a] - 1);
     return (int)f;
 }
 
@@ -120,23 +128,25 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* p, uint64_t n, uin
 // integrate applies to it (src/integration.cuh:80-95) on a copy, then
 // src/sorting.cuh:7-19.  The position itself is wrapped when it is gathered
 // into the sorted order.
-__device__ __forceinline__ uint32_t cell_key(const DemDev& d, double x, double y, double z, uint32_t& err) {
-    (void)wrap_xy(d, x, y);
-    const int ix = cell_coord(d, x, 0, err);
-    const int iy = cell_coord(d, y, 1, err);
-    const int iz = cell_coord(d, z, 2, err);
-    return ((uint32_t)iz * d.num[1]) * d.num[0] + (uint32_t)iy * d.num[0] + (uint32_t)ix;
+__device__ __forceinline__ uint32_t cell_key(const Sys& S, double x, double y, double z, uint32_t& err) {
+    (void)wrap_xy(S, x, y);
+    const int ix = cell_coord(S, x, 0, err);
+    const int iy = cell_coord(S, y, 1, err);
+    const int iz = cell_coord(S, z, 2, err);
+    return S.cbase + (((uint32_t)iz * S.num[1]) * S.num[0] + (uint32_t)iy * S.num[0] + (uint32_t)ix);
 }
 
 // Cell ids of the stored positions, for a state that did not come out of a
 // step (after open or a position update); steps compute the next step's ids
 // in the force kernel.
+template <bool B>
 __global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
+    const Sys& S = sys_at<B>(d, i);
     const double4 x = d.ro.x[i];
     uint32_t err = 0;
-    d.key[i] = cell_key(d, x.x, x.y, x.z, err);
+    d.key[i] = cell_key(S, x.x, x.y, x.z, err);
     if (err) atomicOr(d.flags_next, err);
 }
 
@@ -301,12 +311,14 @@ __global__ __launch_bounds__(BLOCK) void k_cellsort(DemDev d) {
     }
 }
 
+template <bool B>
 __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
     if (q >= d.np) return;
+    const Sys& S = sys_at<B>(d, q);
     const uint32_t p = d.perm[q];
     double4 x = d.ro.x[p];
-    (void)wrap_xy(d, x.x, x.y);
+    (void)wrap_xy(S, x.x, x.y);
     d.xf[q] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
     const uint32_t id = d.ro.id[p];
     d.sid[q] = id;
@@ -326,9 +338,10 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
 __global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.nb0) return;
+    const Sys& S = d.sys;
     const uint2 bond = d.bonds[k];
     if (!(bond.x < d.np)) return;
-    const double dt = d.dt;
+    const double dt = S.dt;
     const double4 d0 = d.bdelta[k], o0 = d.bomega[k];
     V3<double> delta0_t = mk(d0.x, d0.y, d0.z);
     const double delta0_n = d0.w;
@@ -354,8 +367,8 @@ __global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
     const double delta_n = delta0_n + ddelta_n;
     const V3<double> ddelta_t = ddelta - n * dot(n, ddelta);
     const V3<double> delta_t = delta0_t + ddelta_t;
-    const V3<double> f_n = (d.k_n * A * delta_n + d.gamma_n * ddelta_n / dt) * n;
-    const V3<double> f_t = -d.k_t * A * delta_t - d.gamma_t * ddelta_t / dt;
+    const V3<double> f_n = (S.k_n * A * delta_n + S.gamma_n * ddelta_n / dt) * n;
+    const V3<double> f_t = -S.k_t * A * delta_t - S.gamma_t * ddelta_t / dt;
     const V3<double> f = f_n + f_t;
     const V3<double> mn = -n;
     omega0_t = omega0_t - mn * dot(omega0_t, mn);
@@ -364,8 +377,8 @@ __global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
     const double omega_n = omega0_n + domega_n;
     const V3<double> domega_t = domega - domega_n * mn;
     const V3<double> omega_t = omega0_t + domega_t;
-    const V3<double> t_n = -d.k_t * J * omega_n * mn;
-    const V3<double> t_t = -d.k_n * I * omega_t;
+    const V3<double> t_n = -S.k_t * J * omega_n * mn;
+    const V3<double> t_t = -S.k_n * I * omega_t;
     const V3<double> t_j = t_n + t_t;
     const double sigma_max = vlen(f_n) / A + vlen(t_t) * R_bar / I;
     const double tau_max = vlen(f_t) / A + vlen(t_n) * R_bar / J;
@@ -388,11 +401,11 @@ struct Acc {
 // src/contactmodels.cuh:283-493 in arithmetic T (double: the reference's
 // FP64 path; float: the mixed mode's pair force).  Sums go to FP64.
 template <class T>
-__device__ __forceinline__ void contact_linear(const DemDev& d, Acc& acc, V3<T> vel_a, V3<T> angvel_a,
+__device__ __forceinline__ void contact_linear(const Sys& S, Acc& acc, V3<T> vel_a, V3<T> angvel_a,
                                                V3<T> vel_b, V3<T> angvel_b, T radius_a, T radius_b,
                                                V3<T> x, T x_length, T delta, V3<T>& delta_t_io) {
-    const T k_n = (T)d.k_n, k_t = (T)d.k_t, gamma_n = (T)d.gamma_n, gamma_t = (T)d.gamma_t;
-    const T mu_s = (T)d.mu_s, mu_d = (T)d.mu_d, kappa = (T)d.kappa, dt = (T)d.dt;
+    const T k_n = (T)S.k_n, k_t = (T)S.k_t, gamma_n = (T)S.gamma_n, gamma_t = (T)S.gamma_t;
+    const T mu_s = (T)S.mu_s, mu_d = (T)S.mu_d, kappa = (T)S.kappa, dt = (T)S.dt;
     const V3<T> delta_t0_uncor = delta_t_io;
     const V3<T> n = x / x_length;
     const V3<T> vel_linear = vel_a - vel_b;
@@ -434,7 +447,7 @@ __device__ __forceinline__ void contact_linear(const DemDev& d, Acc& acc, V3<T> 
 }
 
 // src/contactmodels.cuh:10-109 contactLinear_wall (FP64 in both modes).
-__device__ __forceinline__ double contact_wall(const DemDev& d, Acc& acc, V3<double> linvel, V3<double> angvel,
+__device__ __forceinline__ double contact_wall(const Sys& S, Acc& acc, V3<double> linvel, V3<double> angvel,
                                                double radius_a, V3<double> n, double delta, double wvel) {
     const V3<double> v0 = linvel + radius_a * cross(n, angvel);
     // operator+(Float3, Float) adds the scalar wall velocity to every component
@@ -442,13 +455,13 @@ __device__ __forceinline__ double contact_wall(const DemDev& d, Acc& acc, V3<dou
     const double vel_n = dot(vel, n);
     const V3<double> vel_t = vel - n * dot(vel, n);
     const double vel_t_length = vlen(vel_t);
-    V3<double> f_n = (-d.k_n * delta - d.gamma_wn * vel_n) * n;
+    V3<double> f_n = (-S.k_n * delta - S.gamma_wn * vel_n) * n;
     if (dot(f_n, n) < 0.0) f_n = mk(0.0, 0.0, 0.0);
     const double f_n_length = vlen(f_n);
     V3<double> f_t = mk(0.0, 0.0, 0.0);
     if (vel_t_length > 0.0) {
-        const double f_t_visc = d.gamma_wt * vel_t_length;
-        const double f_t_limit = vel_t_length > (double)0.001f ? d.mu_wd * f_n_length : d.mu_ws * f_n_length;
+        const double f_t_visc = S.gamma_wt * vel_t_length;
+        const double f_t_limit = vel_t_length > (double)0.001f ? S.mu_wd * f_n_length : S.mu_ws * f_n_length;
         if (f_t_visc < f_t_limit) f_t = (-1.0 * f_t_visc) * vel_t / vel_t_length;
         else f_t = (-1.0 * f_t_limit) * vel_t / vel_t_length;
     }
@@ -459,10 +472,10 @@ __device__ __forceinline__ double contact_wall(const DemDev& d, Acc& acc, V3<dou
     return dot(f_n, n);
 }
 
-__device__ __forceinline__ void decode_image(const DemDev& d, uint32_t code, double& dx, double& dy) {
+__device__ __forceinline__ void decode_image(const Sys& S, uint32_t code, double& dx, double& dy) {
     const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
-    dx = cx == 1u ? d.L[0] : (cx == 2u ? -d.L[0] : 0.0);
-    dy = cy == 1u ? d.L[1] : (cy == 2u ? -d.L[1] : 0.0);
+    dx = cx == 1u ? S.L[0] : (cx == 2u ? -S.L[0] : 0.0);
+    dy = cy == 1u ? S.L[1] : (cy == 2u ? -S.L[1] : 0.0);
 }
 
 template <class T> struct SlotStore;
@@ -530,7 +543,7 @@ __device__ __noinline__ void slot_insert(uint32_t* slot, SlotState& st, uint32_t
 template <bool FROM_LDS>
 __device__ __forceinline__ double4 staged_pos(const DemDev& d, const TopoShared& sh, uint32_t lb, uint32_t sb) {
     if constexpr (FROM_LDS) return sh.pos[lb];
-    else return spos(d, sb);
+    else return spos(d, d.sys, sb);
 }
 
 // Walks the 27 cells of every home particle of the tile: 9 rows (z, y order),
@@ -580,7 +593,7 @@ __device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& s
                     if (lo == hi) continue;
                     const uint32_t code = (codes >> (8 * c)) & 0xffu;
                     double dmx, dmy;
-                    decode_image(d, code, dmx, dmy);
+                    decode_image(d.sys, code, dmx, dmy);
                     const uint32_t gb = sh.cglob[e + c] - lo;
                     for (uint32_t lb = lo; lb < hi; ++lb) {
                         const double4 xb = staged_pos<FROM_LDS>(d, sh, lb, gb + lb);
@@ -609,10 +622,11 @@ __device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& s
 
 __global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
     __shared__ TopoShared sh;
+    const Sys& S = d.sys;
     const uint32_t bt = xcd_block();
     const uint32_t txi = bt % d.tiles_x, rest = bt / d.tiles_x;
     const uint32_t tyi = rest % d.tiles_y, tz = rest / d.tiles_y;
-    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
+    const int nx = (int)S.num[0], ny = (int)S.num[1], nz = (int)S.num[2];
     const int x0 = (int)(txi * d.tile_x), y0 = (int)(tyi * d.tile_y);
     const int nxh = min((int)d.tile_x, nx - x0), nyh = min((int)d.tile_y, ny - y0);
     const int W = nxh + 2, H = nyh + 2;
@@ -630,15 +644,15 @@ __global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
         uint32_t code = 0, g0 = 0, cnt = 0;
         bool ok = cz >= 0 && cz < nz;
         if (ok && (cy < 0 || cy >= ny)) {
-            if (d.periodic == 1) { code |= (cy < 0 ? 1u : 2u) << 2; cy = cy < 0 ? ny - 1 : 0; }
+            if (S.periodic == 1) { code |= (cy < 0 ? 1u : 2u) << 2; cy = cy < 0 ? ny - 1 : 0; }
             else ok = false;
         }
         if (ok && (cx < 0 || cx >= nx)) {
-            if (d.periodic != 0) { code |= cx < 0 ? 1u : 2u; cx = cx < 0 ? nx - 1 : 0; }
+            if (S.periodic != 0) { code |= cx < 0 ? 1u : 2u; cx = cx < 0 ? nx - 1 : 0; }
             else ok = false;
         }
         if (ok) {
-            const uint32_t cid = (uint32_t)cx + (uint32_t)cy * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)cz;
+            const uint32_t cid = (uint32_t)cx + (uint32_t)cy * S.num[0] + (S.num[0] * S.num[1]) * (uint32_t)cz;
             g0 = d.off[cid];
             cnt = d.off[cid + 1] - g0;
         }
@@ -662,7 +676,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (sh.cstart[mid] <= i) lo = mid; else hi = mid - 1;
             }
-            sh.pos[i] = spos(d, sh.cglob[lo] + (i - sh.cstart[lo]));
+            sh.pos[i] = spos(d, S, sh.cglob[lo] + (i - sh.cstart[lo]));
         }
     }
     __syncthreads();
@@ -684,6 +698,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo(DemDev d) {
 struct RowSlots {
     uint32_t mask, fresh, nc, err, dirty;
     bool stop, loaded;
+    uint32_t ld;  // stride of the LDS slot column (threads per block)
 };
 
 // (Takes plain pointers rather than the DemDev, which a non-inlined call
@@ -701,7 +716,7 @@ __device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ s
     if (!st.loaded) {
         for (uint32_t m = st.mask; m; m &= m - 1) {
             const int i = __ffs(m) - 1;
-            sw[i * BLOCK] = gslot[i];
+            sw[i * st.ld] = gslot[i];
         }
         st.loaded = true;
     }
@@ -710,7 +725,7 @@ __device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ s
     int cpos = -1;
     for (uint32_t m = st.mask; m; m &= m - 1) {
         const int i = __ffs(m) - 1;
-        if ((sw[i * BLOCK] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
+        if ((sw[i * st.ld] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
     }
     if (cpos < 0) {
         const uint32_t empty = ~st.mask;
@@ -718,7 +733,7 @@ __device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ s
         cpos = 31 - __clz(empty);
         st.fresh |= 1u << cpos;
     }
-    sw[cpos * BLOCK] = bo | (code << 28);
+    sw[cpos * st.ld] = bo | (code << 28);
     st.dirty |= 1u << cpos;
     st.mask |= 1u << cpos;
     if (++st.nc > (uint32_t)NC) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; }
@@ -731,7 +746,7 @@ constexpr int ROW_BATCH = 8;
 // With BUILD, the pass also records every candidate within the list reach
 // (r_a + r_b + skin, same prefilter margin) in the particle's neighbour list.
 template <bool BUILD>
-__device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t hi, float dmx, float dmy,
+__device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t lo, uint32_t hi, float dmx, float dmy,
                                          uint32_t code, uint32_t s, float4 xaf, float ra_eps, float ra_skin,
                                          double4 xa, uint32_t* sw, const uint32_t* gslot, RowSlots& st,
                                          uint32_t* list, uint32_t& lcnt) {
@@ -764,7 +779,7 @@ __device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t 
             for (; hits; hits &= hits - 1) {
                 const uint32_t sb = sb0 + (uint32_t)(__ffs(hits) - 1);
                 if (sb == s) continue;
-                rows_try(pos_view(d), d.sid, sw, gslot, st, xa, sb, code);
+                rows_try(pos_view(d, S), d.sid, sw, gslot, st, xa, sb, code);
                 if (st.stop) return;
             }
         }
@@ -773,17 +788,18 @@ __device__ __forceinline__ void rows_run(const DemDev& d, uint32_t lo, uint32_t 
 
 // BUILD (search mode 5): runs only on steps whose neighbour lists must be
 // rebuilt, and then also rebuilds the lists.
-template <bool BUILD>
+template <bool BUILD, bool B>
 __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
-    __shared__ uint32_t sw_all[NC * BLOCK];  // slot i of thread t at [i * BLOCK + t]
+    extern __shared__ uint32_t sw_all[];  // NC x blockDim: slot i of thread t at [i * blockDim + t]
     if (BUILD && d.nstate[0] == 0u) return;
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
+    const Sys& S = sys_at<B>(d, s);
     uint32_t* sw = sw_all + threadIdx.x;
-    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
-    const uint32_t key = d.skey[s];
-    const int gx = (int)(key % d.num[0]), gy = (int)((key / d.num[0]) % d.num[1]), gz = (int)(key / (d.num[0] * d.num[1]));
-    const double4 xa = spos(d, s);
+    const int nx = (int)S.num[0], ny = (int)S.num[1], nz = (int)S.num[2];
+    const uint32_t key = d.skey[s] - S.cbase;
+    const int gx = (int)(key % S.num[0]), gy = (int)((key / S.num[0]) % S.num[1]), gz = (int)(key / (S.num[0] * S.num[1]));
+    const double4 xa = spos(d, S, s);
     const float4 xaf = d.xf[s];
     const float ra_eps = xaf.w + d.search_eps;
     const float ra_skin = ra_eps + d.list_skin;
@@ -794,6 +810,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     RowSlots st;
     st.mask = d.ro.cmask[d.perm[s]];
     st.fresh = 0; st.nc = 0; st.err = 0; st.dirty = 0; st.stop = false; st.loaded = false;
+    st.ld = blockDim.x;
     for (int dz = -1; dz <= 1 && !st.stop; ++dz) {
         const int tz = gz + dz;
         if (tz < 0 || tz == nz) continue;
@@ -801,14 +818,14 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
             int ty = gy + dy;
             uint32_t cy = 0;
             if (ty < 0 || ty == ny) {
-                if (d.periodic != 1) continue;
+                if (S.periodic != 1) continue;
                 cy = ty < 0 ? 1u : 2u;
                 ty = ty < 0 ? ny - 1 : 0;
             }
-            const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+            const uint32_t row = S.cbase + (uint32_t)ty * S.num[0] + (S.num[0] * S.num[1]) * (uint32_t)tz;
             if (gx >= 1 && gx + 1 < nx && cy == 0) {
                 // no periodic image anywhere in the run: one contiguous range
-                rows_run<BUILD>(d, d.off[row + gx - 1], d.off[row + gx + 2], 0.0f, 0.0f, 0u, s, xaf, ra_eps,
+                rows_run<BUILD>(d, S, d.off[row + gx - 1], d.off[row + gx + 2], 0.0f, 0.0f, 0u, s, xaf, ra_eps,
                                 ra_skin, xa, sw, gslot, st, list, lcnt);
                 continue;
             }
@@ -816,20 +833,20 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
                 int tx = gx + dx;
                 uint32_t cx = 0;
                 if (tx < 0 || tx == nx) {
-                    if (d.periodic == 0) continue;
+                    if (S.periodic == 0) continue;
                     cx = tx < 0 ? 1u : 2u;
                     tx = tx < 0 ? nx - 1 : 0;
                 }
-                const float dmx = cx == 1u ? d.Lf[0] : (cx == 2u ? -d.Lf[0] : 0.0f);
-                const float dmy = cy == 1u ? d.Lf[1] : (cy == 2u ? -d.Lf[1] : 0.0f);
-                rows_run<BUILD>(d, d.off[row + tx], d.off[row + tx + 1], dmx, dmy, cx | (cy << 2), s, xaf,
+                const float dmx = cx == 1u ? S.Lf[0] : (cx == 2u ? -S.Lf[0] : 0.0f);
+                const float dmy = cy == 1u ? S.Lf[1] : (cy == 2u ? -S.Lf[1] : 0.0f);
+                rows_run<BUILD>(d, S, d.off[row + tx], d.off[row + tx + 1], dmx, dmy, cx | (cy << 2), s, xaf,
                                 ra_eps, ra_skin, xa, sw, gslot, st, list, lcnt);
             }
         }
     }
     for (uint32_t m = st.dirty; m; m &= m - 1) {
         const int i = __ffs(m) - 1;
-        gslot[i] = sw[i * BLOCK];
+        gslot[i] = sw[i * st.ld];
     }
     d.tmask[s] = st.mask;
     d.tfresh[s] = st.fresh;
@@ -875,7 +892,7 @@ __device__ __noinline__ uint32_t half_hit(PosView pv, uint32_t* hcnt, uint64_t* 
 
 // Candidates (s, hi) of one run; the run's cells start their offset index at
 // vi0 and advance at sorted positions m1 and m2.
-__device__ __forceinline__ uint32_t half_run(const DemDev& d, uint32_t lo, uint32_t hi, uint32_t m1, uint32_t m2,
+__device__ __forceinline__ uint32_t half_run(const DemDev& d, const Sys& S, uint32_t lo, uint32_t hi, uint32_t m1, uint32_t m2,
                                              uint32_t vi0, float dmx, float dmy, uint32_t code, uint32_t s,
                                              float4 xaf, float ra_eps, double4 xa) {
     uint32_t err = 0;
@@ -897,20 +914,22 @@ __device__ __forceinline__ uint32_t half_run(const DemDev& d, uint32_t lo, uint3
             for (; hits; hits &= hits - 1) {
                 const uint32_t sb = sb0 + (uint32_t)(__ffs(hits) - 1);
                 const uint32_t vi = vi0 + (sb >= m1 ? 1u : 0u) + (sb >= m2 ? 1u : 0u);
-                err |= half_hit(pos_view(d), d.hcnt, d.hits, xa, s, sb, vi, code);
+                err |= half_hit(pos_view(d, S), d.hcnt, d.hits, xa, s, sb, vi, code);
             }
         }
     }
     return err;
 }
 
+template <bool B>
 __global__ __launch_bounds__(BLOCK) void k_topo_half(DemDev d) {
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
-    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
-    const uint32_t key = d.skey[s];
-    const int gx = (int)(key % d.num[0]), gy = (int)((key / d.num[0]) % d.num[1]), gz = (int)(key / (d.num[0] * d.num[1]));
-    const double4 xa = spos(d, s);
+    const Sys& S = sys_at<B>(d, s);
+    const int nx = (int)S.num[0], ny = (int)S.num[1], nz = (int)S.num[2];
+    const uint32_t key = d.skey[s] - S.cbase;
+    const int gx = (int)(key % S.num[0]), gy = (int)((key / S.num[0]) % S.num[1]), gz = (int)(key / (S.num[0] * S.num[1]));
+    const double4 xa = spos(d, S, s);
     const float4 xaf = d.xf[s];
     const float ra_eps = xaf.w + d.search_eps;
     uint32_t err = 0;
@@ -921,16 +940,16 @@ __global__ __launch_bounds__(BLOCK) void k_topo_half(DemDev d) {
             int ty = gy + dy;
             uint32_t cy = 0;
             if (ty < 0 || ty == ny) {
-                if (d.periodic != 1) continue;
+                if (S.periodic != 1) continue;
                 cy = ty < 0 ? 1u : 2u;
                 ty = ty < 0 ? ny - 1 : 0;
             }
-            const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+            const uint32_t row = S.cbase + (uint32_t)ty * S.num[0] + (S.num[0] * S.num[1]) * (uint32_t)tz;
             const uint32_t vrow = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3);
             if (gx >= 1 && gx + 1 < nx && cy == 0) {
                 const uint32_t hi = d.off[row + gx + 2];
                 if (hi <= s + 1) continue;  // the whole run lies at or below s
-                err |= half_run(d, d.off[row + gx - 1], hi, d.off[row + gx], d.off[row + gx + 1], vrow, 0.0f, 0.0f,
+                err |= half_run(d, S, d.off[row + gx - 1], hi, d.off[row + gx], d.off[row + gx + 1], vrow, 0.0f, 0.0f,
                                 0u, s, xaf, ra_eps, xa);
                 continue;
             }
@@ -938,13 +957,13 @@ __global__ __launch_bounds__(BLOCK) void k_topo_half(DemDev d) {
                 int tx = gx + dx;
                 uint32_t cx = 0;
                 if (tx < 0 || tx == nx) {
-                    if (d.periodic == 0) continue;
+                    if (S.periodic == 0) continue;
                     cx = tx < 0 ? 1u : 2u;
                     tx = tx < 0 ? nx - 1 : 0;
                 }
-                const float dmx = cx == 1u ? d.Lf[0] : (cx == 2u ? -d.Lf[0] : 0.0f);
-                const float dmy = cy == 1u ? d.Lf[1] : (cy == 2u ? -d.Lf[1] : 0.0f);
-                err |= half_run(d, d.off[row + tx], d.off[row + tx + 1], NO_SPLIT, NO_SPLIT, vrow + (uint32_t)(dx + 1),
+                const float dmx = cx == 1u ? S.Lf[0] : (cx == 2u ? -S.Lf[0] : 0.0f);
+                const float dmy = cy == 1u ? S.Lf[1] : (cy == 2u ? -S.Lf[1] : 0.0f);
+                err |= half_run(d, S, d.off[row + tx], d.off[row + tx + 1], NO_SPLIT, NO_SPLIT, vrow + (uint32_t)(dx + 1),
                                 dmx, dmy, cx | (cy << 2), s, xaf, ra_eps, xa);
             }
         }
@@ -1060,18 +1079,18 @@ constexpr int LIST_BATCH = 4;
 // offset and image as the reference's search would reach it and the exact
 // overlap test; returns the hit key, or 0 for no overlap (a real key is
 // never 0: its sorted position differs from the particle's own).
-__device__ __forceinline__ uint64_t list_pair(const DemDev& d, uint32_t sb, uint32_t kb, double4 x, double4 xa,
+__device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_t kb, double4 x, double4 xa,
                                               int gx, int gy, int gz) {
-    const uint32_t nx = d.num[0], ny = d.num[1];
+    const uint32_t nx = S.num[0], ny = S.num[1];
     int dx, dy, dz;
     uint32_t cx, cy, cz;
-    if (!cell_offset(gz, (int)(kb / (nx * ny)), (int)d.num[2], false, dz, cz)) return 0ull;
-    if (!cell_offset(gy, (int)((kb / nx) % ny), (int)ny, d.periodic == 1, dy, cy)) return 0ull;
-    if (!cell_offset(gx, (int)(kb % nx), (int)nx, d.periodic != 0, dx, cx)) return 0ull;
+    if (!cell_offset(gz, (int)(kb / (nx * ny)), (int)S.num[2], false, dz, cz)) return 0ull;
+    if (!cell_offset(gy, (int)((kb / nx) % ny), (int)ny, S.periodic == 1, dy, cy)) return 0ull;
+    if (!cell_offset(gx, (int)(kb % nx), (int)nx, S.periodic != 0, dx, cx)) return 0ull;
     const uint32_t code = cx | (cy << 2);
-    (void)wrap_xy(d, x.x, x.y);
-    const double dmx = cx == 1u ? d.L[0] : (cx == 2u ? -d.L[0] : 0.0);
-    const double dmy = cy == 1u ? d.L[1] : (cy == 2u ? -d.L[1] : 0.0);
+    (void)wrap_xy(S, x.x, x.y);
+    const double dmx = cx == 1u ? S.L[0] : (cx == 2u ? -S.L[0] : 0.0);
+    const double dmy = cy == 1u ? S.L[1] : (cy == 2u ? -S.L[1] : 0.0);
     const double ex = (xa.x - x.x) + dmx, ey = (xa.y - x.y) + dmy, ez = xa.z - x.z;
     const double d2 = ex * ex + ey * ey + ez * ez;
     if (!(sqrt(d2) - (xa.w + x.w) < 0.0)) return 0ull;  // exact test of src/contactsearch.cuh:242-248
@@ -1082,11 +1101,13 @@ __device__ __forceinline__ uint64_t list_pair(const DemDev& d, uint32_t sb, uint
 // An overlap whose partner already holds a slot is applied on the spot (its
 // slot does not depend on the order, a partner occurs once); the others are
 // collected and given empty slots in key order at the end.
+template <bool B>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
     if (d.nstate[0] != 0u) return;  // k_topo_rows<true> rebuilds the lists this step
     const uint32_t s = blockIdx.x * APPLY_BLOCK + threadIdx.x;
     if (s >= d.np) return;
+    const Sys& S = sys_at<B>(d, s);
     uint32_t* sw = sw_all + threadIdx.x;
     const uint32_t p = d.perm[s];
     const float4 xaf = d.xf[s];
@@ -1095,7 +1116,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     const uint32_t* list = d.nlist + li;  // entry k at list[k * np]
     const uint32_t nl = d.ncnt[li];
     const uint32_t mask0 = d.ro.cmask[p];
-    const bool px = d.periodic != 0, py = d.periodic == 1;
+    const bool px = S.periodic != 0, py = S.periodic == 1;
     uint64_t* hk = d.hits + (size_t)s * NC;
     uint32_t n = 0, nnew = 0, err = 0, dirty = 0;
     bool have_a = false;
@@ -1116,19 +1137,19 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
         for (int k = 0; k < LIST_BATCH; ++k) {
             float ex = xaf.x - c[k].x, ey = xaf.y - c[k].y;
             const float ez = xaf.z - c[k].z;
-            if (px) ex -= d.Lf[0] * rintf(ex / d.Lf[0]);
-            if (py) ey -= d.Lf[1] * rintf(ey / d.Lf[1]);
+            if (px) ex -= S.Lf[0] * rintf(ex / S.Lf[0]);
+            if (py) ey -= S.Lf[1] * rintf(ey / S.Lf[1]);
             const float d2 = ex * ex + ey * ey + ez * ez;
             const float rs = ra_eps + c[k].w;
             pass |= (i0 + k < nl && d2 <= rs * rs) ? 1u << k : 0u;
         }
         if (pass == 0u) continue;
         if (!have_a) {
-            xa = spos(d, s);
-            const uint32_t key = d.skey[s];
-            gx = (int)(key % d.num[0]);
-            gy = (int)((key / d.num[0]) % d.num[1]);
-            gz = (int)(key / (d.num[0] * d.num[1]));
+            xa = spos(d, S, s);
+            const uint32_t key = d.skey[s] - S.cbase;
+            gx = (int)(key % S.num[0]);
+            gy = (int)((key / S.num[0]) % S.num[1]);
+            gz = (int)(key / (S.num[0] * S.num[1]));
             gslot = d.cslot + (size_t)d.sid[s] * NC;
             for (uint32_t m = mask0; m; m &= m - 1) {
                 const int i = __ffs(m) - 1;
@@ -1148,7 +1169,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
 #pragma unroll
         for (int k = 0; k < LIST_BATCH; ++k) {
             if (!((pass >> k) & 1u)) continue;
-            const uint64_t v = list_pair(d, sb[k], kb[k], xb[k], xa, gx, gy, gz);
+            const uint64_t v = list_pair(S, sb[k], kb[k] - S.cbase, xb[k], xa, gx, gy, gz);
             if (v == 0ull) continue;
             ++n;
             int cpos = -1;
@@ -1233,14 +1254,14 @@ __device__ __forceinline__ Cand load_cand(const DemDev& d, uint32_t t, uint32_t 
     c.sb = sb;
     c.code = code;
     if (c.ok) {
-        const double4 xb = spos(d, sb);
+        const double4 xb = spos(d, d.sys, sb);
         c.x = xb.x; c.y = xb.y; c.z = xb.z; c.r = xb.w;
         c.bid = d.sid[sb];
     } else {
         c.x = c.y = c.z = c.r = 0.0;
         c.bid = 0;
     }
-    decode_image(d, code, c.dmx, c.dmy);
+    decode_image(d.sys, code, c.dmx, c.dmy);
     return c;
 }
 
@@ -1285,13 +1306,14 @@ __device__ __forceinline__ void member_chunk(const Cand& cb, uint32_t lane, uint
 }
 
 __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
+    const Sys& S = d.sys;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t c = xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6);
     if (c >= d.K) return;
     const uint32_t m0 = d.off[c], m1 = d.off[c + 1];
     if (m0 == m1) return;
-    const int nx = (int)d.num[0], ny = (int)d.num[1], nz = (int)d.num[2];
-    const int gx = (int)(c % d.num[0]), gy = (int)((c / d.num[0]) % d.num[1]), gz = (int)(c / (d.num[0] * d.num[1]));
+    const int nx = (int)S.num[0], ny = (int)S.num[1], nz = (int)S.num[2];
+    const int gx = (int)(c % S.num[0]), gy = (int)((c / S.num[0]) % S.num[1]), gz = (int)(c / (S.num[0] * S.num[1]));
 
     // run j = lane (0..26): cell (gx + j%3 - 1, gy + (j/3)%3 - 1, gz + j/9 - 1);
     // a row without periodic image is one run, held by its dx = -1 lane
@@ -1303,16 +1325,16 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
         bool ok = tz >= 0 && tz < nz;
         uint32_t cy = 0, cx = 0;
         if (ok && (ty < 0 || ty == ny)) {
-            if (d.periodic == 1) { cy = ty < 0 ? 1u : 2u; ty = ty < 0 ? ny - 1 : 0; }
+            if (S.periodic == 1) { cy = ty < 0 ? 1u : 2u; ty = ty < 0 ? ny - 1 : 0; }
             else ok = false;
         }
         if (ok) {
-            const uint32_t row = (uint32_t)ty * d.num[0] + (d.num[0] * d.num[1]) * (uint32_t)tz;
+            const uint32_t row = S.cbase + (uint32_t)ty * S.num[0] + (S.num[0] * S.num[1]) * (uint32_t)tz;
             if (gx >= 1 && gx + 1 < nx && cy == 0) {
                 if (dx == -1) { r_lo = d.off[row + gx - 1]; r_len = d.off[row + gx + 2] - r_lo; }
             } else {
                 if (tx < 0 || tx == nx) {
-                    if (d.periodic != 0) { cx = tx < 0 ? 1u : 2u; tx = tx < 0 ? nx - 1 : 0; }
+                    if (S.periodic != 0) { cx = tx < 0 ? 1u : 2u; tx = tx < 0 ? nx - 1 : 0; }
                     else ok = false;
                 }
                 if (ok) {
@@ -1339,7 +1361,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
         cand[j] = load_cand(d, 64u * j + lane, T, runs, r_lo, r_pre, r_len, r_code);
 
     for (uint32_t m = m0; m < m1; ++m) {
-        const double4 xa = spos(d, m);
+        const double4 xa = spos(d, S, m);
         const uint32_t o = d.sid[m];
         const uint32_t* slot = d.cslot + (size_t)o * NC;
         MemberState st;
@@ -1366,14 +1388,15 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
 // ------------------------------------------------------------ force kernel
 // One thread per sorted particle: contact forces from the occupied slots in
 // slot order, wall contacts, bond forces, leapfrog.
-template <class T>
+template <class T, bool B>
 __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
+    const Sys& S = sys_at<B>(d, s);
     const uint32_t o = d.sid[s];
     const uint32_t prev = d.perm[s];  // where this particle's records are stored
     double4 xa = d.ro.x[prev];
-    (void)wrap_xy(d, xa.x, xa.y);
+    (void)wrap_xy(S, xa.x, xa.y);
     const double ra = xa.w;
 
     // contact slots after the search kernel
@@ -1394,16 +1417,16 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         const uint32_t bo = w & SLOT_IDX_MASK;
         const uint32_t pb = d.perm[d.pos_of[bo]];
         double4 xb = d.ro.x[pb];
-        (void)wrap_xy(d, xb.x, xb.y);
+        (void)wrap_xy(S, xb.x, xb.y);
         double dmx, dmy;
-        decode_image(d, w >> 28, dmx, dmy);
+        decode_image(S, w >> 28, dmx, dmy);
         const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
         const double len = vlen(xab);
         const double delta_n = len - (ra + xb.w);
         if (delta_n < 0.0) {
             const size_t mp = (size_t)o * NC + i;
             V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
-            contact_linear<T>(d, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.ro.vel[pb])), cvt<T>(xyz(d.ro.angvel[pb])),
+            contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.ro.vel[pb])), cvt<T>(xyz(d.ro.angvel[pb])),
                               (T)ra, (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
             SlotStore<T>::store(d, mp, dt0);
         } else {
@@ -1414,41 +1437,41 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     // ---- walls (src/contactsearch.cuh:587-663)
     double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
     {
-        const double4 w0 = d.wnx[0];
+        const double4 w0 = d.wnx[S.wbase + 0];
         const double dw = w0.w - (xa.z + ra);
-        if (dw < 0.0) wf0 = contact_wall(d, acc, va, wa, ra, mk(w0.x, w0.y, w0.z), dw, d.wmvfd[0].y);
+        if (dw < 0.0) wf0 = contact_wall(S, acc, va, wa, ra, mk(w0.x, w0.y, w0.z), dw, d.wmvfd[S.wbase + 0].y);
     }
     {
-        const double dw = xa.z - ra - d.origo[2];
-        if (dw < 0.0) (void)contact_wall(d, acc, va, wa, ra, mk(0.0, 0.0, 1.0), dw, 0.0);
+        const double dw = xa.z - ra - S.origo[2];
+        if (dw < 0.0) (void)contact_wall(S, acc, va, wa, ra, mk(0.0, 0.0, 1.0), dw, 0.0);
     }
-    if (d.periodic == 0) {
-        const double4 w1 = d.wnx[1], w2 = d.wnx[2];
+    if (S.periodic == 0) {
+        const double4 w1 = d.wnx[S.wbase + 1], w2 = d.wnx[S.wbase + 2];
         double dw = w1.w - (xa.x + ra);
-        if (dw < 0.0) wf1 = contact_wall(d, acc, va, wa, ra, mk(w1.x, w1.y, w1.z), dw, d.wmvfd[1].y);
+        if (dw < 0.0) wf1 = contact_wall(S, acc, va, wa, ra, mk(w1.x, w1.y, w1.z), dw, d.wmvfd[S.wbase + 1].y);
         dw = xa.x - ra - w2.w;
-        if (dw < 0.0) wf2 = contact_wall(d, acc, va, wa, ra, mk(w2.x, w2.y, w2.z), dw, d.wmvfd[2].y);
+        if (dw < 0.0) wf2 = contact_wall(S, acc, va, wa, ra, mk(w2.x, w2.y, w2.z), dw, d.wmvfd[S.wbase + 2].y);
     }
-    if (d.periodic == 0 || d.periodic == 2) {
-        const double4 w3 = d.wnx[3], w4 = d.wnx[4];
+    if (S.periodic == 0 || S.periodic == 2) {
+        const double4 w3 = d.wnx[S.wbase + 3], w4 = d.wnx[S.wbase + 4];
         double dw = w3.w - (xa.y + ra);
-        if (dw < 0.0) wf3 = contact_wall(d, acc, va, wa, ra, mk(w3.x, w3.y, w3.z), dw, d.wmvfd[3].y);
+        if (dw < 0.0) wf3 = contact_wall(S, acc, va, wa, ra, mk(w3.x, w3.y, w3.z), dw, d.wmvfd[S.wbase + 3].y);
         dw = xa.y - ra - w4.w;
-        if (dw < 0.0) wf4 = contact_wall(d, acc, va, wa, ra, mk(w4.x, w4.y, w4.z), dw, d.wmvfd[4].y);
+        if (dw < 0.0) wf4 = contact_wall(S, acc, va, wa, ra, mk(w4.x, w4.y, w4.z), dw, d.wmvfd[S.wbase + 4].y);
     }
 
     // ---- per-particle outputs (src/contactsearch.cuh:669-687)
-    const double dt = d.dt;
+    const double dt = S.dt;
     const double4 q0 = d.ro.pr0[prev], q1 = d.ro.pr1[prev], q2 = d.ro.pr2[prev];
     double4 ea = make_double4(q2.y, q2.z, q2.w, wa4.w);  // xysum.x, xysum.y, es, ev
     ea.z += acc.es_dot * dt;
     ea.w += acc.ev_dot * dt;
     d.rates[s] = make_double4(acc.es_dot, acc.ev_dot, acc.p, 0.0);
-    if (d.nw > 0) d.wall_pp[o] = wf0;
-    if (d.nw > 1) d.wall_pp[(size_t)d.np + o] = wf1;
-    if (d.nw > 2) d.wall_pp[2 * (size_t)d.np + o] = wf2;
-    if (d.nw > 3) d.wall_pp[3 * (size_t)d.np + o] = wf3;
-    if (d.nw > 4) d.wall_pp[4 * (size_t)d.np + o] = wf4;
+    if (S.nw > 0) d.wall_pp[o] = wf0;
+    if (S.nw > 1) d.wall_pp[(size_t)d.np + o] = wf1;
+    if (S.nw > 2) d.wall_pp[2 * (size_t)d.np + o] = wf2;
+    if (S.nw > 3) d.wall_pp[3 * (size_t)d.np + o] = wf3;
+    if (S.nw > 4) d.wall_pp[4 * (size_t)d.np + o] = wf4;
 
     // ---- bond forces in ascending bond index (src/cohesion.cuh:220-226)
     V3<double> force = acc.F, torque = acc.T;
@@ -1468,18 +1491,18 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
 
     // ---- leapfrog (src/integration.cuh:20-185)
     const double radius = ra;
-    const double m = 4.0 / 3.0 * PI_REF * radius * radius * radius * d.rho;
+    const double m = 4.0 / 3.0 * PI_REF * radius * radius * radius * S.rho;
     double ax = force.x / m, ay = force.y / m, az = force.z / m;
     double bx = torque.x * 1.0 / (2.0 / 5.0 * m * radius * radius);
     double by = torque.y * 1.0 / (2.0 / 5.0 * m * radius * radius);
     double bz = torque.z * 1.0 / (2.0 / 5.0 * m * radius * radius);
-    ax += d.g[0]; ay += d.g[1]; az += d.g[2];
+    ax += S.g[0]; ay += S.g[1]; az += S.g[2];
     if (va4.w > 0.0) {
-        ax = 0.0; ay = 0.0; az -= d.g[2];
+        ax = 0.0; ay = 0.0; az -= S.g[2];
         bx = 0.0; by = 0.0; bz = 0.0;
     }
     double px = xa.x, py = xa.y, pz = xa.z;
-    (void)wrap_xy(d, px, py);
+    (void)wrap_xy(S, px, py);
     double4 v0 = make_double4(q0.x, q0.y, q0.z, 0.0), w0 = make_double4(q0.w, q1.x, q1.y, 0.0);
     double4 ap = make_double4(q1.z, q1.w, q2.x, 0.0);
     v0.x += ax * dt; v0.y += ay * dt; v0.z += az * dt;
@@ -1504,7 +1527,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
 
     // cell id of the new position for the next step's sort
     uint32_t cerr = 0;
-    d.key[s] = cell_key(d, px, py, pz, cerr);
+    d.key[s] = cell_key(S, px, py, pz, cerr);
     if (cerr) atomicOr(d.flags_next, cerr);
 
     // neighbour lists (search mode 5): the list index travels with the
@@ -1515,20 +1538,24 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         const float4 xr = d.xref[li];
         float ddx = (float)px - xr.x, ddy = (float)py - xr.y;
         const float ddz = (float)pz - xr.z;
-        if (d.periodic == 1 || d.periodic == 2) ddx -= d.Lf[0] * rintf(ddx / d.Lf[0]);
-        if (d.periodic == 1) ddy -= d.Lf[1] * rintf(ddy / d.Lf[1]);
+        if (S.periodic == 1 || S.periodic == 2) ddx -= S.Lf[0] * rintf(ddx / S.Lf[0]);
+        if (S.periodic == 1) ddy -= S.Lf[1] * rintf(ddy / S.Lf[1]);
         if (ddx * ddx + ddy * ddy + ddz * ddz > d.list_lim * d.list_lim) atomicOr(&d.nstate[1], 1u);
     }
 }
 
 // ------------------------------------------------------------------- walls
 // src/integration.cuh:191-221: blocks of 256 original indices, tree halving.
+// In an ensemble each system's blocks start at its first index, so every
+// system is summed exactly as on its own.
+template <bool B>
 __global__ __launch_bounds__(BLOCK) void k_wallsum(DemDev d) {
     __shared__ double cache[BLOCK];
     const uint32_t w = blockIdx.y;
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    const Sys& S = B ? d.systems[d.sys_of_block[blockIdx.x]] : d.sys;
+    const uint32_t i = S.pbase + (blockIdx.x - S.bbase) * BLOCK + threadIdx.x;
     double temp = 0.0;
-    if (i < d.np) temp += d.wall_pp[(size_t)w * d.np + i];
+    if (i < S.pbase + S.np) temp += d.wall_pp[(size_t)w * d.np + i];
     cache[threadIdx.x] = temp;
     __syncthreads();
     for (uint32_t h = BLOCK / 2; h != 0; h >>= 1) {
@@ -1541,19 +1568,22 @@ __global__ __launch_bounds__(BLOCK) void k_wallsum(DemDev d) {
 // src/integration.cuh:224-295, one thread per wall; then time.current += dt.
 // The wall force is the serial sum of the block partials in block order; the
 // block stages the partials through LDS in chunks so the summing thread reads
-// LDS instead of waiting on HBM for every term.
+// LDS instead of waiting on HBM for every term.  One block per system.
 constexpr uint32_t WALL_CHUNK = 4096;
+template <bool B>
 __global__ __launch_bounds__(BLOCK) void k_walls(DemDev d) {
     __shared__ double part[WALL_CHUNK];
     __shared__ double fz[6];
     const uint32_t w = threadIdx.x;
-    const double t_current = *d.t_cur;
-    for (uint32_t wall = 0; wall < d.nw; ++wall) {
-        if (d.wmode[wall] == 0) continue;
-        const double* src = d.wall_partial + (size_t)wall * d.nwall_blocks;
+    const Sys& S = B ? d.systems[blockIdx.x] : d.sys;
+    const uint32_t k = B ? blockIdx.x : 0u;
+    const double t_current = d.t_cur[k];
+    for (uint32_t wall = 0; wall < S.nw; ++wall) {
+        if (d.wmode[S.wbase + wall] == 0) continue;
+        const double* src = d.wall_partial + (size_t)wall * d.nwall_blocks + S.bbase;
         double acc = 0.0;
-        for (uint32_t c0 = 0; c0 < d.nwall_blocks; c0 += WALL_CHUNK) {
-            const uint32_t cn = d.nwall_blocks - c0 < WALL_CHUNK ? d.nwall_blocks - c0 : WALL_CHUNK;
+        for (uint32_t c0 = 0; c0 < S.nblocks; c0 += WALL_CHUNK) {
+            const uint32_t cn = S.nblocks - c0 < WALL_CHUNK ? S.nblocks - c0 : WALL_CHUNK;
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < cn; i += BLOCK) part[i] = src[c0 + i];
             __syncthreads();
@@ -1571,25 +1601,25 @@ __global__ __launch_bounds__(BLOCK) void k_walls(DemDev d) {
         if (threadIdx.x == 0) fz[wall] = acc;
     }
     __syncthreads();
-    if (w < d.nw && d.wmode[w] != 0) {
-        const int wmode = d.wmode[w];
-        double4 w_nx = d.wnx[w], w_mvfd = d.wmvfd[w];
-        double vel0 = d.wvel0[w];
+    if (w < S.nw && d.wmode[S.wbase + w] != 0) {
+        const int wmode = d.wmode[S.wbase + w];
+        double4 w_nx = d.wnx[S.wbase + w], w_mvfd = d.wmvfd[S.wbase + w];
+        double vel0 = d.wvel0[S.wbase + w];
         w_mvfd.z = fz[w];
-        const double dt = d.dt;
-        const double sigma_0 = w_mvfd.w + d.devs_A * sin(2.0 * PI_WALL * d.devs_f * t_current);
-        const double N = -sigma_0 * d.L[0] * d.L[1];
+        const double dt = S.dt;
+        const double sigma_0 = w_mvfd.w + S.devs_A * sin(2.0 * PI_WALL * S.devs_f * t_current);
+        const double N = -sigma_0 * S.L[0] * S.L[1];
         double acc = (w_mvfd.z + N) / w_mvfd.x;
         if (wmode == 2) acc = 0.0;
         vel0 += acc * dt;
         w_nx.w += vel0 * dt;
         w_mvfd.y = vel0 + 0.5 * acc * dt;
-        d.wnx[w] = w_nx;
-        d.wmvfd[w] = w_mvfd;
-        d.wvel0[w] = vel0;
+        d.wnx[S.wbase + w] = w_nx;
+        d.wmvfd[S.wbase + w] = w_mvfd;
+        d.wvel0[S.wbase + w] = vel0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) *d.t_cur = t_current + d.dt;
+    if (threadIdx.x == 0) d.t_cur[k] = t_current + S.dt;
 }
 
 // Plain streaming copy used to calibrate bandwidth counters.
@@ -1609,7 +1639,10 @@ void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
 void launch_bonds(const DemDev& d, hipStream_t s) {
     if (d.nb0) k_bonds<<<nblk(d.nb0), BLOCK, 0, s>>>(d);
 }
-void launch_cellid(const DemDev& d, hipStream_t s) { k_cellid<<<nblk(d.np), BLOCK, 0, s>>>(d); }
+void launch_cellid(const DemDev& d, hipStream_t s) {
+    if (d.systems) k_cellid<true><<<nblk(d.np), BLOCK, 0, s>>>(d);
+    else k_cellid<false><<<nblk(d.np), BLOCK, 0, s>>>(d);
+}
 // (the counts start at zero: k_scan_down clears them after use)
 void launch_count(const DemDev& d, hipStream_t s) { k_count<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_scan(const DemDev& d, hipStream_t s) {
@@ -1619,21 +1652,36 @@ void launch_scan(const DemDev& d, hipStream_t s) {
 }
 void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_cellsort(const DemDev& d, hipStream_t s) { k_cellsort<<<nblk(d.K), BLOCK, 0, s>>>(d); }
-void launch_reorder(const DemDev& d, hipStream_t s) { k_reorder<<<nblk(d.np), BLOCK, 0, s>>>(d); }
-void launch_topology(const DemDev& d, hipStream_t s) {
+void launch_reorder(const DemDev& d, hipStream_t s) {
+    if (d.systems) k_reorder<true><<<nblk(d.np), BLOCK, 0, s>>>(d);
+    else k_reorder<false><<<nblk(d.np), BLOCK, 0, s>>>(d);
+}
+template <bool B>
+void topology(const DemDev& d, hipStream_t s) {
     if (d.search_mode == 5) {  // one of the two returns at once (nstate[0])
-        k_topo_rows<true><<<nblk(d.np), BLOCK, 0, s>>>(d);
-        k_list_search<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
+        k_topo_rows<true, B><<<nblk(d.np, d.tpb_search), d.tpb_search, NC * 4 * d.tpb_search, s>>>(d);
+        k_list_search<B><<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
     } else if (d.search_mode == 4) {  // hcnt starts at zero: k_topo_apply clears it
-        k_topo_half<<<nblk(d.np), BLOCK, 0, s>>>(d);
+        k_topo_half<B><<<nblk(d.np, d.tpb_search), d.tpb_search, 0, s>>>(d);
         k_topo_apply<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
-    } else if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
-    else if (d.search_mode == 2) k_topo_rows<false><<<nblk(d.np), BLOCK, 0, s>>>(d);
-    else k_topo<<<d.tiles_x * d.tiles_y * d.num[2], BLOCK, 0, s>>>(d);
+    } else if (d.search_mode == 2) k_topo_rows<false, B><<<nblk(d.np, d.tpb_search), d.tpb_search, NC * 4 * d.tpb_search, s>>>(d);
+    else if (B) return;  // (modes 1 and 3 walk one grid; the host refuses them for ensembles)
+    else if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
+    else k_topo<<<d.tiles_x * d.tiles_y * d.sys.num[2], BLOCK, 0, s>>>(d);
+}
+void launch_topology(const DemDev& d, hipStream_t s) {
+    if (d.systems) topology<true>(d, s);
+    else topology<false>(d, s);
 }
 void launch_forces(const DemDev& d, int precision, hipStream_t s) {
-    if (precision == SPH_PREC_MIXED) k_force<float><<<nblk(d.np), BLOCK, 0, s>>>(d);
-    else k_force<double><<<nblk(d.np), BLOCK, 0, s>>>(d);
+    const bool mixed = precision == SPH_PREC_MIXED;
+    if (d.systems) {
+        if (mixed) k_force<float, true><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+        else k_force<double, true><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+    } else {
+        if (mixed) k_force<float, false><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+        else k_force<double, false><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+    }
 }
 void launch_contact(const DemDev& d, int precision, hipStream_t s) {
     launch_topology(d, s);
@@ -1643,8 +1691,13 @@ void launch_copy_probe(const double4* src, double4* dst, uint64_t n, hipStream_t
     k_copy_probe<<<2048, BLOCK, 0, s>>>(src, dst, n);
 }
 void launch_walls(const DemDev& d, hipStream_t s) {
-    if (d.nw) k_wallsum<<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);
-    k_walls<<<1, BLOCK, 0, s>>>(d);
+    if (d.systems) {
+        if (d.nw) k_wallsum<true><<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);
+        k_walls<true><<<d.nsys, BLOCK, 0, s>>>(d);
+    } else {
+        if (d.nw) k_wallsum<false><<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);
+        k_walls<false><<<1, BLOCK, 0, s>>>(d);
+    }
 }
 
 }  // namespace sph

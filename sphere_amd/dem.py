@@ -110,6 +110,12 @@ class SphTiming(C.Structure):
                 ("search_mode", C.c_int32), ("search_auto", C.c_int32)]
 
 
+class SphEnsInfo(C.Structure):
+    _fields_ = [("members", C.c_uint32), ("np", C.c_uint64), ("ncells", C.c_uint64), ("steps_done", C.c_uint64),
+                ("cell_factor", C.c_double), ("tuned_ms", C.c_double * 4), ("search_mode", C.c_int32),
+                ("force_block", C.c_uint32), ("search_block", C.c_uint32), ("precision", C.c_int32)]
+
+
 EXPORTS = {
     "sph_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "sph_open_mem": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
@@ -138,6 +144,22 @@ EXPORTS = {
                                C.c_void_p]),
     "sph_adjoint_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
     "sph_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "sph_ens_open_mem": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_uint32, C.c_double, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    "sph_ens_open": (C.c_int, [C.POINTER(C.c_char_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
+    "sph_ens_close": (None, [C.c_void_p]),
+    "sph_ens_last_error": (C.c_char_p, [C.c_void_p]),
+    "sph_ens_step": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "sph_ens_field_bytes": (C.c_size_t, [C.c_void_p, C.c_uint32, C.c_int]),
+    "sph_ens_get": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t]),
+    "sph_ens_member_info": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(SphInfo)]),
+    "sph_ens_info": (C.c_int, [C.c_void_p, C.POINTER(SphEnsInfo)]),
+    "sph_ens_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_ens_set_search": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_ens_set_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "sph_ens_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_ens_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_ens_get_timing": (C.c_int, [C.c_void_p, C.POINTER(SphTiming)]),
 }
 
 _lib = None
@@ -333,6 +355,105 @@ class DEM:
             sb = Spherebin.readbin(p)
         sb.sid = self.sid
         return sb
+
+
+class Ensemble:
+    """Many independent systems stepped together (sph_ens_*; no counterpart
+    in the reference, which runs one DEM per process).  ``members`` are
+    :class:`Spherebin` objects or ``.bin`` paths; ``cell_factor`` < 0 keeps
+    their grids, >= 1 regrids every member to cells of cell_factor x 2 r_max,
+    0 lets the autotuner choose.  Member k evolves as ``DEM(members[k])``
+    would on the same grid."""
+
+    def __init__(self, members, cell_factor: float = -1.0, verbosity: int = 0):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        n = len(members)
+        if all(isinstance(m, Spherebin) for m in members):
+            data = [m.to_bytes() for m in members]
+            self._bufs = [C.create_string_buffer(b, len(b)) for b in data]
+            ptrs = (C.c_void_p * n)(*[C.cast(b, C.c_void_p) for b in self._bufs])
+            sizes = (C.c_size_t * n)(*[len(b) for b in data])
+            rc = self._lib.sph_ens_open_mem(ptrs, sizes, n, float(cell_factor), int(verbosity), C.byref(self._h))
+            self._bufs = None
+        else:
+            paths = (C.c_char_p * n)(*[os.fspath(m).encode() for m in members])
+            rc = self._lib.sph_ens_open(paths, n, float(cell_factor), int(verbosity), C.byref(self._h))
+        if rc != SPH_OK:
+            _raise(rc, self._lib.sph_ens_last_error(None).decode())
+        self.members = n
+
+    def _check(self, rc: int):
+        if rc != SPH_OK:
+            _raise(rc, self._lib.sph_ens_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.sph_ens_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, n: int = 1):
+        self._check(self._lib.sph_ens_step(self._h, int(n)))
+
+    def info(self) -> SphEnsInfo:
+        out = SphEnsInfo()
+        self._check(self._lib.sph_ens_info(self._h, C.byref(out)))
+        return out
+
+    def member_info(self, k: int) -> SphInfo:
+        out = SphInfo()
+        self._check(self._lib.sph_ens_member_info(self._h, int(k), C.byref(out)))
+        return out
+
+    def get(self, k: int, name: str) -> np.ndarray:
+        i = self.member_info(k)
+        n, nw = int(i.np), int(i.nw)
+        shapes = {
+            "x": (n, 4), "vel": (n, 4), "angvel": (n, 3), "force": (n, 3), "torque": (n, 3),
+            "angpos": (n, 3), "vel0": (n, 3), "angvel0": (n, 3), "xysum": (n, 2), "es_dot": (n,),
+            "es": (n,), "ev_dot": (n,), "ev": (n,), "p": (n,), "contacts": (n, NC), "delta_t": (n, NC, 3),
+            "walls_nx": (nw, 4), "walls_mvfd": (nw, 4), "walls_vel0": (nw,), "wall_pp": (nw, n), "time": (4,),
+        }
+        if name not in shapes:
+            raise Unsupported(f"field {name} is not available per ensemble member")
+        out = np.empty(shapes[name], dtype=np.uint32 if name in _U32_FIELDS else np.float64)
+        self._check(self._lib.sph_ens_get(self._h, int(k), FIELDS[name], out.ctypes.data_as(C.c_void_p),
+                                          out.nbytes))
+        return out
+
+    def set_precision(self, prec):
+        if isinstance(prec, str):
+            prec = {"fp64": PREC_FP64, "mixed": PREC_MIXED}[prec]
+        self._check(self._lib.sph_ens_set_precision(self._h, int(prec)))
+
+    def set_search(self, mode: int = 0):
+        self._check(self._lib.sph_ens_set_search(self._h, int(mode)))
+
+    def set_blocks(self, force_block: int, search_block: int):
+        self._check(self._lib.sph_ens_set_blocks(self._h, int(force_block), int(search_block)))
+
+    def set_graph(self, on: bool = True):
+        self._check(self._lib.sph_ens_set_graph(self._h, 1 if on else 0))
+
+    def set_profiling(self, on: bool = True):
+        self._check(self._lib.sph_ens_set_profiling(self._h, 1 if on else 0))
+
+    def timing(self) -> Dict[str, float]:
+        t = SphTiming()
+        self._check(self._lib.sph_ens_get_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in SphTiming._fields_}
 
 
 def pack_rsa(radii, box, periodic: int, seed: int, max_tries: int = 100000) -> np.ndarray:

@@ -153,9 +153,25 @@ def test_bond_pair_parity(tmp_path):
 
 
 def test_bonded_lattice_parity(tmp_path):
+    """Bond strengths low enough that bonds break from about step 100 on
+    (32 of 620 broken by step 300 in the oracle): breakage indices and the
+    surviving bonds' state bit for bit."""
     _say("bonded fcc")
-    sb = configs.fcc_bonded(cells=(3, 3, 4), sigma_b=2e7, tau_b=2e7)
-    errs = run_pair(sb, tmp_path, [1, 50, 300])
+    sb = configs.fcc_bonded(cells=(3, 3, 4), sigma_b=3e4, tau_b=3e4)
+    errs = run_pair(sb, tmp_path, [1, 50, 150, 150])
+    assert not errs, errs
+
+
+def test_bonded_solid_full_size(tmp_path):
+    """The 262,144-sphere bonded FCC solid (about 1.5M bonds) at full size,
+    with strengths that break bonds within the first steps, against the
+    oracle for 25 steps."""
+    from oracle_lib import Oracle
+
+    _say("bonded solid 262,144 spheres")
+    sb = configs.fcc_bonded(cells=(32, 32, 64), sigma_b=2e3, tau_b=2e3)
+    assert sb.np == 262144 and sb.nb0 > 1_400_000
+    errs = run_pair(sb, tmp_path, [1, 9, 15])
     assert not errs, errs
 
 
