@@ -121,19 +121,25 @@ def random_compression(n=1048576, box=(2.07, 2.07, 4.13), periodic=1, stress=10e
 
 
 def ensemble_member(n=4096, k_n=1e6, mu=0.5, stress=10e3, gravity=-9.81, steps=1000, packing=0.3,
-                    seed=BASE_SEED + 4, sid="c4-member") -> Spherebin:
+                    gamma=None, seed=BASE_SEED + 4, sid="c4-member") -> Spherebin:
     """One packing of the calibration ensemble: ``n`` lognormal spheres (as
     :func:`random_compression`) placed at random without overlap in an
     x/y-periodic box of aspect 1:1:2 sized for the packing fraction, settling
     under gravity below a top wall held at ``stress`` (10 kPa); contact
     stiffness ``k_n`` (= k_t) and friction ``mu`` per member, dt = 0.17
-    sqrt(m_min / k_n)."""
+    sqrt(m_min / k_n).  The dashpots default to 100 sqrt(k_n / 1e6) N s/m:
+    the damping ratio of the 1e6 N/m, 100 N s/m contacts of the other
+    workloads at every stiffness.  (A fixed 100 N s/m on the softest members,
+    k_n = 1e5, gives gamma dt / m_min = 1.5; the compressed top layer then
+    blows apart within about 1,100 steps, in the oracle as on the GPU.)"""
+    if gamma is None:
+        gamma = 100.0 * math.sqrt(k_n / 1e6)
     from .dem import pack_rsa
 
     rng = np.random.default_rng(seed)
     sb = Spherebin(n, nw=1, sid=sid)
     sb.radius = lognormal_radii(rng, n)
-    _contact_params(sb, k_n=k_n, k_t=k_n, mu=mu)
+    _contact_params(sb, k_n=k_n, k_t=k_n, gamma_n=gamma, gamma_t=gamma, mu=mu)
     solid = float(np.sum(4.0 / 3.0 * math.pi * sb.radius ** 3))
     side = (solid / packing / 2.0) ** (1.0 / 3.0)
     r_max = float(np.max(sb.radius))

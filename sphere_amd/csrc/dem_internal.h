@@ -136,6 +136,7 @@ struct DemDev {
     uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed
     float list_skin;    // mode 5: list reach beyond r_a + r_b (m)
     float list_lim;     // mode 5: displacement since the build that requests a rebuild (m)
+    uint32_t list_shell;  // mode 5: list builds also scan the cells two away
     uint32_t* hcnt;     // half-shell search: overlaps found per particle
     uint64_t* hits;     // np * NC keys: offset index << 40 | image << 32 | sorted position
     uint32_t* tmask;    // slot occupancy after the search, sorted order

@@ -100,21 +100,30 @@ void update_search_eps(sph_dem* h) {
     h->d.search_eps = (float)(16.0 * ulp + 2e-6 * rmax);
     // neighbour-list reach beyond r_a + r_b: a tenth of the smallest cell
     // (at least 2 r_max / 10), so lists hold the touching and nearly touching
-    // spheres.  A rebuild is due once a sphere has moved 0.45 skin, or half
-    // the room the cell leaves beyond the largest pair, less the prefilter
-    // margin for FP32 rounding: two spheres more than a cell apart at the
-    // build (outside its 27-cell scan) then cannot meet before the next one.
-    // A cell no wider than the largest pair gives 0: rebuild on any motion.
-    // (an ensemble takes the smallest cell and the least room of its members)
+    // spheres; a rebuild is due once a sphere has moved 0.45 skin.  Two
+    // spheres more than a cell apart at the build (outside its 27-cell scan)
+    // can then meet before the next one if the cell leaves less room than
+    // 0.9 skin beyond the largest pair: the build then also scans the cells
+    // two away (list_shell), or, where a periodic axis has fewer than 5 cells,
+    // the rebuild distance shrinks to half that room less the prefilter
+    // margin (0 for a cell no wider than the largest pair: rebuild on any
+    // motion).  (An ensemble takes the smallest cell and least room of its
+    // members.)
     double cmin = HUGE_VAL, room = HUGE_VAL;
+    bool shell_ok = true;
     for (const HostState* m : systems_of(h)) {
         double c = m->L[0] / m->num[0];
         for (int a = 1; a < 3; ++a) c = std::min(c, m->L[a] / m->num[a]);
         cmin = std::min(cmin, c);
         room = std::min(room, c - 2.0 * (h->members.empty() ? rmax : max_radius(*m)));
+        if ((m->periodic == 1 || m->periodic == 2) && m->num[0] < 5) shell_ok = false;
+        if (m->periodic == 1 && m->num[1] < 5) shell_ok = false;
     }
     h->d.list_skin = (float)(0.1 * cmin);
-    const double lim = std::min(0.45 * (double)h->d.list_skin, 0.5 * room - (double)h->d.search_eps);
+    const double eps = (double)h->d.search_eps, skin = (double)h->d.list_skin;
+    const bool tight = room - 2.0 * eps < 0.9 * skin;
+    h->d.list_shell = tight && shell_ok ? 1u : 0u;
+    const double lim = tight && !shell_ok ? std::min(0.45 * skin, 0.5 * room - eps) : 0.45 * skin;
     h->d.list_lim = lim > 0.0 ? (float)lim : 0.0f;
 }
 

@@ -786,6 +786,53 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
     }
 }
 
+// Second pass of a list build: the spheres two cells away (the outer shell of
+// the 5 x 5 x 5 cells around the particle's cell) within the list reach.  A
+// pair two cells apart is more than a cell apart; when a cell is narrower
+// than the largest pair plus 0.9 skin such a pair can meet before the next
+// rebuild, so the host asks for this pass (list_shell) instead of shrinking
+// the rebuild distance to fit the 27 cells.  It does so only when every
+// periodic axis has at least 5 cells: no cell is then reached twice.
+__device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const float4* __restrict__ xf,
+                                        uint32_t* list, uint32_t np, uint32_t& lcnt, int gx, int gy, int gz, int nx,
+                                        int ny, int nz, int periodic, float Lx, float Ly, uint32_t cbase,
+                                        float4 xaf, float ra_skin) {
+    for (int dz = -2; dz <= 2; ++dz) {
+        const int tz = gz + dz;
+        if (tz < 0 || tz >= nz) continue;
+        for (int dy = -2; dy <= 2; ++dy) {
+            int ty = gy + dy;
+            float dmy = 0.0f;
+            if (ty < 0 || ty >= ny) {
+                if (periodic != 1) continue;
+                dmy = ty < 0 ? Ly : -Ly;
+                ty = ty < 0 ? ty + ny : ty - ny;
+            }
+            for (int dx = -2; dx <= 2; ++dx) {
+                if (dx > -2 && dx < 2 && dy > -2 && dy < 2 && dz > -2 && dz < 2) continue;  // the 27 cells
+                int tx = gx + dx;
+                float dmx = 0.0f;
+                if (tx < 0 || tx >= nx) {
+                    if (periodic == 0) continue;
+                    dmx = tx < 0 ? Lx : -Lx;
+                    tx = tx < 0 ? tx + nx : tx - nx;
+                }
+                const uint32_t c = cbase + (uint32_t)tx + (uint32_t)ty * (uint32_t)nx + (uint32_t)tz * (uint32_t)(nx * ny);
+                const uint32_t hi = off[c + 1];
+                for (uint32_t sb = off[c]; sb < hi; ++sb) {
+                    const float4 cb = xf[sb];
+                    const float ex = (xaf.x - cb.x) + dmx, ey = (xaf.y - cb.y) + dmy, ez = xaf.z - cb.z;
+                    const float rl = ra_skin + cb.w;
+                    if (ex * ex + ey * ey + ez * ez <= rl * rl) {
+                        if (lcnt < (uint32_t)NLIST) list[(size_t)lcnt * np] = sb;
+                        ++lcnt;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // BUILD (search mode 5): runs only on steps whose neighbour lists must be
 // rebuilt, and then also rebuilds the lists.
 template <bool BUILD, bool B>
@@ -852,6 +899,9 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     d.tfresh[s] = st.fresh;
     if (st.err) atomicOr(d.flags, st.err);
     if constexpr (BUILD) {  // list index = this sorted position
+        if (d.list_shell && !st.stop)
+            list_shell(d.off, d.xf, list, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0], S.Lf[1], S.cbase,
+                       xaf, ra_skin);
         d.ncnt[s] = lcnt < (uint32_t)NLIST ? lcnt : (uint32_t)NLIST;
         d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
         if (lcnt > (uint32_t)NLIST || st.stop) atomicOr(&d.nstate[2], 1u);  // list unusable: rebuild next step
@@ -1055,11 +1105,12 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
 // the build; the force kernel requests a rebuild once any sphere has moved
 // more than list_lim since.  list_lim is at most 0.45 skin, so no pair
 // outside the reach can overlap before the rebuild (both spheres would have
-// had to close a gap of more than skin), and at most half the room a cell
-// leaves beyond the largest pair (cell - 2 r_max), so a pair that the 27-cell
-// scan of the build did not see (more than a cell apart) cannot overlap
-// either.  Testing the list pairs then finds exactly the overlaps the full
-// search finds.
+// had to close a gap of more than skin).  A pair more than a cell apart at the
+// build cannot overlap either: the build also scans the cells two away
+// (list_shell) when a cell leaves less room than 0.9 skin beyond the largest
+// pair, and where it cannot (a periodic axis of fewer than 5 cells) list_lim
+// is at most half that room.  Testing the list pairs then finds exactly the
+// overlaps the full search finds.
 // The overlaps are keyed by the reference's visiting order -- the offset of
 // the partner's cell (periodic wrap included, as findDistMod resolves it,
 // src/contactsearch.cuh:12-74) and its sorted position -- and replayed

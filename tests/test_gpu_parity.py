@@ -51,22 +51,28 @@ def test_c0_settle_parity(tmp_path, search):
     assert not errs, errs
 
 
-def test_lists_cover_pairs_beyond_the_build_scan(tmp_path):
+@pytest.mark.parametrize("periodic,L", [(0, 20.0), (1, 4.22)], ids=["shell-pass", "small-periodic-grid"])
+def test_lists_cover_pairs_beyond_the_build_scan(tmp_path, periodic, L):
     """Neighbour lists when a cell is barely wider than the largest pair: two
-    spheres two cells apart at the list build (outside its 27-cell scan) but
-    within r_a + r_b + 0.9 skin close in on each other; the lists must be
-    rebuilt before they can overlap."""
-    _say("lists: pair two cells apart at the build")
-    sb = configs.two_spheres(k=1e7, periodic=0, sid="list-gap")
-    cell = sb.L[0] / sb.num[0]
+    spheres two cells apart along z at the list build (outside its 27-cell
+    scan) but within r_a + r_b + 0.9 skin close in on each other.  The build
+    must list them (its pass over the cells two away), or, on a periodic axis
+    of 4 cells where that pass would visit a cell twice, rebuild before they
+    can overlap."""
+    _say(f"lists: pair two cells apart at the build, periodic={periodic}")
+    sb = configs.two_spheres(k=1e7, periodic=periodic, sid=f"list-gap-{periodic}")
+    sb.L = np.array([L, L, 20.0])
+    sb.num = np.maximum(1, np.floor(sb.L / (2.0 * 0.5 * 1.0001))).astype(np.uint32)
+    assert periodic == 0 or sb.num[0] == 4
+    cell = sb.L[2] / sb.num[2]
     # a at the top of its cell, b 1.08 (r_a + r_b = 1, skin = cell / 10) away,
     # in the cell after next; they approach at 10 m/s each
-    xa = (np.floor(10.0 / cell) + 1.0) * cell - 0.02
-    sb.x[0] = (xa, 10.0, 10.0)
-    sb.x[1] = (xa + 1.08, 10.0, 10.0)
-    assert int(sb.x[1, 0] // cell) - int(sb.x[0, 0] // cell) == 2
-    sb.vel[0] = (10.0, 0.0, 0.0)
-    sb.vel[1] = (-10.0, 0.0, 0.0)
+    za = (np.floor(10.0 / cell) + 1.0) * cell - 0.02
+    sb.x[0] = (0.5 * L, 0.5 * L, za)
+    sb.x[1] = (0.5 * L, 0.5 * L, za + 1.08)
+    assert int(sb.x[1, 2] // cell) - int(sb.x[0, 2] // cell) == 2
+    sb.vel[0] = (0.0, 0.0, 10.0)
+    sb.vel[1] = (0.0, 0.0, -10.0)
     steps = [1] + [5] * 20
     errs = run_pair(sb, tmp_path, steps, setup=lambda d: d.set_search(5))
     assert not errs, errs
