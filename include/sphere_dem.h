@@ -143,6 +143,14 @@ int sph_run(sph_t* h, const char* outdir, const char* sid);
 /* DEM::writebin (src/file_io.cpp:304-490). */
 int sph_writebin(sph_t* h, const char* path);
 
+/* DEM::readbin (src/file_io.cpp:29-301) on an open handle: replaces the
+ * simulation with the file's (host state, device buffers, step count),
+ * keeping the handle's settings (precision, search kernel, graph capture,
+ * profiling, verbosity) and applying checkValues as the handle was opened
+ * with.  The file is parsed and checked before anything is replaced: on
+ * SPH_E_IO, SPH_E_INVALID or SPH_E_UNSUPPORTED the handle is unchanged. */
+int sph_readbin(sph_t* h, const char* path);
+
 /* DEM::checkValues (src/sphere.cpp:104-246) on the current device state. */
 int sph_check_values(sph_t* h);
 

@@ -56,13 +56,9 @@ public:
     DEM(const DEM&) = delete;
     DEM& operator=(const DEM&) = delete;
 
-    // DEM::readbin (src/file_io.cpp:29-301): replaces the simulation.
-    void readbin(const char* target) {
-        sph_close(h_);
-        h_ = nullptr;
-        const int rc = sph_open(target, verbose_, 1, &h_);
-        if (rc) throw Error(rc, sph_last_error(nullptr));
-    }
+    // DEM::readbin (src/file_io.cpp:29-301): replaces the simulation, keeping
+    // the handle's settings; on a read or check error the state is unchanged.
+    void readbin(const char* target) { check(sph_readbin(handle(), target)); }
     // DEM::writebin (src/file_io.cpp:304-490)
     void writebin(const char* target) { check(sph_writebin(handle(), target)); }
     // DEM::checkValues (src/sphere.cpp:104-246)

@@ -590,7 +590,7 @@ std::vector<Region> state_regions(sph_dem* h) {
     r.push_back({d.flags, 8});
     r.push_back({d.force, n * 32});
     r.push_back({d.torque, n * 32});
-    r.push_back({d.rates, n * 32});
+    r.push_back({d.pres, n * 8});
     if (h->hs.nw) r.push_back({d.wall_pp, (size_t)h->hs.nw * n * 8});
     return r;
 }

@@ -109,8 +109,9 @@ struct DemDev {
     uint32_t* pos_of;   // original index -> sorted position of this step
 
     // per-step outputs, sorted order of the step
-    double4 *force, *torque;
-    double4 *rates;     // es_dot, ev_dot, p, 0
+    double4 *force;     // fx, fy, fz, es_dot
+    double4 *torque;    // tx, ty, tz, ev_dot
+    double* pres;       // p
 
     // contact history, keyed by original index
     uint32_t* cslot;    // np * NC

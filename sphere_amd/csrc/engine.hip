@@ -320,6 +320,8 @@ int upload_all(sph_dem* h) {
             w4[4 * i + a] = s.angvel[3 * i + a];
             f4[4 * i + a] = s.force[3 * i + a];
             t4[4 * i + a] = s.torque[3 * i + a];
+            f4[4 * i + 3] = s.es_dot[i];
+            t4[4 * i + 3] = s.ev_dot[i];
             r0[4 * i + a] = s.vel[4 * i + a];
         }
         w4[4 * i + 3] = s.ev[i];
@@ -335,12 +337,6 @@ int upload_all(sph_dem* h) {
     }
     std::vector<uint32_t> iota(n);
     for (size_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
-    std::vector<double> rates(4 * n, 0.0);
-    for (size_t i = 0; i < n; ++i) {
-        rates[4 * i + 0] = s.es_dot[i];
-        rates[4 * i + 1] = s.ev_dot[i];
-        rates[4 * i + 2] = s.p[i];
-    }
     HIP_TRY(hipMemcpy(own(h).x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).vel, s.vel.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(own(h).angvel, w4.data(), n * 32, hipMemcpyHostToDevice));
@@ -355,7 +351,7 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.a.cmask, 0, n * 4));
     HIP_TRY(hipMemcpy(d.force, f4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.torque, t4.data(), n * 32, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.rates, rates.data(), n * 32, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.pres, s.p.data(), n * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.cslot, 0, n * sph::NC * 4));
     HIP_TRY(hipMemset(d.cdt, 0, n * sph::NC * 3 * 8));
     HIP_TRY(hipMemset(d.flags, 0, 8));
@@ -481,7 +477,7 @@ int allocate(sph_dem* h) {
     int rc;
     if ((rc = alloc_pstate(h, d.a, n)) || (rc = alloc_pstate(h, d.b, n))) return rc;
     ALLOC(d.pos_of, n);
-    ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.rates, n);
+    ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.pres, n);
     ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
     d.cdt32 = nullptr;
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
@@ -638,11 +634,13 @@ int download_all(sph_dem* h) {
     };
     if ((rc = fetch4(h, d.force, id, t))) return rc;
     unpad(s.force);
+    for (size_t i = 0; i < n; ++i) s.es_dot[i] = t[4 * i + 3];
     if ((rc = fetch4(h, own(h).angvel, id, t))) return rc;
     unpad(s.angvel);
     for (size_t i = 0; i < n; ++i) s.ev[i] = t[4 * i + 3];
     if ((rc = fetch4(h, d.torque, id, t))) return rc;
     unpad(s.torque);
+    for (size_t i = 0; i < n; ++i) s.ev_dot[i] = t[4 * i + 3];
     std::vector<double> t1, t2;
     if ((rc = fetch4(h, own(h).pr1, id, t1)) || (rc = fetch4(h, own(h).pr2, id, t2))) return rc;
     for (size_t i = 0; i < n; ++i) {
@@ -652,8 +650,11 @@ int download_all(sph_dem* h) {
         s.xysum[2 * i] = t2[4 * i + 1]; s.xysum[2 * i + 1] = t2[4 * i + 2];
         s.es[i] = t2[4 * i + 3];
     }
-    if ((rc = fetch4(h, d.rates, id, t))) return rc;
-    for (size_t i = 0; i < n; ++i) { s.es_dot[i] = t[4 * i]; s.ev_dot[i] = t[4 * i + 1]; s.p[i] = t[4 * i + 2]; }
+    {
+        std::vector<double> pv(n);
+        HIP_TRY(hipMemcpy(pv.data(), d.pres, n * 8, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q) s.p[id[q]] = pv[q];
+    }
     if (s.nw) {
         double wnx[24], wmv[24];
         HIP_TRY(hipMemcpy(wnx, d.wnx, sizeof(wnx), hipMemcpyDeviceToHost));
@@ -681,6 +682,7 @@ int download_all(sph_dem* h) {
 
 int open_common(sph_dem* h, int verbosity, int check_vals) {
     h->verbosity = verbosity;
+    h->check_vals = check_vals;
     int rc;
     if (check_vals == 1 && (rc = sph::check_values(h->hs, h->err))) return rc;
     if ((rc = validate_for_device(h))) return rc;
@@ -933,6 +935,51 @@ int sph_open(const char* path, int verbosity, int check_vals, sph_t** out) {
     return finish_open(h, rc, out);
 }
 
+int sph_readbin(sph_t* h, const char* path) {
+    if (!h || !path) return SPH_E_INVALID;
+    if (!h->members.empty()) {
+        h->err = "sph_readbin: not on an ensemble";
+        return SPH_E_STATE;
+    }
+    // parse and check on a scratch handle first: errors leave h unchanged
+    auto* t = new sph_dem();
+    int rc = sph::read_bin_file(path, t->hs, t->err);
+    if (!rc && h->check_vals == 1) rc = sph::check_values(t->hs, t->err);
+    if (!rc) rc = validate_for_device(t);
+    if (rc) {
+        h->err = t->err;
+        delete t;
+        return rc;
+    }
+    const int precision = h->precision, profiling = h->profiling, verbosity = h->verbosity, vals = h->check_vals;
+    const bool use_graph = h->use_graph, search_auto = h->search_auto;
+    const int search_mode = h->d.search_mode;
+    const uint32_t tpb_force = h->d.tpb_force, tpb_search = h->d.tpb_search;
+    h->adj.reset();
+    free_all(h);
+    h->events_made = false;
+    h->hs = std::move(t->hs);
+    delete t;
+    h->d = DemDev{};
+    h->d.search_mode = search_mode;
+    h->d.tpb_force = tpb_force;
+    h->d.tpb_search = tpb_search;
+    h->own_cur = 0;
+    h->precision = SPH_PREC_FP64;
+    h->steps_done = 0;
+    h->tuned = false;
+    h->steps_since_tune = 0;
+    h->timing = sph_timing_t{};
+    rc = open_common(h, verbosity, vals);
+    if (rc) return rc;
+    h->use_graph = use_graph;
+    h->search_auto = search_auto;
+    h->profiling = profiling;
+    if (search_mode == 5) request_list_rebuild(h);
+    if (precision != SPH_PREC_FP64) rc = sph_set_precision(h, precision);
+    return rc;
+}
+
 void sph_close(sph_t* h) {
     if (!h) return;
     free_all(h);
@@ -1087,9 +1134,15 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_TORQUE: return gather(d.torque, 0, 3);
         case SPH_ANGPOS: case SPH_VEL0: case SPH_ANGVEL0: case SPH_XYSUM: case SPH_ES: case SPH_EV:
             return get_comps(h, packed_comps(own(h), f), dd);
-        case SPH_ES_DOT: return gather(d.rates, 0, 1);
-        case SPH_EV_DOT: return gather(d.rates, 1, 1);
-        case SPH_P: return gather(d.rates, 2, 1);
+        case SPH_ES_DOT: return gather(d.force, 3, 1);
+        case SPH_EV_DOT: return gather(d.torque, 3, 1);
+        case SPH_P: {
+            std::vector<double> pv(n);
+            if ((rc = fetch_ids(h, id))) return rc;
+            HIP_TRY(hipMemcpy(pv.data(), d.pres, n * 8, hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < n; ++q) dd[id[q]] = pv[q];
+            return SPH_OK;
+        }
         case SPH_SORT_IDX: return copy(d.sid);
         case SPH_CELL_KEYS: case SPH_CELL_START: case SPH_CELL_END: {
             std::vector<uint32_t> off(K + 1);
@@ -1209,9 +1262,17 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
         case SPH_TORQUE: return scatter(d.torque, 0, 3);
         case SPH_ANGPOS: case SPH_VEL0: case SPH_ANGVEL0: case SPH_XYSUM: case SPH_ES: case SPH_EV:
             return set_comps(h, packed_comps(own(h), f), sd);
-        case SPH_ES_DOT: return scatter(d.rates, 0, 1);
-        case SPH_EV_DOT: return scatter(d.rates, 1, 1);
-        case SPH_P: return scatter(d.rates, 2, 1);
+        case SPH_ES_DOT: return scatter(d.force, 3, 1);
+        case SPH_EV_DOT: return scatter(d.torque, 3, 1);
+        case SPH_P: {
+            std::vector<uint32_t> id;
+            int r;
+            if ((r = fetch_ids(h, id))) return r;
+            std::vector<double> pv(n);
+            for (size_t q = 0; q < n; ++q) pv[q] = sd[id[q]];
+            HIP_TRY(hipMemcpy(d.pres, pv.data(), n * 8, hipMemcpyHostToDevice));
+            return SPH_OK;
+        }
         case SPH_BONDS: return copy(d.bonds);
         case SPH_BONDS_DELTA: return copy(d.bdelta);
         case SPH_BONDS_OMEGA: return copy(d.bomega);

@@ -52,6 +52,7 @@ struct sph_dem {
     std::vector<sph::HostState> members;
     std::vector<sph::Sys> sys_host;
     bool blocks_auto = false;  // tune threads per block with the search kernel
+    int check_vals = 1;        // as sph_open was called (sph_readbin applies it again)
 };
 
 namespace sph {

@@ -1517,7 +1517,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     double4 ea = make_double4(q2.y, q2.z, q2.w, wa4.w);  // xysum.x, xysum.y, es, ev
     ea.z += acc.es_dot * dt;
     ea.w += acc.ev_dot * dt;
-    d.rates[s] = make_double4(acc.es_dot, acc.ev_dot, acc.p, 0.0);
+    d.pres[s] = acc.p;
     if (S.nw > 0) d.wall_pp[o] = wf0;
     if (S.nw > 1) d.wall_pp[(size_t)d.np + o] = wf1;
     if (S.nw > 2) d.wall_pp[2 * (size_t)d.np + o] = wf2;
@@ -1537,8 +1537,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
             else { force = force - f; torque = torque + t; }
         }
     }
-    d.force[s] = make_double4(force.x, force.y, force.z, 0.0);
-    d.torque[s] = make_double4(torque.x, torque.y, torque.z, 0.0);
+    d.force[s] = make_double4(force.x, force.y, force.z, acc.es_dot);
+    d.torque[s] = make_double4(torque.x, torque.y, torque.z, acc.ev_dot);
 
     // ---- leapfrog (src/integration.cuh:20-185)
     const double radius = ra;

@@ -124,6 +124,7 @@ EXPORTS = {
     "sph_step": (C.c_int, [C.c_void_p, C.c_uint64]),
     "sph_run": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "sph_writebin": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "sph_readbin": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sph_check_values": (C.c_int, [C.c_void_p]),
     "sph_field_bytes": (C.c_size_t, [C.c_void_p, C.c_int]),
     "sph_get": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
@@ -235,9 +236,10 @@ class DEM:
 
     # --------------------------------------------------- reference members
     def readbin(self, target: str):
-        """Replaces the simulation with the contents of ``target``."""
-        self.close()
-        self.__init__(target, verbosity=0, checkVals=1)
+        """Replaces the simulation with the contents of ``target`` (settings
+        kept; on a read or check error the state is unchanged)."""
+        self._check(self._lib.sph_readbin(self._h, os.fspath(target).encode()))
+        self.sid = os.path.basename(os.fspath(target)).rsplit(".", 1)[0]
 
     def writebin(self, target: str):
         self._check(self._lib.sph_writebin(self._h, os.fspath(target).encode()))

@@ -221,3 +221,41 @@ def test_search_variants_parity(tmp_path, name, setup, periodic):
                                 gamma_n=0.0, gamma_t=10.0, sid=f"search-{name}-{periodic}")
     errs = run_pair(sb, tmp_path, [1, 50, 300], setup=setup)
     assert not errs, errs
+
+
+def test_readbin_replaces_the_simulation(tmp_path):
+    """sph_readbin on an open handle (DEM::readbin): the new file's state,
+    step count reset, the handle's settings kept; a missing file leaves the
+    handle as it was."""
+    from sphere_amd.dem import DEM, SphereIOError
+
+    _say("readbin on an open handle")
+    a = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=1, gamma_n=0.0,
+                               gamma_t=10.0, sid="rb-a")
+    b = configs.lattice_settle(nside=(5, 5, 4), spacing=2.15e-2, jitter=5e-4, periodic=0, nw=1, seed=3, sid="rb-b")
+    pa, pb = str(tmp_path / "rb-a.bin"), str(tmp_path / "rb-b.bin")
+    a.writebin(pa)
+    b.writebin(pb)
+    d = DEM(pa, verbosity=0)
+    d.set_precision("mixed")
+    d.set_search(5)
+    d.step(20)
+    d.readbin(pb)
+    info = d.info()
+    assert info.np == b.np and info.steps_done == 0 and info.precision == 1
+    ref = DEM(pb, verbosity=0)
+    ref.set_precision("mixed")
+    ref.set_search(5)
+    d.step(50)
+    ref.step(50)
+    for f in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t", "walls_nx", "walls_mvfd", "time"):
+        assert np.array_equal(d.get(f), ref.get(f)), f
+    x = d.get("x")
+    with pytest.raises(SphereIOError):
+        d.readbin(str(tmp_path / "missing.bin"))
+    assert np.array_equal(d.get("x"), x)
+    d.step(5)
+    ref.step(5)
+    assert np.array_equal(d.get("x"), ref.get("x"))
+    d.close()
+    ref.close()
