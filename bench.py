@@ -284,8 +284,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    # (the first sph_step of at least 24 steps, 42 for the ensemble, times the
-    # contact-search kernels; a warm-up of 50 keeps that out of the timed steps)
+    # (the first sph_step of at least 36 steps, 54 for the ensemble, times the
+    # search and force kernel variants; a warm-up of 50 keeps that out of the
+    # timed steps, the ensemble's warm-up is at least 60)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -401,7 +402,7 @@ def main():
         t0 = time.perf_counter()
         members = configs.calibration_ensemble(members=args.ensemble_members, steps=args.ensemble_steps)
         log(f"generated the ensemble in {time.perf_counter() - t0:.1f}s")
-        result["ensemble"] = ensemble_run(members, args.ensemble_steps, max(args.warmup, 50))
+        result["ensemble"] = ensemble_run(members, args.ensemble_steps, max(args.warmup, 60))
         del members
     if not args.no_cpu:
         rate, secs = cpu_baseline(bin_path, n, args.cpu_steps)

@@ -192,6 +192,13 @@ int sph_set_tile(sph_t* h, uint32_t tile_x, uint32_t tile_y);
  * keeps the fastest.  All kernels give bit-identical results. */
 int sph_set_search(sph_t* h, int mode);
 
+/* Force kernel: 1 = each contact's partner velocities are gathered when the
+ * contact is processed, 2 = gathered ahead with the partner positions, four
+ * slots at a time (more registers: faster with many contacts per sphere),
+ * 0 = automatic (the default): timed with the search kernels on the first
+ * long enough sph_step and every 4096 steps.  All give the same bits. */
+int sph_set_force_kernel(sph_t* h, int variant);
+
 /* Measurement helper with no counterpart in the reference: copies `bytes`
  * between two fresh device buffers `reps` times with a plain streaming kernel
  * (k_copy_probe) and returns the achieved read+write rate in GB/s. */

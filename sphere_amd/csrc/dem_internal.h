@@ -126,6 +126,7 @@ struct DemDev {
     uint32_t tile_x, tile_y, tiles_x, tiles_y;
     int search_mode;    // 1: LDS tiles, 2: rows per particle, 3: wave per cell, 4: half shell, 5: lists
     uint32_t tpb_search, tpb_force;  // threads per block of k_topo_rows / k_topo_half and of k_force (<= BLOCK)
+    uint32_t force_vw;  // k_force gathers the partners' velocity records with their positions
     uint32_t* skey;     // cell id per sorted position
     float4* xf;         // FP32 copy of a.x for the search's distance prefilter
     float search_eps;   // prefilter margin (m), above the FP32 rounding of a distance

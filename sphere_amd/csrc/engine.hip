@@ -726,9 +726,16 @@ int make_events(sph_dem* h) {
 
 // Times the contact search of four steps with each of the row (2),
 // half-shell (4) and neighbour-list (5) kernels and keeps the fastest by the
-// mean of the last three (the first warms up, or builds the lists); the steps
-// are ordinary steps of the run (all kernels give the same bits).
-constexpr uint64_t TUNE_STEPS = 12;
+// mean of the last three (the first warms up, or builds the lists), then the
+// force kernel with and without the partners' velocity gathers batched
+// (three steps each), each when its choice is automatic; the steps are
+// ordinary steps of the run (all kernels give the same bits).
+constexpr uint64_t TUNE_SEARCH_STEPS = 12, TUNE_FORCE_STEPS = 6, TUNE_BLOCK_STEPS = 9;
+
+uint64_t tune_steps(const sph_dem* h) {
+    return (h->search_auto ? TUNE_SEARCH_STEPS : 0) + (h->force_auto ? TUNE_FORCE_STEPS : 0) +
+           (h->blocks_auto ? TUNE_BLOCK_STEPS : 0);
+}
 
 // One ordinary step, timed per stage: ms[0] search, ms[1] force.
 int timed_step(sph_dem* h, float ms[2]) {
@@ -753,7 +760,7 @@ int sph::tune_search(sph_dem* h, uint64_t* ran) {
     const int modes[3] = {2, 4, 5};
     float mean[3] = {0.0f, 0.0f, 0.0f};
     uint64_t steps = 0;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 3 && h->search_auto; ++k) {
         h->d.search_mode = modes[k];
         if (modes[k] == 5) request_list_rebuild(h);
         for (int rep = 0; rep < 4; ++rep) {
@@ -763,11 +770,24 @@ int sph::tune_search(sph_dem* h, uint64_t* ran) {
             if (rep > 0) mean[k] += ms[0] / 3.0f;
         }
     }
-    int best = 0;
-    for (int k = 1; k < 3; ++k)
-        if (mean[k] < mean[best]) best = k;
-    h->d.search_mode = modes[best];
-    if (modes[best] == 5) request_list_rebuild(h);
+    if (h->search_auto) {
+        int best = 0;
+        for (int k = 1; k < 3; ++k)
+            if (mean[k] < mean[best]) best = k;
+        h->d.search_mode = modes[best];
+        if (modes[best] == 5) request_list_rebuild(h);
+    }
+    float fv[2] = {0.0f, 0.0f};
+    for (int v = 0; v < 2 && h->force_auto; ++v) {
+        h->d.force_vw = (uint32_t)v;
+        for (int rep = 0; rep < 3; ++rep) {
+            float ms[2];
+            if ((rc = timed_step(h, ms))) return rc;
+            ++steps;
+            if (rep > 0) fv[v] += ms[1] / 2.0f;
+        }
+    }
+    if (h->force_auto) h->d.force_vw = fv[1] < fv[0] ? 1u : 0u;
     if (h->blocks_auto) {
         // threads per block of the chosen search kernel and of k_force, on
         // the same steps (mean of the last two of three per size)
@@ -794,8 +814,9 @@ int sph::tune_search(sph_dem* h, uint64_t* ran) {
     h->tuned = true;
     h->steps_since_tune = 0;
     if (h->verbosity > 1)
-        std::printf("sphere_dem: contact search %.4f (rows) %.4f (half shell) %.4f (lists) ms: mode %d\n", mean[0],
-                    mean[1], mean[2], h->d.search_mode);
+        std::printf("sphere_dem: contact search %.4f (rows) %.4f (half shell) %.4f (lists) ms: mode %d; "
+                    "force %.4f / %.4f ms: velocity gathers %s\n", mean[0], mean[1], mean[2], h->d.search_mode,
+                    fv[0], fv[1], h->d.force_vw ? "batched" : "per slot");
     return check_flags(h);
 }
 
@@ -829,8 +850,8 @@ int run_steps(sph_dem* h, uint64_t n) {
         h->timing = t;
         return check_flags(h);
     }
-    const uint64_t tune_steps = TUNE_STEPS + (h->blocks_auto ? 9 : 0);
-    if (h->search_auto && n >= 2 * tune_steps && (!h->tuned || h->steps_since_tune >= 4096)) {
+    const uint64_t ts = tune_steps(h);
+    if (ts && n >= 2 * ts && (!h->tuned || h->steps_since_tune >= 4096)) {
         uint64_t ran = 0;
         if ((rc = sph::tune_search(h, &ran))) return rc;
         n -= ran;
@@ -954,7 +975,7 @@ int sph_readbin(sph_t* h, const char* path) {
     const int precision = h->precision, profiling = h->profiling, verbosity = h->verbosity, vals = h->check_vals;
     const bool use_graph = h->use_graph, search_auto = h->search_auto;
     const int search_mode = h->d.search_mode;
-    const uint32_t tpb_force = h->d.tpb_force, tpb_search = h->d.tpb_search;
+    const uint32_t tpb_force = h->d.tpb_force, tpb_search = h->d.tpb_search, force_vw = h->d.force_vw;
     h->adj.reset();
     free_all(h);
     h->events_made = false;
@@ -964,6 +985,7 @@ int sph_readbin(sph_t* h, const char* path) {
     h->d.search_mode = search_mode;
     h->d.tpb_force = tpb_force;
     h->d.tpb_search = tpb_search;
+    h->d.force_vw = force_vw;
     h->own_cur = 0;
     h->precision = SPH_PREC_FP64;
     h->steps_done = 0;
@@ -1378,6 +1400,15 @@ int sph_set_search(sph_t* h, int mode) {
     return SPH_OK;
 }
 
+int sph_set_force_kernel(sph_t* h, int variant) {
+    if (!h || variant < 0 || variant > 2) return SPH_E_INVALID;
+    h->force_auto = variant == 0;
+    h->tuned = false;
+    h->d.force_vw = variant == 2 ? 1u : 0u;
+    destroy_graph(h);
+    return SPH_OK;
+}
+
 int sph_set_graph(sph_t* h, int on) {
     if (!h) return SPH_E_INVALID;
     h->use_graph = on != 0;
@@ -1511,7 +1542,7 @@ int trial_ms(const std::vector<HostState>& ms, double factor, double& ms_step, s
     int rc = ens_open(std::move(t), 0, &e, err);
     if (rc) return rc;
     sph_dem* h = e->h;
-    rc = run_steps(h, 2 * (TUNE_STEPS + 9));
+    rc = run_steps(h, 2 * tune_steps(h));
     hipEvent_t a = nullptr, b = nullptr;
     if (!rc && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) rc = SPH_E_HIP;
     float ms10 = 0.0f;

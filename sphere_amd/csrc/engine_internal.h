@@ -42,6 +42,7 @@ struct sph_dem {
     int device = 0;
     // automatic contact-search choice (search mode 0)
     bool search_auto = true;
+    bool force_auto = true;  // k_force variant chosen by timing (sph_set_force_kernel 0)
     uint64_t steps_since_tune = 0;
     bool tuned = false;
     // checkpoints, tape and adjoint buffers of sph_adjoint_* (adjoint.hip)

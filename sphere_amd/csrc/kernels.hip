@@ -1439,7 +1439,12 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
 // ------------------------------------------------------------ force kernel
 // One thread per sorted particle: contact forces from the occupied slots in
 // slot order, wall contacts, bond forces, leapfrog.
-template <class T, bool B>
+constexpr int FORCE_BATCH = 4;
+
+// VW: the partners' velocity records are gathered with their positions, ahead
+// of the slots' processing (more registers, fewer waves: faster with many
+// contacts per sphere, slower with few; the host picks by timing).
+template <class T, bool B, bool VW>
 __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
@@ -1462,26 +1467,57 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     acc.F = mk(0.0, 0.0, 0.0);
     acc.T = mk(0.0, 0.0, 0.0);
     acc.es_dot = 0.0; acc.ev_dot = 0.0; acc.p = 0.0;
-    for (uint32_t m = mask; m; m &= m - 1) {
-        const int i = __ffs(m) - 1;
-        const uint32_t w = slot[i];
-        const uint32_t bo = w & SLOT_IDX_MASK;
-        const uint32_t pb = d.perm[d.pos_of[bo]];
-        double4 xb = d.ro.x[pb];
-        (void)wrap_xy(S, xb.x, xb.y);
-        double dmx, dmy;
-        decode_image(S, w >> 28, dmx, dmy);
-        const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
-        const double len = vlen(xab);
-        const double delta_n = len - (ra + xb.w);
-        if (delta_n < 0.0) {
-            const size_t mp = (size_t)o * NC + i;
-            V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
-            contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(d.ro.vel[pb])), cvt<T>(xyz(d.ro.angvel[pb])),
-                              (T)ra, (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
-            SlotStore<T>::store(d, mp, dt0);
-        } else {
-            mask &= ~(1u << i);  // contact ended: the slot is empty again
+    // Slots in groups of FORCE_BATCH: the group's dependent gathers (slot
+    // word -> partner's sorted position -> where its record is stored -> the
+    // record) are issued together, then the slots are processed one by one in
+    // slot order.
+    for (uint32_t m = mask; m;) {
+        int si[FORCE_BATCH];
+        uint32_t sw[FORCE_BATCH], pb[FORCE_BATCH];
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < FORCE_BATCH; ++k) {
+            si[k] = m ? __ffs(m) - 1 : -1;
+            if (m) { m &= m - 1; ++cnt; }
+        }
+#pragma unroll
+        for (int k = 0; k < FORCE_BATCH; ++k) sw[k] = k < cnt ? slot[si[k]] : 0u;
+#pragma unroll
+        for (int k = 0; k < FORCE_BATCH; ++k) pb[k] = k < cnt ? d.pos_of[sw[k] & SLOT_IDX_MASK] : 0u;
+#pragma unroll
+        for (int k = 0; k < FORCE_BATCH; ++k) pb[k] = k < cnt ? d.perm[pb[k]] : 0u;
+        double4 xbs[FORCE_BATCH], vbs[VW ? FORCE_BATCH : 1], wbs[VW ? FORCE_BATCH : 1];
+#pragma unroll
+        for (int k = 0; k < FORCE_BATCH; ++k) xbs[k] = k < cnt ? d.ro.x[pb[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
+        if constexpr (VW) {
+#pragma unroll
+            for (int k = 0; k < FORCE_BATCH; ++k) {
+                vbs[k] = k < cnt ? d.ro.vel[pb[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
+                wbs[k] = k < cnt ? d.ro.angvel[pb[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < FORCE_BATCH; ++k) {
+            if (k >= cnt) break;
+            const int i = si[k];
+            double4 xb = xbs[k];
+            (void)wrap_xy(S, xb.x, xb.y);
+            double dmx, dmy;
+            decode_image(S, sw[k] >> 28, dmx, dmy);
+            const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
+            const double len = vlen(xab);
+            const double delta_n = len - (ra + xb.w);
+            if (delta_n < 0.0) {
+                const size_t mp = (size_t)o * NC + i;
+                V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
+                const double4 vb = VW ? vbs[VW ? k : 0] : d.ro.vel[pb[k]];
+                const double4 wb = VW ? wbs[VW ? k : 0] : d.ro.angvel[pb[k]];
+                contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(vb)), cvt<T>(xyz(wb)), (T)ra,
+                                  (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
+                SlotStore<T>::store(d, mp, dt0);
+            } else {
+                mask &= ~(1u << i);  // contact ended: the slot is empty again
+            }
         }
     }
 
@@ -1724,14 +1760,19 @@ void launch_topology(const DemDev& d, hipStream_t s) {
     if (d.systems) topology<true>(d, s);
     else topology<false>(d, s);
 }
+template <bool B, bool VW>
+void forces(const DemDev& d, bool mixed, hipStream_t s) {
+    if (mixed) k_force<float, B, VW><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+    else k_force<double, B, VW><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+}
 void launch_forces(const DemDev& d, int precision, hipStream_t s) {
     const bool mixed = precision == SPH_PREC_MIXED;
     if (d.systems) {
-        if (mixed) k_force<float, true><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
-        else k_force<double, true><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+        if (d.force_vw) forces<true, true>(d, mixed, s);
+        else forces<true, false>(d, mixed, s);
     } else {
-        if (mixed) k_force<float, false><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
-        else k_force<double, false><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+        if (d.force_vw) forces<false, true>(d, mixed, s);
+        else forces<false, false>(d, mixed, s);
     }
 }
 void launch_contact(const DemDev& d, int precision, hipStream_t s) {

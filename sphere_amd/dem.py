@@ -138,6 +138,7 @@ EXPORTS = {
     "sph_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_set_tile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "sph_set_search": (C.c_int, [C.c_void_p, C.c_int]),
+    "sph_set_force_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "sph_dry_run": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t,
                               C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
@@ -342,6 +343,9 @@ class DEM:
 
     def set_search(self, mode: int = 0):
         self._check(self._lib.sph_set_search(self._h, int(mode)))
+
+    def set_force_kernel(self, variant: int = 0):
+        self._check(self._lib.sph_set_force_kernel(self._h, int(variant)))
 
     def timing(self) -> Dict[str, float]:
         t = SphTiming()

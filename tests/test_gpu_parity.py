@@ -183,6 +183,8 @@ def test_bonded_solid_full_size(tmp_path):
 
 SEARCH_VARIANTS = [
     ("rows", lambda d: d.set_search(2)),
+    ("rows-vw", lambda d: (d.set_search(2), d.set_force_kernel(2))),
+    ("lists-vw", lambda d: (d.set_search(5), d.set_force_kernel(2))),
     ("wave", lambda d: d.set_search(3)),
     ("half", lambda d: d.set_search(4)),
     ("lists", lambda d: d.set_search(5)),
