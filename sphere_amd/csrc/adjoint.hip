@@ -69,6 +69,8 @@ struct TapeView {
 struct AdjDev {
     TapeView t;
     Lam next, cur;
+    double4 *gfa, *gfb;   // cotangents of each particle's force and torque, sorted order of the step:
+                          // (gF.x, gF.y, gF.z, gT.x), (gT.y, gT.z, -, -)
     const double4* wobj;  // objective weights, original order
     double inv_n;         // 1 / number of steps of the objective
     const double* lfw;    // adjoint of each wall's force sum for this step
@@ -76,12 +78,6 @@ struct AdjDev {
     const int* wmode;
     double wmass[NWALL];
 };
-
-__device__ __forceinline__ void add3(double4* p, V3<double> v) {
-    unsafeAtomicAdd(&p->x, v.x);
-    unsafeAtomicAdd(&p->y, v.y);
-    unsafeAtomicAdd(&p->z, v.z);
-}
 
 template <class T> __device__ __forceinline__ V3<double> up(V3<T> v) {
     return mk<double>((double)v.x, (double)v.y, (double)v.z);
@@ -336,7 +332,41 @@ __device__ __forceinline__ void block_add(double v, double* dst, double* sh) {
     }
 }
 
-// One adjoint step, one thread per particle of the taped sorted order.
+// Adjoint of the leapfrog and of the objective for one taped step, one thread
+// per particle: the cotangents of its half-step velocities (to cur) and of
+// its force and torque (gfa, gfb, by sorted position, for k_adj_step).
+__global__ __launch_bounds__(256) void k_adj_lf(DemDev d, AdjDev a) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.np) return;
+    const TapeView& t = a.t;
+    const uint32_t o = t.id[s];
+    const double ra = t.x[s].w;
+    const bool fix = t.vel[s].w > 0.0;
+    const double dt = d.sys.dt;
+    const double m = 4.0 / 3.0 * PI_REF * ra * ra * ra * d.sys.rho;
+    const double inertia = 2.0 / 5.0 * m * ra * ra;
+    const V3<double> Lx = xyz(a.next.x[o]), Lv0 = xyz(a.next.v0[o]), Lv = xyz(a.next.v[o]);
+    const V3<double> Lw0 = xyz(a.next.w0[o]), Lw = xyz(a.next.w[o]);
+    const V3<double> Lv0t = Lv0 + Lx * dt + Lv;
+    const V3<double> La = Lv0t * dt + Lv * (0.5 * dt);
+    const V3<double> Lw0t = Lw0 + Lw;
+    const V3<double> Lal = Lw0t * dt + Lw * (0.5 * dt);
+    V3<double> LF = fix ? mk(0.0, 0.0, La.z / m) : La / m;
+    const V3<double> LT = fix ? mk(0.0, 0.0, 0.0) : Lal / inertia;
+    const double4 wo = a.wobj[o];
+    LF = LF + mk(wo.x, wo.y, wo.z) * a.inv_n;
+    a.cur.v0[o] = make_double4(Lv0t.x, Lv0t.y, Lv0t.z, 0.0);
+    a.cur.w0[o] = make_double4(Lw0t.x, Lw0t.y, Lw0t.z, 0.0);
+    a.gfa[s] = make_double4(LF.x, LF.y, LF.z, LT.x);
+    a.gfb[s] = make_double4(LT.y, LT.z, 0.0, 0.0);
+}
+
+// One adjoint step, one thread per particle of the taped sorted order.  A
+// particle gathers every cotangent of its own state: those of its own pair
+// forces and wall forces, and those of each partner's pair force with it,
+// recomputed from the partner's side (the partner's slot holding this
+// particle, its spring and its force cotangent) -- so every write of cur is
+// a plain store and no atomics are needed.
 template <class T>
 __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
     __shared__ double red[4];
@@ -350,41 +380,19 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
         const uint32_t o = t.id[s];
         const double4 xa4 = t.x[s], va4 = t.vel[s], wa4 = t.angvel[s];
         const double ra = xa4.w;
-        const bool fix = va4.w > 0.0;
-        const double dt = d.sys.dt;
-        const double m = 4.0 / 3.0 * PI_REF * ra * ra * ra * d.sys.rho;
-        const double inertia = 2.0 / 5.0 * m * ra * ra;
-        // 1. leapfrog backward
-        const V3<double> Lx = xyz(a.next.x[o]), Lv0 = xyz(a.next.v0[o]), Lv = xyz(a.next.v[o]);
-        const V3<double> Lw0 = xyz(a.next.w0[o]), Lw = xyz(a.next.w[o]);
-        // this particle's entries of next are consumed: clear them, so next
-        // is zero when it becomes cur (accumulated with atomics) a step earlier
-        const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
-        a.next.x[o] = z4;
-        a.next.v[o] = z4;
-        a.next.w[o] = z4;
-        const V3<double> Lv0t = Lv0 + Lx * dt + Lv;
-        const V3<double> La = Lv0t * dt + Lv * (0.5 * dt);
-        const V3<double> Lw0t = Lw0 + Lw;
-        const V3<double> Lal = Lw0t * dt + Lw * (0.5 * dt);
-        V3<double> LF = fix ? mk(0.0, 0.0, La.z / m) : La / m;
-        const V3<double> LT = fix ? mk(0.0, 0.0, 0.0) : Lal / inertia;
-        // 2. objective
-        const double4 wo = a.wobj[o];
-        LF = LF + mk(wo.x, wo.y, wo.z) * a.inv_n;
-        a.cur.v0[o] = make_double4(Lv0t.x, Lv0t.y, Lv0t.z, 0.0);
-        a.cur.w0[o] = make_double4(Lw0t.x, Lw0t.y, Lw0t.z, 0.0);
-        V3<double> gx = Lx, gv = mk(0.0, 0.0, 0.0), gw = mk(0.0, 0.0, 0.0);
+        const double4 fa = a.gfa[s], fb = a.gfb[s];
+        const V3<double> LF = mk(fa.x, fa.y, fa.z), LT = mk(fa.w, fb.x, fb.y);
+        V3<double> gx = xyz(a.next.x[o]), gv = mk(0.0, 0.0, 0.0), gw = mk(0.0, 0.0, 0.0);
         // 3. contacts in slot order
         const uint32_t mask = t.tmask[s], fresh = t.tfresh[s];
         const V3<double> va = xyz(va4), wa = xyz(wa4);
-        for (int i = 0; i < NC; ++i) {
+        // (only the slots occupied after the search: the adjoint of the step
+        // before reads a spring adjoint only for a slot that was occupied at
+        // the end of that step, i.e. at this step's start)
+        for (uint32_t mm = mask; mm; mm &= mm - 1) {
+            const int i = __ffs(mm) - 1;
             const size_t mp = (size_t)o * NC + i;
             double* cdt_out = a.cur.dt + 3 * mp;
-            if (!((mask >> i) & 1u)) {
-                cdt_out[0] = cdt_out[1] = cdt_out[2] = 0.0;
-                continue;
-            }
             const uint32_t w = t.cslot[mp];
             const uint32_t bo = w & SLOT_IDX_MASK;
             const uint32_t sb = t.pos_of[bo];
@@ -408,16 +416,46 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
             PairGrad<T> g;
             g.gx = g.gva = g.gvb = g.gwa = g.gwb = g.gdt0u = mk<T>(0, 0, 0);
             g.gkn = g.gkt = g.ggn = g.gmus = g.gmud = (T)0;
-            contact_linear_vjp<T>(d, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(t.vel[sb])), cvt<T>(xyz(t.angvel[sb])),
-                                  (T)ra, (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0u, cvt<T>(LF), cvt<T>(LT),
+            const V3<double> vb = xyz(t.vel[sb]), wb = xyz(t.angvel[sb]);
+            contact_linear_vjp<T>(d, cvt<T>(va), cvt<T>(wa), cvt<T>(vb), cvt<T>(wb), (T)ra, (T)xb.w, cvt<T>(xab),
+                                  (T)len, (T)delta_n, dt0u, cvt<T>(LF), cvt<T>(LT),
                                   mk<T>((T)gdp[0], (T)gdp[1], (T)gdp[2]), g);
-            const V3<double> gxd = up(g.gx);
-            gx = gx + gxd;
-            add3(a.cur.x + bo, -gxd);
+            gx = gx + up(g.gx);
             gv = gv + up(g.gva);
-            add3(a.cur.v + bo, up(g.gvb));
             gw = gw + up(g.gwa);
-            add3(a.cur.w + bo, up(g.gwb));
+            // the partner's pair force with this particle, from its side
+            const uint32_t mb = t.tmask[sb];
+            const uint32_t* bslot = t.cslot + (size_t)bo * NC;
+            int jb = -1;
+            for (uint32_t q = mb; q; q &= q - 1) {
+                const int k = __ffs(q) - 1;
+                if ((bslot[k] & SLOT_IDX_MASK) == o) { jb = k; break; }
+            }
+            if (jb >= 0) {
+                const size_t mq = (size_t)bo * NC + jb;
+                const uint32_t cb = bslot[jb] >> 28, bx = cb & 3u, by = (cb >> 2) & 3u;
+                const double emx = bx == 1u ? d.sys.L[0] : (bx == 2u ? -d.sys.L[0] : 0.0);
+                const double emy = by == 1u ? d.sys.L[1] : (by == 2u ? -d.sys.L[1] : 0.0);
+                const V3<double> xba = mk((xb.x - xa4.x) + emx, (xb.y - xa4.y) + emy, (xb.z - xa4.z) + 0.0);
+                const double lenb = vlen(xba);
+                const double delb = lenb - (xb.w + ra);
+                V3<T> dtb = mk<T>(0, 0, 0);
+                if (!((t.tfresh[sb] >> jb) & 1u)) {
+                    if (t.cdt32) dtb = mk<T>((T)t.cdt32[3 * mq], (T)t.cdt32[3 * mq + 1], (T)t.cdt32[3 * mq + 2]);
+                    else dtb = mk<T>((T)t.cdt[3 * mq], (T)t.cdt[3 * mq + 1], (T)t.cdt[3 * mq + 2]);
+                }
+                const double4 pa = a.gfa[sb], pb = a.gfb[sb];
+                const double* gdq = a.next.dt + 3 * mq;
+                PairGrad<T> h;
+                h.gx = h.gva = h.gvb = h.gwa = h.gwb = h.gdt0u = mk<T>(0, 0, 0);
+                h.gkn = h.gkt = h.ggn = h.gmus = h.gmud = (T)0;
+                contact_linear_vjp<T>(d, cvt<T>(vb), cvt<T>(wb), cvt<T>(va), cvt<T>(wa), (T)xb.w, (T)ra, cvt<T>(xba),
+                                      (T)lenb, (T)delb, dtb, mk<T>((T)pa.x, (T)pa.y, (T)pa.z),
+                                      mk<T>((T)pa.w, (T)pb.x, (T)pb.y), mk<T>((T)gdq[0], (T)gdq[1], (T)gdq[2]), h);
+                gx = gx - up(h.gx);
+                gv = gv + up(h.gvb);
+                gw = gw + up(h.gwb);
+            }
             const bool f = (fresh >> i) & 1u;
             cdt_out[0] = f ? 0.0 : (double)g.gdt0u.x;
             cdt_out[1] = f ? 0.0 : (double)g.gdt0u.y;
@@ -481,12 +519,14 @@ __global__ __launch_bounds__(256) void k_adj_step(DemDev d, AdjDev a) {
                 gx.y += g.gdelta; gwp[4] -= g.gdelta; gwv[4] += g.gwvel;
             }
         }
-        add3(a.cur.x + o, gx);
-        add3(a.cur.v + o, gv);
-        add3(a.cur.w + o, gw);
+        a.cur.x[o] = make_double4(gx.x, gx.y, gx.z, 0.0);
+        a.cur.v[o] = make_double4(gv.x, gv.y, gv.z, 0.0);
+        a.cur.w[o] = make_double4(gw.x, gw.y, gw.z, 0.0);
     }
     for (int k = 0; k < 4; ++k) block_add(th[k], a.theta + k, red);
-    for (int w = 0; w < 5; ++w) {
+    // (walls past nw stay where the reference's defaults put them: their
+    // adjoints feed nothing)
+    for (int w = 0; w < 5 && w < (int)d.nw; ++w) {
         block_add(gwp[w], a.cur.wall + 3 * w + 0, red);
         block_add(gwv[w], a.cur.wall + 3 * w + 1, red);
     }
@@ -516,7 +556,6 @@ struct TapeOut {
 
 __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     if (q < d.np) {
         const uint32_t p = d.perm[q];
         double4 x = d.ro.x[p];
@@ -533,19 +572,30 @@ __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
         t.angvel[q] = d.ro.angvel[p];
         t.id[q] = d.sid[q];
         t.pos_of[q] = d.pos_of[q];
-        t.tmask[q] = d.tmask[q];
+        const uint32_t mask = d.tmask[q];
+        t.tmask[q] = mask;
         t.tfresh[q] = d.tfresh[q];
+        // the occupied slots only: the adjoint step reads no other (its own
+        // and its partners' occupied slots of this step)
+        const size_t o = (size_t)d.sid[q] * NC;
+        for (uint32_t m = mask; m; m &= m - 1) {
+            const size_t i = o + (size_t)(__ffs(m) - 1);
+            t.cslot[i] = d.cslot[i];
+            if (t.cdt32) {
+                t.cdt32[3 * i] = d.cdt32[3 * i];
+                t.cdt32[3 * i + 1] = d.cdt32[3 * i + 1];
+                t.cdt32[3 * i + 2] = d.cdt32[3 * i + 2];
+            } else {
+                t.cdt[3 * i] = d.cdt[3 * i];
+                t.cdt[3 * i + 1] = d.cdt[3 * i + 1];
+                t.cdt[3 * i + 2] = d.cdt[3 * i + 2];
+            }
+        }
     }
     if (q < 6) {
         t.wnx[q] = d.wnx[q];
         t.wmvfd[q] = d.wmvfd[q];
     }
-    const uint64_t ns = (uint64_t)d.np * NC;
-    for (uint64_t i = q; i < ns; i += stride) t.cslot[i] = d.cslot[i];
-    if (t.cdt32)
-        for (uint64_t i = q; i < 3 * ns; i += stride) t.cdt32[i] = d.cdt32[i];
-    else
-        for (uint64_t i = q; i < 3 * ns; i += stride) t.cdt[i] = d.cdt[i];
 }
 
 inline uint32_t nb(uint64_t n) { return (uint32_t)((n + 255) / 256); }
@@ -610,6 +660,7 @@ struct AdjointState {
     double* tcdt = nullptr;
     float* tcdt32 = nullptr;
     Lam lam[2] = {};
+    double4 *gfa = nullptr, *gfb = nullptr;  // force and torque cotangents of the step being processed
     double4* wobj = nullptr;
     double* scal = nullptr;  // J, theta[4], lfw[6]
     bool have_forward = false;
@@ -774,7 +825,8 @@ int sph_adjoint_backward(sph_t* h, double grad_out[4]) {
             (rc = sph::adj_alloc(h, a, &a->tw, S * n)) || (rc = sph::adj_alloc(h, a, &a->twn, S * 6)) ||
             (rc = sph::adj_alloc(h, a, &a->twm, S * 6)) || (rc = sph::adj_alloc(h, a, &a->tid, S * n)) ||
             (rc = sph::adj_alloc(h, a, &a->tpos, S * n)) || (rc = sph::adj_alloc(h, a, &a->tmask, S * n)) ||
-            (rc = sph::adj_alloc(h, a, &a->tfresh, S * n)) || (rc = sph::adj_alloc(h, a, &a->tslot, S * n * nc)))
+            (rc = sph::adj_alloc(h, a, &a->tfresh, S * n)) || (rc = sph::adj_alloc(h, a, &a->tslot, S * n * nc)) ||
+            (rc = sph::adj_alloc(h, a, &a->gfa, n)) || (rc = sph::adj_alloc(h, a, &a->gfb, n)))
             return rc;
         if (h->precision == SPH_PREC_MIXED) rc = sph::adj_alloc(h, a, &a->tcdt32, S * n * nc * 3);
         else rc = sph::adj_alloc(h, a, &a->tcdt, S * n * nc * 3);
@@ -841,7 +893,10 @@ int sph_adjoint_backward(sph_t* h, double grad_out[4]) {
             ad.theta = a->scal + 1;
             ad.wmode = d_wmode;
             for (uint32_t w = 0; w < 6; ++w) ad.wmass[w] = w < h->hs.nw ? h->hs.wmvfd[4 * w] : 1.0;
+            ad.gfa = a->gfa;
+            ad.gfb = a->gfb;
             sph::k_adj_walls<<<1, 64, 0, st>>>(h->d, ad, a->scal + 5);
+            sph::k_adj_lf<<<sph::nb(n), 256, 0, st>>>(h->d, ad);
             if (h->precision == SPH_PREC_MIXED) sph::k_adj_step<float><<<sph::nb(n), 256, 0, st>>>(h->d, ad);
             else sph::k_adj_step<double><<<sph::nb(n), 256, 0, st>>>(h->d, ad);
             nxt ^= 1;
