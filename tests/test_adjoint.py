@@ -90,8 +90,8 @@ def test_autograd_function_gives_the_adjoint():
     J2 = dem.adjoint_forward(120, shear_stress_weights(sb), seg_len=50)
     g2 = dem.adjoint_backward()
     dem.close()
-    # partner contributions and block sums are FP64 atomics, so the last
-    # bits depend on their order
+    # the theta block sums are FP64 atomics, so the last bits depend on
+    # their order
     assert abs(float(J) - J2) <= 1e-12 * abs(J2)
     np.testing.assert_allclose(theta.grad.numpy(), g2, rtol=1e-10, atol=0.0)
 
