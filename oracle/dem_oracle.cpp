@@ -197,7 +197,11 @@ extern "C" orc_t* orc_open(const char* path, int nc, char* err, size_t errlen) {
     std::fclose(f);
     if (!r.ok) { delete s; seterr(err, errlen, "truncated"); return nullptr; }
     if (s->nu > 0.0) { delete s; seterr(err, errlen, "fluid (nu>0) unsupported"); return nullptr; }
-    if (s->contactmodel != 2) { delete s; seterr(err, errlen, "only contactmodel 2"); return nullptr; }
+    if (s->contactmodel != 1 && s->contactmodel != 2) {
+        delete s;
+        seterr(err, errlen, "only contactmodels 1 and 2");
+        return nullptr;
+    }
     // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel; walls vel0 <- mvfd.y
     s->vel0.resize(n); s->angvel0.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -221,6 +225,10 @@ extern "C" orc_t* orc_open(const char* path, int nc, char* err, size_t errlen) {
 }
 
 extern "C" void orc_close(orc_t* s) { delete s; }
+
+double det_exp(double x);
+// The exponential the capillary force uses, for its own accuracy check.
+extern "C" double orc_exp(double x) { return det_exp(x); }
 
 // Writes the layout of src/file_io.cpp:319-452.
 extern "C" int orc_writebin(orc_t* s, const char* path) {
@@ -361,8 +369,10 @@ int find_dist_mod(const orc_t* s, int& tx, int& ty, int tz, V3& dm) {
 
 }  // namespace
 
-// src/contactsearch.cuh:181-307 findContactsInCell + :314-355 topology.
+// src/contactsearch.cuh:181-307 findContactsInCell + :314-355 topology
+// (called for contact models 2 and 3 only, src/device.cu:809).
 static void stage_topology(orc_t* s) {
+    if (s->contactmodel == 1) return;
     const uint32_t NC = (uint32_t)s->nc;
     const uint32_t n = s->np;
     for (uint32_t a = 0; a < n; ++a) {
@@ -495,7 +505,103 @@ double contact_linear_wall(orc_t* s, Acc& acc, const D4& linvel4, V3 angvel, dou
 
 }  // namespace
 
-// src/contactsearch.cuh:369-689 interact (contactmodel 2).
+// The reference's expf of a double argument, restated as a double-precision
+// exponential of fixed steps: Cody-Waite reduction x = k ln2 + r with
+// |r| <= ln2/2 (ln2 split into a 32-bit-exact high part and a low part), the
+// Taylor series of e^r to degree 13 by Horner's rule (truncation error below
+// 1e-17 relative on that range), and the scaling by 2^k.
+double det_exp(double x) {
+    const double inv_ln2 = 1.4426950408889634074;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double k = std::nearbyint(x * inv_ln2);
+    const double r = (x - k * ln2_hi) - k * ln2_lo;
+    double q = 1.0 / 6227020800.0;                      // 1/13!
+    const double c[13] = {1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0,
+                          1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0,
+                          1.0 / 6.0, 0.5, 1.0, 1.0};  // 1/12! .. 1/0!
+    for (int i = 0; i < 13; ++i) q = q * r + c[i];
+    return std::ldexp(q, (int)k);
+}
+
+namespace {
+
+// src/contactmodels.cuh:114-253 contactLinearViscous: no spring history, no
+// torque (commented out in the reference), viscous tangential force capped
+// by Coulomb friction.
+void contact_viscous(orc_t* s, Acc& acc, const D4& vel_a, V3 angvel_a, const D4& vel_b, V3 angvel_b,
+                     double radius_a, double radius_b, V3 x, double x_length, double delta) {
+    const V3 n = divs(x, x_length);
+    const V3 vel_linear = v3(vel_a.x - vel_b.x, vel_a.y - vel_b.y, vel_a.z - vel_b.z);
+    const V3 vel = add3(add3(vel_linear, smul(radius_a + delta / 2.0, cross3(n, angvel_a))),
+                        smul(radius_b + delta / 2.0, cross3(n, angvel_b)));
+    const double vel_n = dot3(vel_linear, n);
+    const V3 vel_t = sub3(vel, muls(n, dot3(vel, n)));
+    const double vel_t_length = len3(vel_t);
+    V3 f_n = smul(-s->k_n * delta - s->gamma_n * vel_n, n);
+    if (dot3(f_n, n) < 0.0) f_n = v3(0.0, 0.0, 0.0);
+    const double f_n_length = len3(f_n);
+    const V3 f_c = smul(-s->kappa * std::sqrt(radius_a * radius_b), n);
+    V3 f_t = v3(0.0, 0.0, 0.0);
+    if (vel_t_length > 0.0) {
+        const double f_t_visc = s->gamma_t * vel_t_length;
+        const double f_t_limit = vel_t_length > (double)0.001f ? s->mu_d * len3(sub3(f_n, f_c))
+                                                               : s->mu_s * len3(sub3(f_n, f_c));
+        if (f_t_visc < f_t_limit) f_t = divs(smul(-f_t_visc, vel_t), vel_t_length);
+        else f_t = divs(smul(-f_t_limit, vel_t), vel_t_length);
+    }
+    acc.F = add3(acc.F, add3(add3(f_n, f_t), f_c));
+    acc.p += f_n_length / (4.0 * PI * radius_a * radius_a);
+}
+
+// src/cohesion.cuh:314-350 capillaryCohesion_exp (as written: the decay
+// length lambda = 0.9 h sqrt(V_b / R_har) has h = -sqrt(r) < 0).
+void capillary(orc_t* s, Acc& acc, double radius_a, double radius_b, double delta, V3 x, double x_length) {
+    const V3 n = divs(x, x_length);
+    const double r = (radius_a / radius_b) > (radius_b / radius_a) ? radius_a / radius_b : radius_b / radius_a;
+    const double h = -std::sqrt(r);
+    const double R_har = (2.0 * radius_a * radius_b) / (radius_a + radius_b);
+    const double R_geo = std::sqrt(radius_a * radius_b);
+    const double lambda = (double)0.9f * h * std::sqrt(s->V_b / R_har);
+    acc.F = add3(acc.F, smul(-s->kappa * R_geo * det_exp(-delta / lambda), n));
+}
+
+// src/contactsearch.cuh:76-173 findAndProcessContactsInCell over the 27
+// cells (:548-576): contact model 1 finds and processes contacts at once.
+void interact_viscous(orc_t* s, uint32_t a, Acc& acc) {
+    const D4 xa = s->x_sorted[a];
+    const double ra = xa.w;
+    const int gx = cell_coord(s, xa.x, 0), gy = cell_coord(s, xa.y, 1), gz = cell_coord(s, xa.z, 2);
+    for (int dz = -1; dz < 2; ++dz)
+        for (int dy = -1; dy < 2; ++dy)
+            for (int dx = -1; dx < 2; ++dx) {
+                int tx = gx + dx, ty = gy + dy;
+                const int tz = gz + dz;
+                V3 dm = v3(0.0, 0.0, 0.0);
+                if (find_dist_mod(s, tx, ty, tz, dm) == -1) continue;
+                const uint32_t c = (uint32_t)tx + (uint32_t)ty * s->num[0] + (s->num[0] * s->num[1]) * (uint32_t)tz;
+                const uint32_t st = s->cellStart[c];
+                if (st == EMPTY) continue;
+                const uint32_t en = s->cellEnd[c];
+                for (uint32_t b = st; b < en; ++b) {
+                    if (b == a) continue;
+                    const D4 xb = s->x_sorted[b];
+                    const double rb = xb.w;
+                    V3 xab = v3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                    xab = add3(xab, dm);
+                    const double len = len3(xab);
+                    const double delta = len - (ra + rb);
+                    if (delta < 0.0)
+                        contact_viscous(s, acc, s->vel_sorted[a], s->angvel_sorted[a], s->vel_sorted[b],
+                                        s->angvel_sorted[b], ra, rb, xab, len, delta);
+                    else if (delta < s->db)
+                        capillary(s, acc, ra, rb, delta, xab, len);
+                }
+            }
+}
+
+}  // namespace
+
+// src/contactsearch.cuh:369-689 interact (contact models 1 and 2).
 static void stage_interact(orc_t* s) {
     const uint32_t NC = (uint32_t)s->nc;
     const uint32_t n = s->np;
@@ -513,7 +619,8 @@ static void stage_interact(orc_t* s) {
         acc.es_dot = 0.0; acc.ev_dot = 0.0; acc.p = 0.0;
         const D4 vel_a = s->vel_sorted[a];
         const V3 angvel_a = s->angvel_sorted[a];
-        for (uint32_t i = 0; i < NC; ++i) {
+        if (s->contactmodel == 1) interact_viscous(s, a, acc);
+        for (uint32_t i = 0; i < NC && s->contactmodel == 2; ++i) {
             const size_t mp = (size_t)ao * NC + i;
             const uint32_t bo = s->contacts[mp];
             if (bo != n) {

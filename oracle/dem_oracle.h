@@ -19,6 +19,14 @@
  *   contactLinear / wall ......... closed forms: restitution, rest-on-floor
  *                                  overlap m*g/k_n, stick-slip at mu_s*|f_n|,
  *                                  Newton's third law
+ *   contactLinearViscous ......... (contact model 1) first-step pair force
+ *                                  -k_n delta - gamma_n v_n, viscous shear
+ *                                  gamma_t |v_t| below the Coulomb cap and
+ *                                  mu_d |f_n| above it, no torque, Newton's
+ *                                  third law, contact slots left untouched
+ *   capillaryCohesion_exp ........ the closed form kappa R_geo exp(-d/lambda)
+ *                                  for a pair at gap d < db; orc_exp against
+ *                                  the C library's exp to 2 ulp
  *   bondsLinear .................. two-particle bond energy conservation and
  *                                  axial frequency sqrt(2 k_n A / m)
  *   integrate .................... leapfrog free-fall closed form
@@ -114,6 +122,9 @@ void orc_dims(const orc_t* s, uint64_t* out5);
 /* k_n, k_t, gamma_n, gamma_t, mu_s, mu_d (set: same order) */
 void orc_get_params(const orc_t* s, double* out6);
 void orc_set_params(orc_t* s, const double* in6);
+
+/* The exponential the capillary force uses (see dem_oracle.cpp). */
+double orc_exp(double x);
 
 #ifdef __cplusplus
 }

@@ -760,6 +760,10 @@ int sph_adjoint_forward(sph_t* h, uint64_t nsteps, uint32_t seg_len, const doubl
         h->err = "sph_adjoint_forward: bonds are not differentiated";
         return SPH_E_UNSUPPORTED;
     }
+    if (h->hs.contactmodel != 2) {
+        h->err = "sph_adjoint_forward: only contact model 2 is differentiated";
+        return SPH_E_UNSUPPORTED;
+    }
     const size_t n = h->hs.np;
     h->adj = std::unique_ptr<AdjointState, void (*)(AdjointState*)>(new AdjointState(), sph::free_adj);
     AdjointState* a = h->adj.get();

@@ -89,6 +89,8 @@ struct Sys {
     double k_n, k_t, gamma_n, gamma_t, mu_s, mu_d;
     double gamma_wn, gamma_wt, mu_ws, mu_wd, rho, kappa;
     double devs_A, devs_f;
+    uint32_t contactmodel;  // 1: visco-frictional, found and processed at once; 2: linear with slots
+    double db, V_b;         // capillary bond reach and liquid volume (contact model 1)
 };
 
 // Everything a step kernel needs, passed by value as one kernel argument.

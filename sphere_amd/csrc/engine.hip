@@ -148,6 +148,8 @@ sph::Sys make_sys(const HostState& s) {
     S.mu_s = s.mu_s; S.mu_d = s.mu_d; S.gamma_wn = s.gamma_wn; S.gamma_wt = s.gamma_wt;
     S.mu_ws = s.mu_ws; S.mu_wd = s.mu_wd; S.rho = s.rho; S.kappa = s.kappa;
     S.devs_A = s.devs_A; S.devs_f = s.devs_f;
+    S.contactmodel = s.contactmodel;
+    S.db = s.db; S.V_b = s.V_b;
     return S;
 }
 
@@ -239,10 +241,16 @@ void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
 }
 
 int validate_system(sph_dem* h, const HostState& s) {
-    if (s.contactmodel != 2) {
-        h->err = "contactmodel " + std::to_string(s.contactmodel) +
-                 " is not on this path (only the linear visco-elastic model 2)";
+    if (s.contactmodel == 3) {
+        // src/contactmodels.cuh:565-566,596-597 raise the (negative) overlap
+        // to the powers 3/2, 1/4 and 1/2: every contact force is NaN
+        h->err = "contactmodel 3: the reference's Hertz law raises the negative overlap to fractional powers "
+                 "(NaN forces); not provided";
         return SPH_E_UNSUPPORTED;
+    }
+    if (s.contactmodel != 1 && s.contactmodel != 2) {
+        h->err = "contactmodel " + std::to_string(s.contactmodel) + " is not a model of the reference";
+        return SPH_E_INVALID;
     }
     if (s.np == 0 || s.np > sph::MAX_PARTICLES) {
         h->err = "np=" + std::to_string(s.np) + " outside [1, 2^28-1]";
@@ -287,6 +295,10 @@ int validate_for_device(sph_dem* h) {
         }
         if (m.nb0) {
             h->err = "member " + std::to_string(k) + " has bonds; ensembles step unbonded systems";
+            return SPH_E_UNSUPPORTED;
+        }
+        if (m.contactmodel != 2) {
+            h->err = "member " + std::to_string(k) + ": ensembles step contact model 2";
             return SPH_E_UNSUPPORTED;
         }
         K += (double)m.num[0] * m.num[1] * m.num[2];
@@ -689,6 +701,10 @@ int open_common(sph_dem* h, int verbosity, int check_vals) {
     fill_params(h);
     update_search_eps(h);
     choose_tile(h);
+    if (h->hs.contactmodel == 1 && h->members.empty()) {  // no contact search to choose
+        h->search_auto = false;
+        h->d.search_mode = 2;
+    }
     HIP_TRY(hipGetDevice(&h->device));
     HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
@@ -995,7 +1011,7 @@ int sph_readbin(sph_t* h, const char* path) {
     rc = open_common(h, verbosity, vals);
     if (rc) return rc;
     h->use_graph = use_graph;
-    h->search_auto = search_auto;
+    h->search_auto = search_auto && h->hs.contactmodel != 1;
     h->profiling = profiling;
     if (search_mode == 5) request_list_rebuild(h);
     if (precision != SPH_PREC_FP64) rc = sph_set_precision(h, precision);

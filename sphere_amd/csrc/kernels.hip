@@ -472,6 +472,73 @@ __device__ __forceinline__ double contact_wall(const Sys& S, Acc& acc, V3<double
     return dot(f_n, n);
 }
 
+// The exponential of the capillary bond (the reference calls expf on a
+// double): fixed steps, the same as the oracle's -- Cody-Waite reduction
+// x = k ln2 + r with |r| <= ln2/2, the Taylor series of e^r to degree 13 by
+// Horner's rule, scaling by 2^k -- so both sides get the same bits.
+__device__ __forceinline__ double det_exp(double x) {
+    const double inv_ln2 = 1.4426950408889634074;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double k = rint(x * inv_ln2);
+    const double r = (x - k * ln2_hi) - k * ln2_lo;
+    double q = 1.0 / 6227020800.0;  // 1/13!
+    q = q * r + 1.0 / 479001600.0;
+    q = q * r + 1.0 / 39916800.0;
+    q = q * r + 1.0 / 3628800.0;
+    q = q * r + 1.0 / 362880.0;
+    q = q * r + 1.0 / 40320.0;
+    q = q * r + 1.0 / 5040.0;
+    q = q * r + 1.0 / 720.0;
+    q = q * r + 1.0 / 120.0;
+    q = q * r + 1.0 / 24.0;
+    q = q * r + 1.0 / 6.0;
+    q = q * r + 0.5;
+    q = q * r + 1.0;
+    q = q * r + 1.0;
+    return ldexp(q, (int)k);
+}
+
+// src/contactmodels.cuh:114-253 contactLinearViscous (contact model 1, FP64
+// in both precision modes): no spring history, no torque (commented out in
+// the reference), viscous tangential force capped by Coulomb friction.
+__device__ __forceinline__ void contact_viscous(const Sys& S, Acc& acc, V3<double> vel_a, V3<double> angvel_a,
+                                                V3<double> vel_b, V3<double> angvel_b, double radius_a,
+                                                double radius_b, V3<double> x, double x_length, double delta) {
+    const V3<double> n = x / x_length;
+    const V3<double> vel_linear = vel_a - vel_b;
+    const V3<double> vel = vel_linear + (radius_a + delta / 2.0) * cross(n, angvel_a)
+                           + (radius_b + delta / 2.0) * cross(n, angvel_b);
+    const double vel_n = dot(vel_linear, n);
+    const V3<double> vel_t = vel - n * dot(vel, n);
+    const double vel_t_length = vlen(vel_t);
+    V3<double> f_n = (-S.k_n * delta - S.gamma_n * vel_n) * n;
+    if (dot(f_n, n) < 0.0) f_n = mk(0.0, 0.0, 0.0);
+    const double f_n_length = vlen(f_n);
+    const V3<double> f_c = (-S.kappa * sqrt(radius_a * radius_b)) * n;
+    V3<double> f_t = mk(0.0, 0.0, 0.0);
+    if (vel_t_length > 0.0) {
+        const double f_t_visc = S.gamma_t * vel_t_length;
+        const double f_t_limit = vel_t_length > (double)0.001f ? S.mu_d * vlen(f_n - f_c) : S.mu_s * vlen(f_n - f_c);
+        if (f_t_visc < f_t_limit) f_t = ((-f_t_visc) * vel_t) / vel_t_length;
+        else f_t = ((-f_t_limit) * vel_t) / vel_t_length;
+    }
+    acc.F = acc.F + (f_n + f_t + f_c);
+    acc.p += f_n_length / (4.0 * PI_REF * radius_a * radius_a);
+}
+
+// src/cohesion.cuh:314-350 capillaryCohesion_exp, as written (the decay
+// length has h = -sqrt(r) < 0).
+__device__ __forceinline__ void capillary(const Sys& S, Acc& acc, double radius_a, double radius_b, double delta,
+                                          V3<double> x, double x_length) {
+    const V3<double> n = x / x_length;
+    const double r = (radius_a / radius_b) > (radius_b / radius_a) ? radius_a / radius_b : radius_b / radius_a;
+    const double h = -sqrt(r);
+    const double R_har = (2.0 * radius_a * radius_b) / (radius_a + radius_b);
+    const double R_geo = sqrt(radius_a * radius_b);
+    const double lambda = (double)0.9f * h * sqrt(S.V_b / R_har);
+    acc.F = acc.F + (-S.kappa * R_geo * det_exp(-delta / lambda)) * n;
+}
+
 __device__ __forceinline__ void decode_image(const Sys& S, uint32_t code, double& dx, double& dy) {
     const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
     dx = cx == 1u ? S.L[0] : (cx == 2u ? -S.L[0] : 0.0);
@@ -1455,9 +1522,10 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     (void)wrap_xy(S, xa.x, xa.y);
     const double ra = xa.w;
 
-    // contact slots after the search kernel
-    uint32_t mask = d.tmask[s];
-    const uint32_t fresh = d.tfresh[s];  // slots taken from empty this step: delta_t starts at 0
+    // contact slots after the search kernel (contact model 1 keeps none)
+    const bool viscous = S.contactmodel == 1;
+    uint32_t mask = viscous ? 0u : d.tmask[s];
+    const uint32_t fresh = viscous ? 0u : d.tfresh[s];  // slots taken from empty this step: delta_t starts at 0
     const uint32_t* slot = d.cslot + (size_t)o * NC;
 
     // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
@@ -1517,6 +1585,54 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
                 SlotStore<T>::store(d, mp, dt0);
             } else {
                 mask &= ~(1u << i);  // contact ended: the slot is empty again
+            }
+        }
+    }
+
+    // ---- contact model 1: contacts found and processed at once over the 27
+    // cells in z, y, x order, sorted order within a cell, with the capillary
+    // bond for gaps below db (src/contactsearch.cuh:76-173,548-576)
+    if (viscous) {
+        const int nx = (int)S.num[0], ny = (int)S.num[1], nz = (int)S.num[2];
+        const uint32_t key = d.skey[s] - S.cbase;
+        const int gx = (int)(key % S.num[0]), gy = (int)((key / S.num[0]) % S.num[1]);
+        const int gz = (int)(key / (S.num[0] * S.num[1]));
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int tz = gz + dz;
+            if (tz < 0 || tz == nz) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int ty = gy + dy;
+                double dmy = 0.0;
+                if (ty < 0 || ty == ny) {
+                    if (S.periodic != 1) continue;
+                    dmy = ty < 0 ? S.L[1] : -S.L[1];
+                    ty = ty < 0 ? ny - 1 : 0;
+                }
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int tx = gx + dx;
+                    double dmx = 0.0;
+                    if (tx < 0 || tx == nx) {
+                        if (S.periodic == 0) continue;
+                        dmx = tx < 0 ? S.L[0] : -S.L[0];
+                        tx = tx < 0 ? nx - 1 : 0;
+                    }
+                    const uint32_t c = S.cbase + (uint32_t)tx + (uint32_t)ty * S.num[0] + (S.num[0] * S.num[1]) * (uint32_t)tz;
+                    const uint32_t hi = d.off[c + 1];
+                    for (uint32_t sb = d.off[c]; sb < hi; ++sb) {
+                        if (sb == s) continue;
+                        const uint32_t pb = d.perm[sb];
+                        double4 xb = d.ro.x[pb];
+                        (void)wrap_xy(S, xb.x, xb.y);
+                        const V3<double> xab = mk((xa.x - xb.x) + dmx, (xa.y - xb.y) + dmy, (xa.z - xb.z) + 0.0);
+                        const double len = vlen(xab);
+                        const double delta = len - (ra + xb.w);
+                        if (delta < 0.0)
+                            contact_viscous(S, acc, va, wa, xyz(d.ro.vel[pb]), xyz(d.ro.angvel[pb]), ra, xb.w, xab, len,
+                                            delta);
+                        else if (delta < S.db)
+                            capillary(S, acc, ra, xb.w, delta, xab, len);
+                    }
+                }
             }
         }
     }
@@ -1745,6 +1861,7 @@ void launch_reorder(const DemDev& d, hipStream_t s) {
 }
 template <bool B>
 void topology(const DemDev& d, hipStream_t s) {
+    if (!B && d.sys.contactmodel == 1) return;  // contact model 1 keeps no slots (src/device.cu:809)
     if (d.search_mode == 5) {  // one of the two returns at once (nstate[0])
         k_topo_rows<true, B><<<nblk(d.np, d.tpb_search), d.tpb_search, NC * 4 * d.tpb_search, s>>>(d);
         k_list_search<B><<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);

@@ -55,6 +55,8 @@ def lib():
         L.orc_dims.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.orc_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.orc_set_params.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_exp.restype = C.c_double
+        L.orc_exp.argtypes = [C.c_double]
         _lib = L
     return _lib
 

@@ -203,6 +203,10 @@ def test_ensemble_refuses_bonds_and_grid_walking_kernels():
 
     with pytest.raises(Unsupported):
         Ensemble([configs.two_spheres(sid="a"), configs.two_spheres(bonded=True, sid="b")])
+    m1 = configs.two_spheres(sid="c")
+    m1.contactmodel = 1
+    with pytest.raises(Unsupported):
+        Ensemble([configs.two_spheres(sid="a"), m1])
     ens = Ensemble([configs.two_spheres(sid="a")])
     for mode in (1, 3):
         with pytest.raises(SphereInvalid):

@@ -261,3 +261,38 @@ def test_readbin_replaces_the_simulation(tmp_path):
     assert np.array_equal(d.get("x"), ref.get("x"))
     d.close()
     ref.close()
+
+
+@pytest.mark.parametrize("periodic", [0, 1, 2])
+def test_contact_model_1_parity(tmp_path, periodic):
+    """Contact model 1 (contactLinearViscous, found and processed in the
+    27-cell scan, no slots): every array bit for bit, the slots untouched."""
+    _say(f"contact model 1 periodic={periodic}")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=periodic, gamma_n=5.0,
+                                gamma_t=10.0, sid=f"m1-{periodic}")
+    sb.contactmodel = 1
+    errs = run_pair(sb, tmp_path, [1, 50, 300])
+    assert not errs, errs
+
+
+def test_capillary_cohesion_parity(tmp_path):
+    """Contact model 1 with the capillary bond between spheres closer than
+    db (capillaryCohesion_exp and its exponential) against the oracle."""
+    _say("capillary cohesion")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.3e-2, jitter=5e-4, periodic=1, gamma_n=5.0, gamma_t=10.0,
+                                sid="m1-capillary")
+    sb.contactmodel = 1
+    sb.kappa, sb.db, sb.V_b = 0.05, 2e-3, 1e-9
+    errs = run_pair(sb, tmp_path, [1, 50, 300])
+    assert not errs, errs
+
+
+def test_contact_model_3_is_refused():
+    """The reference's Hertz law raises the negative overlap to fractional
+    powers (NaN forces); the handle refuses it with a message."""
+    from sphere_amd.dem import DEM, Unsupported
+
+    sb = configs.two_spheres(sid="hertz")
+    sb.contactmodel = 3
+    with pytest.raises(Unsupported, match="Hertz"):
+        DEM(sb, verbosity=0)
