@@ -45,7 +45,7 @@ STAGE_KERNELS = {
     "bonds": "k_bonds", "cellid": "k_fill_u32/k_count (ids from k_force)", "sort": "k_scan_*/k_scatter/k_cellsort",
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
-PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>", "force": "k_force<double, false>",
+PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>", "force": "k_force<double, false, false>",
                   "cellid": "k_count"}
 SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply",
                   5: "k_list_search (k_topo_rows<true> on rebuild steps)"}

@@ -135,6 +135,8 @@ struct DemDev {
     uint32_t* nlist;    // mode 5: np * NLIST partners (their list index), list index = sorted position at the build
     uint32_t* ncnt;     // mode 5: list length per list index
     float4* xref;       // mode 5: position at the last list build, per list index
+    float* nlistg;      // mode 5: np * NLIST, each entry's gap at the build (FP32), transposed like nlist
+    float* ldisp;       // mode 5: displacement since the build, per list index (FP32)
     uint32_t* linv;     // mode 5: list index -> sorted position of this step
     uint32_t* slidx;    // mode 5: list index per sorted position of this step
     uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed

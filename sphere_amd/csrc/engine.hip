@@ -495,7 +495,7 @@ int allocate(sph_dem* h) {
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
     ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
     ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
-    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
+    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.nlistg, n * sph::NLIST); ALLOC(d.ldisp, n); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     const size_t ns = h->members.empty() ? 1 : h->members.size();
     ALLOC(d.wnx, 6 * ns); ALLOC(d.wmvfd, 6 * ns); ALLOC(d.wvel0, 6 * ns); ALLOC(d.wmode, 6 * ns);

@@ -836,9 +836,14 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
         }
         if constexpr (BUILD) {
             for (; near; near &= near - 1) {
-                const uint32_t sb = sb0 + (uint32_t)(__ffs(near) - 1);
+                const int k = __ffs(near) - 1;
+                const uint32_t sb = sb0 + (uint32_t)k;
                 if (sb == s) continue;
-                if (lcnt < (uint32_t)NLIST) list[(size_t)lcnt * d.np] = sb;  // the partner's list index
+                if (lcnt < (uint32_t)NLIST) {
+                    list[(size_t)lcnt * d.np] = sb;  // the partner's list index
+                    const float ex = (xaf.x - c[k].x) + dmx, ey = (xaf.y - c[k].y) + dmy, ez = xaf.z - c[k].z;
+                    d.nlistg[(size_t)lcnt * d.np + s] = sqrtf(ex * ex + ey * ey + ez * ez) - (xaf.w + c[k].w);
+                }
                 ++lcnt;
             }
         }
@@ -861,7 +866,7 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
 // the rebuild distance to fit the 27 cells.  It does so only when every
 // periodic axis has at least 5 cells: no cell is then reached twice.
 __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const float4* __restrict__ xf,
-                                        uint32_t* list, uint32_t np, uint32_t& lcnt, int gx, int gy, int gz, int nx,
+                                        uint32_t* list, float* gap, uint32_t np, uint32_t& lcnt, int gx, int gy, int gz, int nx,
                                         int ny, int nz, int periodic, float Lx, float Ly, uint32_t cbase,
                                         float4 xaf, float ra_skin) {
     for (int dz = -2; dz <= 2; ++dz) {
@@ -890,8 +895,12 @@ __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const 
                     const float4 cb = xf[sb];
                     const float ex = (xaf.x - cb.x) + dmx, ey = (xaf.y - cb.y) + dmy, ez = xaf.z - cb.z;
                     const float rl = ra_skin + cb.w;
-                    if (ex * ex + ey * ey + ez * ez <= rl * rl) {
-                        if (lcnt < (uint32_t)NLIST) list[(size_t)lcnt * np] = sb;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    if (d2 <= rl * rl) {
+                        if (lcnt < (uint32_t)NLIST) {
+                            list[(size_t)lcnt * np] = sb;
+                            gap[(size_t)lcnt * np] = sqrtf(d2) - (xaf.w + cb.w);
+                        }
                         ++lcnt;
                     }
                 }
@@ -967,8 +976,8 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     if (st.err) atomicOr(d.flags, st.err);
     if constexpr (BUILD) {  // list index = this sorted position
         if (d.list_shell && !st.stop)
-            list_shell(d.off, d.xf, list, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0], S.Lf[1], S.cbase,
-                       xaf, ra_skin);
+            list_shell(d.off, d.xf, list, d.nlistg + s, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0],
+                       S.Lf[1], S.cbase, xaf, ra_skin);
         d.ncnt[s] = lcnt < (uint32_t)NLIST ? lcnt : (uint32_t)NLIST;
         d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
         if (lcnt > (uint32_t)NLIST || st.stop) atomicOr(&d.nstate[2], 1u);  // list unusable: rebuild next step
@@ -1241,11 +1250,27 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
     int gx = 0, gy = 0, gz = 0;
     uint32_t* gslot = nullptr;
+    // A pair whose gap at the build exceeds the two spheres' displacements
+    // since then cannot overlap now: skip it without touching the partner's
+    // record (margins of three FP32 prefilter widths for the rounding of the
+    // gap and of the displacements)
+    const float da = d.ldisp[li] + 3.0f * d.search_eps;
+    const float* gap = d.nlistg + li;
     for (uint32_t i0 = 0; i0 < nl; i0 += LIST_BATCH) {
-        uint32_t sb[LIST_BATCH];
+        uint32_t lb[LIST_BATCH], sb[LIST_BATCH];
+        float g[LIST_BATCH];
         float4 c[LIST_BATCH];
 #pragma unroll
-        for (int k = 0; k < LIST_BATCH; ++k) sb[k] = i0 + k < nl ? d.linv[list[(size_t)(i0 + k) * d.np]] : s;
+        for (int k = 0; k < LIST_BATCH; ++k) {
+            lb[k] = i0 + k < nl ? list[(size_t)(i0 + k) * d.np] : li;
+            g[k] = i0 + k < nl ? gap[(size_t)(i0 + k) * d.np] : 1e30f;
+        }
+        uint32_t near = 0;
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) near |= g[k] <= da + d.ldisp[lb[k]] ? 1u << k : 0u;
+        if (near == 0u) continue;
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) sb[k] = (near >> k) & 1u ? d.linv[lb[k]] : s;
 #pragma unroll
         for (int k = 0; k < LIST_BATCH; ++k) c[k] = d.xf[sb[k]];
         // FP32 prefilter at the minimum image (an overlap is always closer
@@ -1259,7 +1284,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
             if (py) ey -= S.Lf[1] * rintf(ey / S.Lf[1]);
             const float d2 = ex * ex + ey * ey + ez * ez;
             const float rs = ra_eps + c[k].w;
-            pass |= (i0 + k < nl && d2 <= rs * rs) ? 1u << k : 0u;
+            pass |= ((near >> k) & 1u) && d2 <= rs * rs ? 1u << k : 0u;
         }
         if (pass == 0u) continue;
         if (!have_a) {
@@ -1743,7 +1768,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         const float ddz = (float)pz - xr.z;
         if (S.periodic == 1 || S.periodic == 2) ddx -= S.Lf[0] * rintf(ddx / S.Lf[0]);
         if (S.periodic == 1) ddy -= S.Lf[1] * rintf(ddy / S.Lf[1]);
-        if (ddx * ddx + ddy * ddy + ddz * ddz > d.list_lim * d.list_lim) atomicOr(&d.nstate[1], 1u);
+        const float dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        d.ldisp[li] = sqrtf(dd2);
+        if (dd2 > d.list_lim * d.list_lim) atomicOr(&d.nstate[1], 1u);
     }
 }
 
