@@ -1752,7 +1752,10 @@ int sph_ens_member_info(const sph_ens_t* e, uint32_t k, sph_info_t* out) {
     for (int a = 0; a < 3; ++a) out->num[a] = S.num[a];
     out->periodic = S.periodic;
     out->dt = m.dt;
-    out->t_current = m.t_cur + (double)h->steps_done * m.dt;  // (sph_ens_get(SPH_TIME) has the exact sum)
+    out->t_current = m.t_cur;  // time.current of the member, accumulated on the device
+    if (hipStreamSynchronize(h->stream) != hipSuccess ||
+        hipMemcpy(&out->t_current, h->d.t_cur + k, 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return SPH_E_HIP;
     out->t_total = m.t_total; out->file_dt = m.file_dt;
     out->steps_done = h->steps_done;
     out->precision = h->precision;

@@ -112,6 +112,8 @@ def test_members_match_single_handles(search):
             d.step(n)
         for k, (sb, d) in enumerate(zip(ms, singles)):
             errs += [f"after +{n}: {e}" for e in _compare(ens, k, d, _fields(sb))]
+            if ens.member_info(k).t_current != d.info().t_current:
+                errs.append(f"after +{n}: member {k} time {ens.member_info(k).t_current} vs {d.info().t_current}")
         if errs:
             break
     for d in singles:
