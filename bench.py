@@ -15,7 +15,8 @@ a dense 1,048,576-sphere FCC crystal (twelve contacts per sphere), with the
 mixed mode's error against the FP64 path per output; the adjoint on a
 65,536-sphere shear cell; the 262,144-sphere bonded solid (about 1.5M
 bonds) over 5,000 steps; and the 512-member calibration ensemble stepped in
-one handle against the same members one at a time.  Per-kernel time comes
+one handle against the same members one at a time; and the reference's
+analysis members (porosity, pair overlaps) on the dense crystal.  Per-kernel time comes
 from hipEvents recorded on the simulation's own stream around each stage,
 over a separate run of the same steps; the roofline entry is for the stage
 with the largest share of the step.
@@ -228,6 +229,60 @@ def bonded_run(tmp, steps, warmup, prof_steps):
                         if stage_ms["bonds"] > 0 else None}}
 
 
+def analysis_run(dpath, reps=5, with_cpu=True):
+    """DEM::porosity and DEM::findOverlaps (src/sphere.cpp:363-604) on the
+    dense crystal after 20 steps: wall-clock per call (device work, the
+    pair download and the call's buffer allocations), and the serial oracle
+    on the same porosity and on the pair search of a 16,384-sphere crystal."""
+    from sphere_amd import configs
+    from sphere_amd.dem import DEM
+
+    dem = DEM(dpath, verbosity=0)
+    n = dem.np
+    dem.step(20)
+    out = {"workload": "1,048,576-sphere FCC crystal after 20 steps (sorted on the device)"}
+    for slices, exact in ((10, False), (10, True), (1000, True)):
+        dem.porosity(slices, exact=exact)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dem.porosity(slices, exact=exact)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[f"porosity_{slices}_{'exact' if exact else 'reference'}_ms"] = ms
+    dem.findOverlaps()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ij, dn = dem.findOverlaps()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["overlaps"] = int(len(ij))
+    out["find_overlaps_ms"] = ms
+    out["find_overlaps_pairs_per_s"] = len(ij) / (ms * 1e-3)
+    dem.close()
+    log(f"analysis: porosity(10) {out['porosity_10_reference_ms']:.2f} ms, find_overlaps {ms:.2f} ms "
+        f"({len(ij):,} pairs)")
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import Oracle
+
+        orc = Oracle(dpath)
+        t0 = time.perf_counter()
+        orc.porosity(10, 0)
+        out["cpu_porosity_10_reference_ms"] = (time.perf_counter() - t0) * 1e3
+        orc.close()
+        small = configs.dense_lattice(cells=(16, 16, 16), sid="dense-small")
+        spath = os.path.join(os.path.dirname(dpath), "dense-small.bin")
+        small.writebin(spath)
+        orc = Oracle(spath)
+        t0 = time.perf_counter()
+        npairs = len(orc.find_overlaps()[0])
+        secs = time.perf_counter() - t0
+        orc.close()
+        out["cpu_find_overlaps"] = {"n": small.np, "pairs": npairs, "seconds": secs,
+                                    "note": "the reference's O(N^2) double loop, serial, one core"}
+        log(f"analysis cpu: porosity(10) {out['cpu_porosity_10_reference_ms']:.1f} ms at {n:,}; "
+            f"find_overlaps {secs:.2f} s at {small.np:,}")
+    return out
+
+
 def ensemble_run(members, steps, warmup, sample_every=32, sample_steps=100):
     """The calibration ensemble in one handle (one launch of each kernel per
     step for every member, graph-replayed), mixed precision, grid and block
@@ -298,6 +353,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-bonded", action="store_true")
+    ap.add_argument("--no-analysis", action="store_true")
     ap.add_argument("--bonded-steps", type=int, default=5000)
     ap.add_argument("--ensemble-members", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=1000)
@@ -392,6 +448,8 @@ def main():
                               "error_vs_fp64": mixed_error(d64, dmx, float(sbd.L[0] * sbd.L[1]),
                                                            float(sbd.radius.min()))}
         result["dense"] = dense
+        if not args.no_analysis:
+            result["analysis"] = analysis_run(dpath, with_cpu=not args.no_cpu)
     result["adjoint_forward_ratio"] = None
     if not args.no_adjoint:
         result["adjoint"] = adjoint_run(tmp, args.adjoint_steps)

@@ -229,6 +229,34 @@ int sph_adjoint_backward(sph_t* h, double grad[4]);
 int sph_pack_rsa(uint64_t n, const double* radii, const double* box, int periodic, uint64_t seed,
                  uint64_t max_tries, double* xyz);
 
+/* ------------------------------------------------------- host analysis
+ * The reference's analysis members, computed on the device from the
+ * handle's current state (after sph_open or any sph_step). */
+
+/* DEM::porosity (src/sphere.cpp:363-483, the `porosity` tool of
+ * src/porosity.cpp): porosity of `slices` horizontal slices of equal
+ * thickness (top - origo_z) / slices from z = 0, top being wall 0's current
+ * position when the input has walls, else L_z; z_pos[k] is the slice's mid
+ * height.  rule SPH_POROSITY_REFERENCE removes what the reference removes,
+ * including V_sphere + cap (not V_sphere - cap) for a sphere whose centre is
+ * in the slice but which crosses a slice plane, and refuses slices no
+ * thicker than twice the last particle's radius (the reference's r_max);
+ * SPH_POROSITY_EXACT removes the exact volume of every sphere inside the
+ * slice and has no thickness limit.  SPH_E_INVALID for slices < 1 or the
+ * refusal (message "Error! The number of z-slices is too high."). */
+#define SPH_POROSITY_REFERENCE 0
+#define SPH_POROSITY_EXACT 1
+int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, double* porosity);
+
+/* DEM::findOverlaps (src/sphere.cpp:553-604, used by the `forcechains`
+ * tool): every pair of particles i < j (original indices) whose centres are
+ * closer than r_i + r_j, periodic images not considered, ordered by i then
+ * j; delta_n = |x_j - x_i| - (r_i + r_j) < 0.  *count receives the number of
+ * pairs.  ij[2k], ij[2k+1] and delta_n[k] are written when capacity >=
+ * *count; with ij or delta_n NULL (a size query) nothing else is written.
+ * SPH_E_INVALID when the buffers are given but too small. */
+int sph_find_overlaps(sph_t* h, uint32_t* ij, double* delta_n, uint64_t capacity, uint64_t* count);
+
 /* ------------------------------------------------------------ ensembles
  * No counterpart in the reference, which steps one DEM per process: an
  * ensemble steps many independent systems (each a .bin image with its own

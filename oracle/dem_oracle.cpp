@@ -984,3 +984,223 @@ extern "C" void orc_get_params(const orc_t* s, double* o) {
 extern "C" void orc_set_params(orc_t* s, const double* p) {
     s->k_n = p[0]; s->k_t = p[1]; s->gamma_n = p[2]; s->gamma_t = p[3]; s->mu_s = p[4]; s->mu_d = p[5];
 }
+
+// ------------------------------------------------------------ host analysis
+namespace {
+
+// sphericalCap, src/sphere.cpp:343-347: volume of a cap of height hc.
+double cap_volume(double hc, double r) { return M_PI * hc * hc / 3.0 * (3.0 * r - hc); }
+
+// Volume of the sphere (centre zc, radius r) between the planes z = lo and
+// z = hi, integrated exactly: pi (r^2 t - t^3 / 3) from t1 to t2.
+double slab_volume(double zc, double r, double lo, double hi) {
+    const double t1 = std::max(lo - zc, -r), t2 = std::min(hi - zc, r);
+    if (!(t2 > t1)) return 0.0;
+    return M_PI * (r * r * (t2 - t1) - (t2 * t2 * t2 - t1 * t1 * t1) / 3.0);
+}
+
+}  // namespace
+
+// DEM::porosity, src/sphere.cpp:363-483.  rule 0 restates the reference's
+// branches as they are (a sphere whose centre lies inside the slice but which
+// crosses one of its planes removes V_sphere + cap, src/sphere.cpp:430-432
+// and :445-447); rule 1 removes the exact volume of the sphere inside the
+// slice.  Slices start at z = 0 (src/sphere.cpp:401) and have the thickness
+// (top - origo_z) / slices, top being wall 0's position when there are
+// walls, else L_z (src/sphere.cpp:367-374).  rule 0 refuses slices no
+// thicker than twice the radius that DEM::r_max returns, which is the last
+// particle's (src/sphere.cpp:350-360 and :379-383).  Returns 0, or 1 for
+// that refusal, 2 for slices < 1.
+extern "C" int orc_porosity(const orc_t* s, int slices, int rule, double* z_pos, double* porosity) {
+    if (slices < 1) return 2;
+    const double top = s->nw > 0 ? s->wnx[0].w : s->L[2];
+    const double h_slice = (top - s->origo[2]) / (double)slices;
+    const double r_last = s->np ? s->x[s->np - 1].w : 0.0;
+    if (rule == 0 && h_slice <= r_last * 2.0) return 1;
+    const double V_slice = h_slice * s->L[0] * s->L[1];
+    for (int iz = 0; iz < slices; ++iz) {
+        double V_void = V_slice;
+        const double lo = iz * h_slice, hi = lo + h_slice;
+        for (uint32_t i = 0; i < s->np; ++i) {
+            const double zc = s->x[i].z, r = s->x[i].w;
+            if (rule != 0) {
+                V_void -= slab_volume(zc, r, lo, hi);
+                continue;
+            }
+            const double zl = zc - r, zh = zc + r;
+            const double Vs = 4.0 / 3.0 * M_PI * r * r * r;
+            if (lo <= zl && zh <= hi) {
+                V_void -= Vs;
+            } else {
+                if (lo >= zc && lo <= zh)
+                    V_void -= cap_volume(zh - lo, r);
+                else if (lo < zc && lo > zl)
+                    V_void -= Vs + cap_volume(lo - zl, r);
+                if (hi <= zc && hi >= zl)
+                    V_void -= cap_volume(hi - zl, r);
+                else if (hi > zc && hi < zh)
+                    V_void -= Vs + cap_volume(zh - hi, r);
+            }
+        }
+        z_pos[iz] = lo + 0.5 * h_slice;
+        porosity[iz] = V_void / V_slice;
+    }
+    return 0;
+}
+
+// DEM::findOverlaps, src/sphere.cpp:553-604: every pair i < j (original
+// indices, i major) whose centre distance is below r_i + r_j, with no
+// periodic images; delta_n = |x_j - x_i| - (r_i + r_j) < 0.  Writes at most
+// `capacity` pairs and returns the number found.
+extern "C" uint64_t orc_find_overlaps(const orc_t* s, uint32_t* ij, double* delta_n, uint64_t capacity) {
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < s->np; ++i) {
+        const D4 a = s->x[i];
+        for (uint32_t j = i + 1; j < s->np; ++j) {
+            const D4 b = s->x[j];
+            const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+            const double delta = std::sqrt(dx * dx + dy * dy + dz * dz) - (a.w + b.w);
+            if (delta < 0.0) {
+                if (n < capacity) {
+                    ij[2 * n] = i;
+                    ij[2 * n + 1] = j;
+                    delta_n[n] = delta;
+                }
+                ++n;
+            }
+        }
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------- raytracer
+// DEM::render (src/raytracer.cuh:14-553): the camera of cameraInit
+// (:281-333) and render (:374-420), the rays of rayInitPerspective (:27-51)
+// and the per-pixel loops of rayIntersectSpheres (:55-137) and
+// rayIntersectSpheresColormap (:140-278), all in FP32 as there.  The vector
+// helpers follow helper_math's host definitions (a third-party header the
+// reference includes, src/darcy.cuh:11): component-wise float3 arithmetic,
+// length = sqrtf(dot), normalize = v * (1 / sqrtf(dot)), division by a
+// scalar per component.
+namespace {
+
+struct F3 {
+    float x, y, z;
+};
+F3 f3(float x, float y, float z) { return {x, y, z}; }
+F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+F3 operator-(F3 a) { return {-a.x, -a.y, -a.z}; }
+F3 operator*(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+F3 operator*(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+F3 operator/(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+float fdot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+float flength(F3 a) { return std::sqrt(fdot(a, a)); }
+F3 fnormalize(F3 a) { return a * (1.0f / std::sqrt(fdot(a, a))); }
+F3 fcross(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// colorbar.h:7-22
+float cb_red(float r) { return std::fmin(1.0f, 0.209f * r * r * r - 2.49f * r * r + 3.0f * r + 0.0109f); }
+float cb_green(float r) { return std::fmin(1.0f, -2.44f * r * r + 2.15f * r + 0.369f); }
+float cb_blue(float r) { return std::fmin(1.0f, -2.21f * r * r + 1.61f * r + 0.573f); }
+
+double vnorm(const V3& v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+}  // namespace
+
+// rgb: width * height * 3 bytes in the reference's pixel order.  Returns 0,
+// or 2 for an unknown method or an empty image.
+extern "C" int orc_render(const orc_t* s, int method, float max_val, float lower_cutoff, uint32_t width,
+                          uint32_t height, uint8_t* rgb) {
+    if (method < 0 || method > 5 || width == 0 || height == 0) return 2;
+    const uint32_t np = s->np;
+    // camera (render, src/raytracer.cuh:409-418; cameraInit, :281-313)
+    const float imgw = (float)(s->L[0] * 1.35f);
+    double mx[3] = {0.0, 0.0, 0.0};  // DEM::maxPos, src/sphere.cpp:520-551
+    for (uint32_t i = 0; i < np; ++i) {
+        mx[0] = std::max(mx[0], s->x[i].x);
+        mx[1] = std::max(mx[1], s->x[i].y);
+        mx[2] = std::max(mx[2], s->x[i].z);
+    }
+    const F3 lookat = f3((float)mx[0], (float)mx[1], (float)mx[2]) / 2.0f;
+    const F3 eye = f3((float)(s->L[0] * 2.3f), (float)(s->L[1] * -5.0f), (float)(s->L[2] * 1.3f));
+    const float hw_ratio = (float)(height / width);
+    const float pl = -0.5f * imgw, pr = 0.5f * imgw, pb = -0.5f * imgw * hw_ratio, pt = 0.5f * imgw * hw_ratio;
+    const F3 view = eye - lookat;
+    const F3 up = f3(0.0f, 0.0f, 1.0f);
+    const F3 w = -view / flength(view);
+    const F3 u = fcross(up, w) / flength(fcross(up, w));
+    const F3 v = fcross(w, u);
+    const float dfoc = flength(view) * 0.8f;
+    const F3 light = fnormalize(-1.0f * eye * f3(1.0f, 0.2f, 0.6f));
+    // per-particle colour values (render, :445-503), in FP64 as there
+    std::vector<double> lin(np, 0.0);
+    for (uint32_t i = 0; i < np; ++i) {
+        switch (method) {
+            case 1: lin[i] = s->p[i]; break;
+            case 2: lin[i] = std::sqrt(s->vel[i].x * s->vel[i].x + s->vel[i].y * s->vel[i].y + s->vel[i].z * s->vel[i].z); break;
+            case 3: lin[i] = vnorm(s->angvel[i]); break;
+            case 4: lin[i] = s->xysum[i].x; break;
+            case 5: lin[i] = vnorm(s->angpos[i]); break;
+            default: break;
+        }
+    }
+    const uint64_t pixels = (uint64_t)width * height;
+    for (uint64_t m = 0; m < pixels; ++m) {
+        const uint32_t mempos = (uint32_t)m;
+        const uint32_t i = mempos % width;
+        const uint32_t j = (uint32_t)(int)std::floor((float)mempos / width) % width;
+        const float p_u = pl + (pr - pl) * (i + 0.5f) / width;
+        const float p_v = pb + (pt - pb) * (j + 0.5f) / height;
+        const F3 e = eye;
+        const F3 d = -dfoc * w + p_u * u + p_v * v;
+        float tdist = 1e10f;
+        F3 n = f3(0, 0, 0);
+        uint32_t hit = 0;
+        for (uint32_t k = 0; k < np; ++k) {
+            const F3 c = f3((float)s->x[k].x, (float)s->x[k].y, (float)s->x[k].z);
+            const float R = (float)s->x[k].w;
+            const float Delta = (2.0f * fdot(d, (e - c))) * (2.0f * fdot(d, (e - c))) -
+                                4.0f * fdot(d, d) * (fdot((e - c), (e - c)) - R * R);
+            if (method != 0 && lower_cutoff > 0.0) {
+                const float val = (float)lin[k];
+                const float fixvel = (float)s->vel[k].w;
+                if (!(Delta > 0.0f && val > lower_cutoff && fixvel == 0.f)) continue;
+            } else if (!(Delta > 0.0f)) {
+                continue;
+            }
+            const float t_minus = ((fdot(-d, (e - c)) -
+                                    std::sqrt(fdot(d, (e - c)) * fdot(d, (e - c)) -
+                                              fdot(d, d) * (fdot((e - c), (e - c)) - R * R))) /
+                                   fdot(d, d));
+            if (std::fabs(t_minus) < tdist) {
+                const F3 p = e + t_minus * d;
+                tdist = std::fabs(t_minus);
+                n = fnormalize(2.0f * (p - c));
+                hit = k;
+            }
+        }
+        uint8_t* px = rgb + 3 * m;
+        px[0] = px[1] = px[2] = 255;  // imageInit, :14-24
+        if (!(tdist < 1e10f)) continue;
+        const float dotprod = std::fmax(0.0f, fdot(n, light));
+        const float I_d = 40.0f, k_d = 5.0f, k_a = 10.0f, I_a = 5.0f;
+        if (method == 0) {
+            px[0] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * 0.48f);
+            px[1] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * 0.41f);
+            px[2] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * 0.27f);
+            continue;
+        }
+        const float fixvel = (float)s->vel[hit].w;
+        float ratio = (float)(lin[hit] / max_val);
+        if (ratio < 0.01f) ratio = 0.01f;
+        if (ratio > 0.99f) ratio = 0.99f;
+        float redv = cb_red(ratio), greenv = cb_green(ratio), bluev = cb_blue(ratio);
+        if (fixvel > 0.f) redv = greenv = bluev = 0.5f;
+        px[0] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * redv);
+        px[1] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * greenv);
+        px[2] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * bluev);
+    }
+    return 0;
+}

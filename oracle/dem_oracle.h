@@ -34,6 +34,17 @@
  *                                  equilibrium
  *   whole step ................... zero-g no-contact bit invariance and the
  *                                  .bin round trip (tests/io_tests.py:8-34)
+ *   porosity (both rules) ........ a sphere inside one slice, textbook cap
+ *                                  volumes for each crossing branch, the
+ *                                  exact rule's slice sum = total volume,
+ *                                  Monte Carlo on a random packing
+ *   findOverlaps ................. scipy's cKDTree pair query on a random
+ *                                  packing (i < j, no periodic images)
+ *   render ....................... ray-sphere hits and the shading of the
+ *                                  chosen surface point from the textbook
+ *                                  quadratic in FP64, occlusion order,
+ *                                  cutoff and fixed-sphere colours, the
+ *                                  integer height / width of the image plane
  */
 #ifndef SPHERE_DEM_ORACLE_H_
 #define SPHERE_DEM_ORACLE_H_
@@ -125,6 +136,18 @@ void orc_set_params(orc_t* s, const double* in6);
 
 /* The exponential the capillary force uses (see dem_oracle.cpp). */
 double orc_exp(double x);
+
+/* DEM::porosity: rule 0 as the reference computes it, 1 exact slab volumes.
+ * Returns 0, 1 when the slices are too thin for rule 0, 2 for slices < 1. */
+int orc_porosity(const orc_t* s, int slices, int rule, double* z_pos, double* porosity);
+/* DEM::findOverlaps: pairs (i, j), i < j, i major; returns the count and
+ * writes at most `capacity` pairs. */
+uint64_t orc_find_overlaps(const orc_t* s, uint32_t* ij, double* delta_n, uint64_t capacity);
+/* DEM::render: rgb receives width * height * 3 bytes; method 0 shaded
+ * spheres, 1..5 colour map of pressure, |vel|, |angvel|, x displacement,
+ * |angpos|.  Returns 0, or 2 for a bad method or an empty image. */
+int orc_render(const orc_t* s, int method, float max_val, float lower_cutoff, uint32_t width, uint32_t height,
+               uint8_t* rgb);
 
 #ifdef __cplusplus
 }

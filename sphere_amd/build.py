@@ -26,7 +26,9 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["kernels.hip", "engine.hip", "adjoint.hip", "binio.cpp", "generate.cpp"]
+SOURCES = ["kernels.hip", "engine.hip", "adjoint.hip", "analysis.hip", "binio.cpp", "generate.cpp"]
+# command-line front ends, each linked against the library next to it
+TOOLS = {"sphere": "sphere_main.cpp", "porosity": "porosity_main.cpp", "forcechains": "forcechains_main.cpp"}
 
 
 def _compile(src: str) -> str:
@@ -51,15 +53,16 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
-    # command-line front end, linked against the library next to it
-    cmd = [HIPCC, *COMMON, "-o", CLI + ".tmp", os.path.join(CSRC, "sphere_main.cpp"),
-           "-L" + HERE, "-lsphere_dem", "-Wl,-rpath,$ORIGIN"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(CLI + ".tmp", CLI)
+    for tool, src in TOOLS.items():
+        exe = os.path.join(HERE, tool)
+        cmd = [HIPCC, *COMMON, "-o", exe + ".tmp", os.path.join(CSRC, src),
+               "-L" + HERE, "-lsphere_dem", "-Wl,-rpath,$ORIGIN"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"{tool} build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+        os.replace(exe + ".tmp", exe)
     if verbose:
-        print(f"built {LIB} and {CLI}")
+        print(f"built {LIB} and {', '.join(os.path.join(HERE, t) for t in TOOLS)}")
     return LIB
 
 

@@ -1,0 +1,539 @@
+// analysis.hip -- the reference's host analysis members, computed on the
+// device from the handle's resident state: DEM::porosity
+// (src/sphere.cpp:363-483) and DEM::findOverlaps (src/sphere.cpp:553-604).
+//
+// Both read the current buffer set (positions and radii in x, original
+// index in id) in whatever order the last step left it.  The reference loops
+// over every particle for every slice and over every pair; here the porosity
+// visits only the slices a block's spheres can reach, and the overlap search
+// bins the spheres into cells of 2 r_max and scans the 27 neighbouring cells,
+// then orders the pairs by (i, j) as the reference's double loop emits them.
+// Distances and volumes use the reference's expressions with FP contraction
+// off, so every delta_n is bit-identical to the double loop's; porosity sums
+// are taken in a fixed order (per-block tree, then blocks in order), which
+// differs from the reference's running subtraction only by rounding.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_internal.h"
+#include "../../include/sphere_dem.h"
+
+namespace {
+
+constexpr int ABLOCK = 256;
+constexpr int PPT = 8;  // particles per thread of k_porosity
+constexpr int SCAN_CHUNK = 4 * ABLOCK;
+
+#define A_TRY(expr)                                                                \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+            return SPH_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+const sph::PState& cur(const sph_dem* h) { return h->own_cur ? h->d.a : h->d.b; }
+
+// Device buffers of one call, freed on every return path.
+struct Scratch {
+    std::vector<void*> p;
+    ~Scratch() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <class T>
+    hipError_t get(T** out, size_t count) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) p.push_back(q);
+        *out = static_cast<T*>(q);
+        return e;
+    }
+};
+
+// ------------------------------------------------------------------ bounds
+// Per block: min x, y, z, max x, y, z, max r.  The radius of original index
+// n - 1 goes to *r_last.
+__global__ __launch_bounds__(ABLOCK) void k_bounds(const double4* __restrict__ x, const uint32_t* __restrict__ id,
+                                                   uint32_t n, double* __restrict__ part, double* __restrict__ r_last) {
+    __shared__ double red[7][ABLOCK];
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (uint32_t i = blockIdx.x * ABLOCK + threadIdx.x; i < n; i += gridDim.x * ABLOCK) {
+        const double4 p = x[i];
+        v[0] = fmin(v[0], p.x); v[1] = fmin(v[1], p.y); v[2] = fmin(v[2], p.z);
+        v[3] = fmax(v[3], p.x); v[4] = fmax(v[4], p.y); v[5] = fmax(v[5], p.z);
+        v[6] = fmax(v[6], p.w);
+        // a non-finite coordinate poisons the bounds so that the host refuses
+        if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z) || !isfinite(p.w)) v[6] = NAN;
+        if (id[i] == n - 1) *r_last = p.w;
+    }
+    for (int c = 0; c < 7; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int w = ABLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = fmin(red[c][threadIdx.x], red[c][threadIdx.x + w]);
+            for (int c = 3; c < 6; ++c) red[c][threadIdx.x] = fmax(red[c][threadIdx.x], red[c][threadIdx.x + w]);
+            const double a = red[6][threadIdx.x], b = red[6][threadIdx.x + w];
+            red[6][threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmax(a, b);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[7 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// bounds[7] and the last particle's radius, on the host
+int bounds(sph_dem* h, const sph::PState& st, uint32_t n, Scratch& s, double out[7], double* r_last) {
+    const uint32_t nblk = std::max<uint32_t>(1, std::min<uint32_t>(1024, (n + ABLOCK - 1) / ABLOCK));
+    double *part, *rl;
+    A_TRY(s.get(&part, 7 * (size_t)nblk));
+    A_TRY(s.get(&rl, 1));
+    const double zero = 0.0;
+    A_TRY(hipMemcpyAsync(rl, &zero, 8, hipMemcpyHostToDevice, h->stream));
+    k_bounds<<<nblk, ABLOCK, 0, h->stream>>>(st.x, st.id, n, part, rl);
+    A_TRY(hipGetLastError());
+    std::vector<double> hp(7 * (size_t)nblk);
+    A_TRY(hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipMemcpyAsync(r_last, rl, 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < 7; ++c) out[c] = hp[c];
+    for (uint32_t b = 1; b < nblk; ++b) {
+        for (int c = 0; c < 3; ++c) out[c] = std::min(out[c], hp[7 * b + c]);
+        for (int c = 3; c < 6; ++c) out[c] = std::max(out[c], hp[7 * b + c]);
+        out[6] = (std::isnan(out[6]) || std::isnan(hp[7 * b + 6])) ? NAN : std::max(out[6], hp[7 * b + 6]);
+    }
+    return SPH_OK;
+}
+
+// ---------------------------------------------------------------- porosity
+// sphericalCap, src/sphere.cpp:343-347
+__device__ inline double cap_volume(double hc, double r) { return M_PI * hc * hc / 3.0 * (3.0 * r - hc); }
+
+// Exact volume of the sphere between the planes lo and hi:
+// pi (r^2 t - t^3 / 3) from t1 to t2, t measured from the centre.
+__device__ inline double slab_volume(double zc, double r, double lo, double hi) {
+    const double t1 = fmax(lo - zc, -r), t2 = fmin(hi - zc, r);
+    if (!(t2 > t1)) return 0.0;
+    return M_PI * (r * r * (t2 - t1) - (t2 * t2 * t2 - t1 * t1 * t1) / 3.0);
+}
+
+// Volume one sphere removes from slice [lo, hi]: the reference's branches
+// (src/sphere.cpp:408-454, each subtraction a separate term) or the exact
+// slab volume.
+__device__ inline double removed(double zc, double r, double lo, double hi, int rule) {
+    if (rule != SPH_POROSITY_REFERENCE) return slab_volume(zc, r, lo, hi);
+    const double zl = zc - r, zh = zc + r;
+    const double Vs = 4.0 / 3.0 * M_PI * r * r * r;
+    if (lo <= zl && zh <= hi) return Vs;
+    double acc = 0.0;
+    if (lo >= zc && lo <= zh)
+        acc += cap_volume(zh - lo, r);
+    else if (lo < zc && lo > zl)
+        acc += Vs + cap_volume(lo - zl, r);
+    if (hi <= zc && hi >= zl)
+        acc += cap_volume(hi - zl, r);
+    else if (hi > zc && hi < zh)
+        acc += Vs + cap_volume(zh - hi, r);
+    return acc;
+}
+
+// One block: ABLOCK * PPT spheres, each thread PPT of them (strided, so the
+// loads coalesce).  Every branch above needs lo <= z + r and hi >= z - r, so
+// a sphere reaches at most slices floor((z - r) / h) - 1 .. floor((z + r) / h)
+// + 1 (two of margin each side; the branch tests themselves decide).  The
+// block visits the union of its spheres' ranges and writes each slice's
+// block sum to part[slice * gridDim.x + block].
+__global__ __launch_bounds__(ABLOCK) void k_porosity(const double4* __restrict__ x, uint32_t n, int slices,
+                                                     double h_slice, int rule, double* __restrict__ part) {
+    __shared__ double red[ABLOCK];
+    __shared__ int range[2];
+    double zc[PPT], rr[PPT];
+    int lo_s = INT_MAX, hi_s = -1;
+    const uint32_t base = blockIdx.x * (uint32_t)(ABLOCK * PPT) + threadIdx.x;
+    for (int k = 0; k < PPT; ++k) {
+        const uint32_t i = base + (uint32_t)k * ABLOCK;
+        zc[k] = 0.0;
+        rr[k] = -1.0;  // no sphere
+        if (i < n) {
+            const double4 p = x[i];
+            zc[k] = p.z;
+            rr[k] = p.w;
+            const double a = fmax(floor((p.z - p.w) / h_slice) - 2.0, 0.0);
+            const double b = fmin(floor((p.z + p.w) / h_slice) + 2.0, (double)(slices - 1));
+            if (a <= b) {
+                lo_s = min(lo_s, (int)a);
+                hi_s = max(hi_s, (int)b);
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        range[0] = INT_MAX;
+        range[1] = -1;
+    }
+    __syncthreads();
+    if (hi_s >= 0) {
+        atomicMin(&range[0], lo_s);
+        atomicMax(&range[1], hi_s);
+    }
+    __syncthreads();
+    const int s0 = range[0], s1 = range[1];
+    for (int s = s0; s <= s1; ++s) {
+        const double lo = s * h_slice, hi = lo + h_slice;
+        double acc = 0.0;
+        for (int k = 0; k < PPT; ++k)
+            if (rr[k] >= 0.0) acc += removed(zc[k], rr[k], lo, hi, rule);
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = ABLOCK / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(size_t)s * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(ABLOCK) void k_porosity_final(const double* __restrict__ part, uint32_t nblk, int slices,
+                                                           double h_slice, double V_slice, double* __restrict__ z_pos,
+                                                           double* __restrict__ por) {
+    const int s = blockIdx.x * ABLOCK + threadIdx.x;
+    if (s >= slices) return;
+    double sum = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) sum += part[(size_t)s * nblk + b];
+    const double lo = s * h_slice;
+    z_pos[s] = lo + 0.5 * h_slice;
+    por[s] = (V_slice - sum) / V_slice;
+}
+
+__global__ __launch_bounds__(ABLOCK) void k_zero_f64(double* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)ABLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * ABLOCK) p[i] = 0.0;
+}
+
+__global__ __launch_bounds__(ABLOCK) void k_zero_u32(uint32_t* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)ABLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * ABLOCK) p[i] = 0u;
+}
+
+uint32_t fill_blocks(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>(4096, (n + ABLOCK - 1) / ABLOCK)); }
+
+// --------------------------------------------------------- exclusive scan
+// out[0..n) = exclusive prefix sums of in[0..n) (u32), in three launches:
+// per-chunk scans with chunk totals, a scan of the totals, the add-back.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < ABLOCK; off <<= 1) {
+        const uint32_t t = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const uint32_t incl = sh[threadIdx.x];
+    *total = sh[ABLOCK - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(ABLOCK) void k_scan_chunks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                        size_t n, uint32_t* __restrict__ chunk_sum) {
+    __shared__ uint32_t sh[ABLOCK];
+    const size_t base = (size_t)blockIdx.x * SCAN_CHUNK + 4 * (size_t)threadIdx.x;
+    uint32_t v[4], t = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        t += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(t, sh, &total);
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) chunk_sum[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the chunk totals in place
+__global__ __launch_bounds__(ABLOCK) void k_scan_sums(uint32_t* __restrict__ sums, uint32_t m) {
+    __shared__ uint32_t sh[ABLOCK];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < m; b0 += ABLOCK) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < m ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, sh, &total);
+        if (i < m) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(ABLOCK) void k_scan_add(uint32_t* __restrict__ out, size_t n,
+                                                     const uint32_t* __restrict__ chunk_sum) {
+    const size_t base = (size_t)blockIdx.x * SCAN_CHUNK + 4 * (size_t)threadIdx.x;
+    const uint32_t add = chunk_sum[blockIdx.x];
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+int exclusive_scan(sph_dem* h, const uint32_t* in, uint32_t* out, size_t n, Scratch& s) {
+    const size_t chunks = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    uint32_t* sums;
+    A_TRY(s.get(&sums, chunks));
+    k_scan_chunks<<<(uint32_t)chunks, ABLOCK, 0, h->stream>>>(in, out, n, sums);
+    k_scan_sums<<<1, ABLOCK, 0, h->stream>>>(sums, (uint32_t)chunks);
+    k_scan_add<<<(uint32_t)chunks, ABLOCK, 0, h->stream>>>(out, n, sums);
+    A_TRY(hipGetLastError());
+    return SPH_OK;
+}
+
+// ---------------------------------------------------------------- overlaps
+// Cells of edge c >= 2 r_max (1 + 1e-9) from the lower corner of the
+// bounding box; a pair closer than r_i + r_j differs by less than one cell
+// edge on every axis, so its cells are neighbours even after the rounding
+// of (x - min) / c (relative 2^-52 on quotients below 2^20).
+struct OGrid {
+    double mn[3];
+    double c;
+    uint32_t nd[3];
+};
+
+__device__ inline uint32_t ocell_axis(double v, double mn, double c, uint32_t nd) {
+    const double q = floor((v - mn) / c);
+    return q >= (double)(nd - 1) ? nd - 1 : (q > 0.0 ? (uint32_t)q : 0u);
+}
+
+__device__ inline uint32_t ocell(const double4& p, const OGrid& g, uint32_t* ix, uint32_t* iy, uint32_t* iz) {
+    *ix = ocell_axis(p.x, g.mn[0], g.c, g.nd[0]);
+    *iy = ocell_axis(p.y, g.mn[1], g.c, g.nd[1]);
+    *iz = ocell_axis(p.z, g.mn[2], g.c, g.nd[2]);
+    return (*iz * g.nd[1] + *iy) * g.nd[0] + *ix;
+}
+
+__global__ __launch_bounds__(ABLOCK) void k_ocount_cells(const double4* __restrict__ x, uint32_t n, OGrid g,
+                                                         uint32_t* __restrict__ cell_of, uint32_t* __restrict__ ccount) {
+    const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+    if (i >= n) return;
+    uint32_t ix, iy, iz;
+    const uint32_t c = ocell(x[i], g, &ix, &iy, &iz);
+    cell_of[i] = c;
+    atomicAdd(&ccount[c], 1u);
+}
+
+// Bins the spheres by cell (any order inside a cell): sorted position and
+// original index.
+__global__ __launch_bounds__(ABLOCK) void k_oscatter(const double4* __restrict__ x, const uint32_t* __restrict__ id,
+                                                     uint32_t n, const uint32_t* __restrict__ cell_of,
+                                                     uint32_t* __restrict__ cursor, double4* __restrict__ sx,
+                                                     uint32_t* __restrict__ sid) {
+    const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t q = atomicAdd(&cursor[cell_of[i]], 1u);
+    sx[q] = x[i];
+    sid[q] = id[i];
+}
+
+// For the sphere at binned position p (original index i): the partners
+// j > i closer than r_i + r_j.  EMIT = false counts them into cnt[i],
+// EMIT = true writes (i, j) and delta_n from off[i] on, in scan order.
+template <bool EMIT>
+__global__ __launch_bounds__(ABLOCK) void k_opairs(const double4* __restrict__ sx, const uint32_t* __restrict__ sid,
+                                                   uint32_t n, OGrid g, const uint32_t* __restrict__ cstart,
+                                                   uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                                                   uint2* __restrict__ pairs, double* __restrict__ delta_n) {
+    const uint32_t p = blockIdx.x * ABLOCK + threadIdx.x;
+    if (p >= n) return;
+    const double4 a = sx[p];
+    const uint32_t ia = sid[p];
+    uint32_t ix, iy, iz;
+    (void)ocell(a, g, &ix, &iy, &iz);
+    uint32_t found = 0, w = EMIT ? off[ia] : 0u;
+    const uint32_t z0 = iz ? iz - 1 : 0, z1 = min(iz + 1, g.nd[2] - 1);
+    const uint32_t y0 = iy ? iy - 1 : 0, y1 = min(iy + 1, g.nd[1] - 1);
+    const uint32_t x0 = ix ? ix - 1 : 0, x1 = min(ix + 1, g.nd[0] - 1);
+    for (uint32_t cz = z0; cz <= z1; ++cz)
+        for (uint32_t cy = y0; cy <= y1; ++cy) {
+            const uint32_t row = (cz * g.nd[1] + cy) * g.nd[0];
+            const uint32_t q0 = cstart[row + x0], q1 = cstart[row + x1 + 1];
+            for (uint32_t q = q0; q < q1; ++q) {
+                const uint32_t ib = sid[q];
+                if (ib <= ia) continue;
+                const double4 b = sx[q];
+                // x_ij = x_j - x_i, src/sphere.cpp:577-585
+                const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+                const double delta = sqrt(dx * dx + dy * dy + dz * dz) - (a.w + b.w);
+                if (delta < 0.0) {
+                    if (EMIT) {
+                        pairs[w] = make_uint2(ia, ib);
+                        delta_n[w] = delta;
+                        ++w;
+                    }
+                    ++found;
+                }
+            }
+        }
+    if (!EMIT) cnt[ia] = found;
+}
+
+// Orders each sphere's pairs by partner index (insertion sort of the
+// segment off[i] .. off[i + 1]; segments hold a sphere's overlaps).
+__global__ __launch_bounds__(ABLOCK) void k_osort(const uint32_t* __restrict__ off, uint32_t n, uint2* __restrict__ pairs,
+                                                  double* __restrict__ delta_n) {
+    const uint32_t i = blockIdx.x * ABLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = off[i], e = off[i + 1];
+    for (uint32_t k = b + 1; k < e; ++k) {
+        const uint2 pk = pairs[k];
+        const double dk = delta_n[k];
+        uint32_t m = k;
+        while (m > b && pairs[m - 1].y > pk.y) {
+            pairs[m] = pairs[m - 1];
+            delta_n[m] = delta_n[m - 1];
+            --m;
+        }
+        pairs[m] = pk;
+        delta_n[m] = dk;
+    }
+}
+
+int check_handle(sph_dem* h, const char* fn) {
+    if (!h) return SPH_E_INVALID;
+    if (h->d.nsys > 1 || !h->members.empty()) {
+        h->err = std::string(fn) + ": not available on an ensemble";
+        return SPH_E_STATE;
+    }
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, double* porosity) {
+    int rc;
+    if ((rc = check_handle(h, "sph_porosity"))) return rc;
+    if (slices < 1 || !z_pos || !porosity || (rule != SPH_POROSITY_REFERENCE && rule != SPH_POROSITY_EXACT)) {
+        h->err = "sph_porosity: need slices >= 1, output arrays and rule 0 or 1";
+        return SPH_E_INVALID;
+    }
+    const sph::HostState& hs = h->hs;
+    const sph::PState& st = cur(h);
+    const uint32_t n = h->d.np;
+    Scratch s;
+    double bnd[7], r_last = 0.0;
+    if (n && (rc = bounds(h, st, n, s, bnd, &r_last))) return rc;
+    double top = hs.L[2];
+    if (hs.nw > 0) {  // wall 0's current position (src/sphere.cpp:367-371)
+        double4 w0;
+        A_TRY(hipMemcpyAsync(&w0, h->d.wnx, sizeof w0, hipMemcpyDeviceToHost, h->stream));
+        A_TRY(hipStreamSynchronize(h->stream));
+        top = w0.w;
+    }
+    const double h_slice = (top - hs.origo[2]) / (double)slices;
+    if (rule == SPH_POROSITY_REFERENCE && h_slice <= r_last * 2.0) {
+        h->err = "Error! The number of z-slices is too high.";
+        return SPH_E_INVALID;
+    }
+    if (!(h_slice > 0.0) || !std::isfinite(h_slice)) {
+        h->err = "sph_porosity: the slices have no positive thickness";
+        return SPH_E_INVALID;
+    }
+    const double V_slice = h_slice * hs.L[0] * hs.L[1];
+    const uint32_t nblk = (n + ABLOCK * PPT - 1) / (ABLOCK * PPT);
+    const size_t parts = (size_t)slices * std::max<uint32_t>(nblk, 1);
+    if (parts > ((size_t)1 << 28)) {
+        h->err = "sph_porosity: slices x particle blocks above 2^28";
+        return SPH_E_INVALID;
+    }
+    double *part, *dz, *dp;
+    A_TRY(s.get(&part, parts));
+    A_TRY(s.get(&dz, (size_t)slices));
+    A_TRY(s.get(&dp, (size_t)slices));
+    k_zero_f64<<<fill_blocks(parts), ABLOCK, 0, h->stream>>>(part, parts);
+    if (nblk) k_porosity<<<nblk, ABLOCK, 0, h->stream>>>(st.x, n, slices, h_slice, rule, part);
+    k_porosity_final<<<(slices + ABLOCK - 1) / ABLOCK, ABLOCK, 0, h->stream>>>(part, nblk, slices, h_slice, V_slice,
+                                                                             dz, dp);
+    A_TRY(hipGetLastError());
+    A_TRY(hipMemcpyAsync(z_pos, dz, (size_t)slices * 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipMemcpyAsync(porosity, dp, (size_t)slices * 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipStreamSynchronize(h->stream));
+    return SPH_OK;
+}
+
+extern "C" int sph_find_overlaps(sph_t* h, uint32_t* ij, double* delta_n, uint64_t capacity, uint64_t* count) {
+    int rc;
+    if ((rc = check_handle(h, "sph_find_overlaps"))) return rc;
+    if (!count) {
+        h->err = "sph_find_overlaps: count is NULL";
+        return SPH_E_INVALID;
+    }
+    *count = 0;
+    const sph::PState& st = cur(h);
+    const uint32_t n = h->d.np;
+    if (n < 2) return SPH_OK;
+    Scratch s;
+    double bnd[7], r_last;
+    if ((rc = bounds(h, st, n, s, bnd, &r_last))) return rc;
+    if (std::isnan(bnd[6]) || !std::isfinite(bnd[0] + bnd[1] + bnd[2] + bnd[3] + bnd[4] + bnd[5])) {
+        h->err = "sph_find_overlaps: non-finite position or radius";
+        return SPH_E_NAN;
+    }
+    if (!(bnd[6] > 0.0)) return SPH_OK;  // no radius, no overlap
+    OGrid g;
+    g.c = 2.0 * bnd[6] * (1.0 + 1e-9);
+    for (;;) {
+        uint64_t K = 1;
+        bool fits = true;
+        for (int a = 0; a < 3; ++a) {
+            g.mn[a] = bnd[a];
+            const double q = std::floor((bnd[3 + a] - bnd[a]) / g.c) + 1.0;
+            if (q > (double)(1u << 20)) fits = false;
+            g.nd[a] = fits ? (uint32_t)q : 1u;
+            K *= g.nd[a];
+        }
+        if (fits && K <= (1u << 22)) break;
+        g.c *= 1.5;  // coarser cells: more candidates per cell, same pairs
+    }
+    const size_t K = (size_t)g.nd[0] * g.nd[1] * g.nd[2];
+    uint32_t *cell_of, *ccount, *cstart, *cursor, *sid, *cnt, *off;
+    double4* sx;
+    A_TRY(s.get(&cell_of, n));
+    A_TRY(s.get(&ccount, K + 1));
+    A_TRY(s.get(&cstart, K + 1));
+    A_TRY(s.get(&cursor, K + 1));
+    A_TRY(s.get(&sx, n));
+    A_TRY(s.get(&sid, n));
+    A_TRY(s.get(&cnt, (size_t)n + 1));
+    A_TRY(s.get(&off, (size_t)n + 1));
+    const uint32_t nb = (n + ABLOCK - 1) / ABLOCK;
+    k_zero_u32<<<fill_blocks(K + 1), ABLOCK, 0, h->stream>>>(ccount, K + 1);
+    k_ocount_cells<<<nb, ABLOCK, 0, h->stream>>>(st.x, n, g, cell_of, ccount);
+    if ((rc = exclusive_scan(h, ccount, cstart, K + 1, s))) return rc;
+    A_TRY(hipMemcpyAsync(cursor, cstart, (K + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
+    k_oscatter<<<nb, ABLOCK, 0, h->stream>>>(st.x, st.id, n, cell_of, cursor, sx, sid);
+    k_zero_u32<<<1, ABLOCK, 0, h->stream>>>(cnt + n, 1);
+    k_opairs<false><<<nb, ABLOCK, 0, h->stream>>>(sx, sid, n, g, cstart, cnt, nullptr, nullptr, nullptr);
+    if ((rc = exclusive_scan(h, cnt, off, (size_t)n + 1, s))) return rc;
+    A_TRY(hipGetLastError());
+    uint32_t total = 0;
+    A_TRY(hipMemcpyAsync(&total, off + n, 4, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipStreamSynchronize(h->stream));
+    *count = total;
+    if (!ij || !delta_n || total == 0) return SPH_OK;
+    if (capacity < total) {
+        h->err = "sph_find_overlaps: " + std::to_string(total) + " pairs, capacity " + std::to_string(capacity);
+        return SPH_E_INVALID;
+    }
+    uint2* pairs;
+    double* dn;
+    A_TRY(s.get(&pairs, total));
+    A_TRY(s.get(&dn, total));
+    k_opairs<true><<<nb, ABLOCK, 0, h->stream>>>(sx, sid, n, g, cstart, nullptr, off, pairs, dn);
+    k_osort<<<nb, ABLOCK, 0, h->stream>>>(off, n, pairs, dn);
+    A_TRY(hipGetLastError());
+    A_TRY(hipMemcpyAsync(ij, pairs, (size_t)total * 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipMemcpyAsync(delta_n, dn, (size_t)total * 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipStreamSynchronize(h->stream));
+    return SPH_OK;
+}

@@ -146,6 +146,8 @@ EXPORTS = {
                                C.c_void_p]),
     "sph_adjoint_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
     "sph_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "sph_porosity": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "sph_find_overlaps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sph_ens_open_mem": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_uint32, C.c_double, C.c_int,
                                    C.POINTER(C.c_void_p)]),
     "sph_ens_open": (C.c_int, [C.POINTER(C.c_char_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
@@ -326,6 +328,28 @@ class DEM:
         g = (C.c_double * 4)()
         self._check(self._lib.sph_adjoint_backward(self._h, g))
         return np.array(list(g), dtype=np.float64)
+
+    def porosity(self, slices: int = 10, exact: bool = False):
+        """DEM::porosity (src/sphere.cpp:363-483): (z_pos, porosity) of
+        `slices` horizontal slices; exact=True removes the exact volume of
+        each sphere inside a slice instead of the reference's rule."""
+        z = np.empty(max(int(slices), 0))
+        p = np.empty_like(z)
+        self._check(self._lib.sph_porosity(self._h, int(slices), 1 if exact else 0,
+                                           z.ctypes.data_as(C.c_void_p), p.ctypes.data_as(C.c_void_p)))
+        return z, p
+
+    def findOverlaps(self):
+        """DEM::findOverlaps (src/sphere.cpp:553-604): (ij, delta_n), the
+        pairs i < j of overlapping spheres (no periodic images), i major."""
+        n = C.c_uint64()
+        self._check(self._lib.sph_find_overlaps(self._h, None, None, 0, C.byref(n)))
+        ij = np.empty((n.value, 2), dtype=np.uint32)
+        dn = np.empty(n.value)
+        if n.value:
+            self._check(self._lib.sph_find_overlaps(self._h, ij.ctypes.data_as(C.c_void_p),
+                                                    dn.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+        return ij, dn
 
     def set_precision(self, prec):
         if isinstance(prec, str):

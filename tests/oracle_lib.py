@@ -57,6 +57,10 @@ def lib():
         L.orc_set_params.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.orc_exp.restype = C.c_double
         L.orc_exp.argtypes = [C.c_double]
+        L.orc_porosity.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_find_overlaps.restype = C.c_uint64
+        L.orc_find_overlaps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+        L.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_void_p]
         _lib = L
     return _lib
 
@@ -130,6 +134,30 @@ class Oracle:
     def set_params(self, vals):
         p = (C.c_double * 6)(*vals)
         self.L.orc_set_params(self.h, p)
+
+    def porosity(self, slices: int, rule: int = 0):
+        """(z_pos, porosity), or the oracle's status code (1: slices too
+        thin for the reference's rule, 2: slices < 1) as an int."""
+        z = np.empty(max(slices, 0))
+        p = np.empty(max(slices, 0))
+        rc = self.L.orc_porosity(self.h, slices, rule, z.ctypes.data_as(C.c_void_p), p.ctypes.data_as(C.c_void_p))
+        return rc if rc else (z, p)
+
+    def find_overlaps(self):
+        n = int(self.L.orc_find_overlaps(self.h, None, None, 0))
+        ij = np.empty((n, 2), dtype=np.uint32)
+        dn = np.empty(n)
+        got = int(self.L.orc_find_overlaps(self.h, ij.ctypes.data_as(C.c_void_p), dn.ctypes.data_as(C.c_void_p), n))
+        assert got == n
+        return ij, dn
+
+    def render(self, method: int = 0, max_val: float = 1e3, lower_cutoff: float = 0.0, width: int = 800,
+               height: int = 800) -> np.ndarray:
+        img = np.empty((height, width, 3), dtype=np.uint8)
+        rc = self.L.orc_render(self.h, method, max_val, lower_cutoff, width, height, img.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"orc_render -> {rc}")
+        return img
 
     def writebin(self, path):
         rc = self.L.orc_writebin(self.h, os.fspath(path).encode())

@@ -44,3 +44,18 @@ def test_library_is_gfx950_only():
     data = open(dem.LIB_PATH, "rb").read()
     targets = set(re.findall(rb"amdgcn-amd-amdhsa-+(gfx[0-9a-z]+)", data))
     assert targets == {b"gfx950"}, targets
+
+
+def test_analysis_tools_parse_arguments_without_a_gpu():
+    """`porosity` and `forcechains` (src/porosity.cpp, src/forcechains.cpp):
+    help, version and the errors raised before any input is opened."""
+    pkg = os.path.join(ROOT, "sphere_amd")
+    for tool, flag in (("porosity", "--slices"), ("forcechains", "--lower-cutoff")):
+        exe = os.path.join(pkg, tool)
+        out = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0 and flag in out.stdout
+        assert subprocess.run([exe, "-V"], capture_output=True, text=True, timeout=60).returncode == 0
+        missing = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+        assert missing.returncode == 1 and "missing input binary file" in missing.stderr
+    bad = subprocess.run([os.path.join(pkg, "porosity"), "-s", "0"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "number of slices must be a positive" in bad.stderr
