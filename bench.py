@@ -256,6 +256,18 @@ def analysis_run(dpath, reps=5, with_cpu=True):
     out["overlaps"] = int(len(ij))
     out["find_overlaps_ms"] = ms
     out["find_overlaps_pairs_per_s"] = len(ij) / (ms * 1e-3)
+    # DEM::render (src/raytracer.cuh:374-553) at the reference's 800 x 800
+    img = dem.render(0, 1e3, 0.0, 800, 800)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        img = dem.render(0, 1e3, 0.0, 800, 800)
+    out["render_800_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    brute = dem.render(0, 1e3, 0.0, 800, 800, cull=False)
+    out["render_800_every_sphere_per_pixel_ms"] = (time.perf_counter() - t0) * 1e3
+    out["render_tiles_equal_every_sphere"] = bool(np.array_equal(img, brute))
+    log(f"analysis: render 800x800 {out['render_800_ms']:.2f} ms tiled, "
+        f"{out['render_800_every_sphere_per_pixel_ms']:.1f} ms testing every sphere per pixel")
     dem.close()
     log(f"analysis: porosity(10) {out['porosity_10_reference_ms']:.2f} ms, find_overlaps {ms:.2f} ms "
         f"({len(ij):,} pairs)")
@@ -278,6 +290,12 @@ def analysis_run(dpath, reps=5, with_cpu=True):
         orc.close()
         out["cpu_find_overlaps"] = {"n": small.np, "pairs": npairs, "seconds": secs,
                                     "note": "the reference's O(N^2) double loop, serial, one core"}
+        orc = Oracle(spath)
+        t0 = time.perf_counter()
+        orc.render(0, 1e3, 0.0, 200, 200)
+        out["cpu_render"] = {"n": small.np, "pixels": 200 * 200, "seconds": time.perf_counter() - t0,
+                             "note": "the reference's every-sphere-per-pixel loop, serial, one core"}
+        orc.close()
         log(f"analysis cpu: porosity(10) {out['cpu_porosity_10_reference_ms']:.1f} ms at {n:,}; "
             f"find_overlaps {secs:.2f} s at {small.np:,}")
     return out

@@ -257,6 +257,20 @@ int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, double* porosity
  * SPH_E_INVALID when the buffers are given but too small. */
 int sph_find_overlaps(sph_t* h, uint32_t* ij, double* delta_n, uint64_t capacity, uint64_t* count);
 
+/* DEM::render (src/raytracer.cuh:374-553, `sphere -r`): ray-traced image of
+ * the current state from the reference's camera (eye at (2.3, -5, 1.3) x L,
+ * looking at half the largest position), in FP32 as there.  method 0
+ * shades every sphere; 1..5 colour by pressure, |velocity|, |angular
+ * velocity|, x displacement, |angular position| over [0, max_val], with
+ * lower_cutoff > 0 hiding spheres at or below it and fixed ones.  rgb
+ * receives width * height * 3 bytes (R, G, B; the reference's pixel
+ * order, as its PPM writes them; white background).  cull 1 bins the
+ * spheres into 16 x 16-pixel tiles first, cull 0 tests every sphere for
+ * every pixel as the reference does; both give the same bytes.
+ * width * height <= 2^24. */
+int sph_render(sph_t* h, int method, float max_val, float lower_cutoff, uint32_t width, uint32_t height, int cull,
+               uint8_t* rgb);
+
 /* ------------------------------------------------------------ ensembles
  * No counterpart in the reference, which steps one DEM per process: an
  * ensemble steps many independent systems (each a .bin image with its own

@@ -190,6 +190,25 @@ public:
         }
     }
 
+    // DEM::render (src/raytracer.cuh:374-553): writes img_out/<sid>.ppm (the
+    // reference's DEM::writePPM format, src/file_io.cpp:493-516).
+    // focalLength is accepted and unused, as in the reference's cameraInit.
+    void render(const int method = 1, const float maxval = 1.0e3f, const float lower_cutoff = 0.0f,
+                const float focalLength = 1.0f, const unsigned int img_width = 800,
+                const unsigned int img_height = 800, const std::string& outdir = "img_out") {
+        (void)focalLength;
+        std::vector<unsigned char> rgb(3 * (size_t)img_width * img_height);
+        check(sph_render(handle(), method, maxval, lower_cutoff, img_width, img_height, 1, rgb.data()));
+        const std::string target = outdir + "/" + sid_ + ".ppm";
+        FILE* f = std::fopen(target.c_str(), "wb");
+        if (!f) throw Error(SPH_E_IO, "Could not create output PPM file '" + target + "'");
+        std::fprintf(f, "P6 %u %u 255\n", img_width, img_height);
+        const size_t wrote = std::fwrite(rgb.data(), 1, rgb.size(), f);
+        std::fclose(f);
+        if (wrote != rgb.size()) throw Error(SPH_E_IO, "short write to '" + target + "'");
+        if (verbose_ == 1) std::printf("  Saving image: %s\n", target.c_str());
+    }
+
     sph_t* handle() const {
         if (!h_) throw Error(SPH_E_STATE, dry_ ? "dry-run DEM has no device state" : "DEM is closed");
         return h_;

@@ -219,6 +219,10 @@ __global__ __launch_bounds__(ABLOCK) void k_zero_u32(uint32_t* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)ABLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * ABLOCK) p[i] = 0u;
 }
 
+__global__ __launch_bounds__(ABLOCK) void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)ABLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * ABLOCK) p[i] = v;
+}
+
 uint32_t fill_blocks(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>(4096, (n + ABLOCK - 1) / ABLOCK)); }
 
 // --------------------------------------------------------- exclusive scan
@@ -399,6 +403,259 @@ __global__ __launch_bounds__(ABLOCK) void k_osort(const uint32_t* __restrict__ o
     }
 }
 
+// --------------------------------------------------------------- raytracer
+// DEM::render (src/raytracer.cuh:14-553) in FP32 as there.  The reference
+// tests every sphere for every pixel; here the spheres are first binned
+// into 16 x 16-pixel tiles by a conservative projection of a sphere grown by
+// more than the FP32 error of the hit test, and each tile's pixels test only
+// the spheres binned to it.  The nearest hit is the least (|t|, original
+// index), which is the sphere the reference's strict "<" over ascending
+// indices keeps; the hit test, roots, normal and shading use the
+// reference's expressions, so the image is the same byte for byte.
+constexpr int RT_TILE = 16;
+
+struct RF3 {
+    float x, y, z;
+};
+__host__ __device__ inline RF3 rf3(float x, float y, float z) { return {x, y, z}; }
+__host__ __device__ inline RF3 operator+(RF3 a, RF3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline RF3 operator-(RF3 a, RF3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline RF3 operator-(RF3 a) { return {-a.x, -a.y, -a.z}; }
+__host__ __device__ inline RF3 operator*(float s, RF3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline RF3 operator*(RF3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline RF3 operator*(RF3 a, RF3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__host__ __device__ inline RF3 operator/(RF3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline float rdot(RF3 a, RF3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline float rlength(RF3 a) { return sqrtf(rdot(a, a)); }
+// helper_math's host normalize: v * (1 / sqrt(v . v))
+__host__ __device__ inline RF3 rnormalize(RF3 a) { return a * (1.0f / sqrtf(rdot(a, a))); }
+__host__ __device__ inline RF3 rcross(RF3 a, RF3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// cameraInit (src/raytracer.cuh:281-333) with render's eye, look-at point
+// and screen width (:409-418)
+struct RtCam {
+    RF3 eye, u, v, w, light;
+    float pl, pr, pb, pt, dfoc;
+    uint32_t width, height;
+};
+
+// colorbar.h:7-22
+__device__ inline float cb_red(float r) { return fminf(1.0f, 0.209f * r * r * r - 2.49f * r * r + 3.0f * r + 0.0109f); }
+__device__ inline float cb_green(float r) { return fminf(1.0f, -2.44f * r * r + 2.15f * r + 0.369f); }
+__device__ inline float cb_blue(float r) { return fminf(1.0f, -2.21f * r * r + 1.61f * r + 0.573f); }
+
+// rayInitPerspective (src/raytracer.cuh:27-51)
+__device__ inline RF3 ray_direction(const RtCam& c, uint32_t mempos) {
+    const uint32_t i = mempos % c.width;
+    const uint32_t j = (uint32_t)(int)floorf((float)mempos / c.width) % c.width;
+    const float p_u = c.pl + (c.pr - c.pl) * (i + 0.5f) / c.width;
+    const float p_v = c.pb + (c.pt - c.pb) * (j + 0.5f) / c.height;
+    return -c.dfoc * c.w + p_u * c.u + p_v * c.v;
+}
+
+// May a sphere of this colour value be drawn (rayIntersectSpheresColormap's
+// cutoff, src/raytracer.cuh:193-203)?
+__device__ inline bool rt_drawn(int method, float lower_cutoff, double lin, double fixvel) {
+    if (method == 0 || !(lower_cutoff > 0.0)) return true;
+    return (float)lin > lower_cutoff && (float)fixvel == 0.f;
+}
+
+// Pixel-independent parts of the hit test: e - c and (e - c).(e - c) - R^2.
+__device__ inline float4 rt_record(const RtCam& c, const double4& x) {
+    const RF3 ctr = rf3((float)x.x, (float)x.y, (float)x.z);
+    const float R = (float)x.w;
+    const RF3 ec = c.eye - ctr;
+    return make_float4(ec.x, ec.y, ec.z, rdot(ec, ec) - R * R);
+}
+
+// Tiles (inclusive rectangle) the sphere can be hit from, or false for none.
+// The sphere is grown to R' = 1.001 R + 2e-3 |q| (q = c - eye): the FP32
+// discriminant errs by about 1e-7 (d.q)^2, i.e. it may accept rays up to
+// sqrt(R^2 + O(1e-6) |q|^2) from the centre.  Its box [q -+ R'] projects
+// through the eye (p_u = -D q_u / q_w, the line of pixel p) inside the
+// corner ratios when it does not straddle the eye plane; otherwise every
+// tile.  One pixel of margin on each side.
+__device__ inline bool rt_tiles(const RtCam& c, const double4& x, uint32_t Hr, uint32_t tiles_x, uint32_t tiles_y,
+                                uint32_t* t0x, uint32_t* t1x, uint32_t* t0y, uint32_t* t1y) {
+    const double qx = (double)(float)x.x - c.eye.x, qy = (double)(float)x.y - c.eye.y,
+                 qz = (double)(float)x.z - c.eye.z;
+    const double qn = sqrt(qx * qx + qy * qy + qz * qz);
+    const double Rg = 1.001 * (double)(float)x.w + 2e-3 * qn + 1e-30;
+    const double qu = qx * c.u.x + qy * c.u.y + qz * c.u.z;
+    const double qv = qx * c.v.x + qy * c.v.y + qz * c.v.z;
+    const double qw = qx * c.w.x + qy * c.w.y + qz * c.w.z;
+    *t0x = 0; *t1x = tiles_x - 1; *t0y = 0; *t1y = tiles_y - 1;
+    if (!isfinite(qn) || !isfinite(Rg) || fabs(qw) <= Rg) return true;
+    const double D = c.dfoc;
+    double ulo = INFINITY, uhi = -INFINITY, vlo = INFINITY, vhi = -INFINITY;
+    for (int a = -1; a <= 1; a += 2)
+        for (int b = -1; b <= 1; b += 2) {
+            const double den = qw + b * Rg;
+            const double pu = -D * (qu + a * Rg) / den, pv = -D * (qv + a * Rg) / den;
+            ulo = fmin(ulo, pu); uhi = fmax(uhi, pu);
+            vlo = fmin(vlo, pv); vhi = fmax(vhi, pv);
+        }
+    // pixel i sits at p_u = pl + (pr - pl) (i + 0.5) / W
+    const double su = ((double)c.pr - c.pl) / c.width;
+    const double ilo = floor((ulo - c.pl) / su - 0.5) - 1.0, ihi = ceil((uhi - c.pl) / su - 0.5) + 1.0;
+    if (ihi < 0.0 || ilo > (double)(c.width - 1)) return false;
+    *t0x = (uint32_t)fmax(ilo, 0.0) / RT_TILE;
+    *t1x = (uint32_t)fmin(ihi, (double)(c.width - 1)) / RT_TILE;
+    const double sv = ((double)c.pt - c.pb) / c.height;
+    if (sv == 0.0)  // zero-height image plane: every row is the line p_v = pb
+        return !(vhi < (double)c.pb - su || vlo > (double)c.pb + su);
+    const double jlo = floor((vlo - c.pb) / sv - 0.5) - 1.0, jhi = ceil((vhi - c.pb) / sv - 0.5) + 1.0;
+    if (jhi < 0.0 || jlo > (double)(Hr - 1)) return false;
+    *t0y = (uint32_t)fmax(jlo, 0.0) / RT_TILE;
+    *t1y = (uint32_t)fmin(jhi, (double)(Hr - 1)) / RT_TILE;
+    return true;
+}
+
+// Pass 1: tiles per drawn sphere into tcount, their total into *total.
+__global__ __launch_bounds__(ABLOCK) void k_rt_count(RtCam cam, const double4* __restrict__ x,
+                                                     const double4* __restrict__ vel, const uint32_t* __restrict__ id,
+                                                     uint32_t n, int method, float lower_cutoff,
+                                                     const double* __restrict__ lin, uint32_t Hr, uint32_t tiles_x,
+                                                     uint32_t tiles_y, uint32_t* __restrict__ tcount,
+                                                     unsigned long long* __restrict__ total) {
+    const uint32_t s = blockIdx.x * ABLOCK + threadIdx.x;
+    if (s >= n) return;
+    if (!rt_drawn(method, lower_cutoff, lin[id[s]], vel[s].w)) return;
+    uint32_t x0, x1, y0, y1;
+    if (!rt_tiles(cam, x[s], Hr, tiles_x, tiles_y, &x0, &x1, &y0, &y1)) return;
+    for (uint32_t ty = y0; ty <= y1; ++ty)
+        for (uint32_t tx = x0; tx <= x1; ++tx) atomicAdd(&tcount[ty * tiles_x + tx], 1u);
+    atomicAdd(total, (unsigned long long)(x1 - x0 + 1) * (y1 - y0 + 1));
+}
+
+// Pass 2: the records of each tile (any order inside a tile).
+__global__ __launch_bounds__(ABLOCK) void k_rt_fill(RtCam cam, const double4* __restrict__ x,
+                                                    const double4* __restrict__ vel, const uint32_t* __restrict__ id,
+                                                    uint32_t n, int method, float lower_cutoff,
+                                                    const double* __restrict__ lin, uint32_t Hr, uint32_t tiles_x,
+                                                    uint32_t tiles_y, uint32_t* __restrict__ cursor,
+                                                    float4* __restrict__ rec, uint2* __restrict__ rec_id) {
+    const uint32_t s = blockIdx.x * ABLOCK + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t orig = id[s];
+    if (!rt_drawn(method, lower_cutoff, lin[orig], vel[s].w)) return;
+    uint32_t x0, x1, y0, y1;
+    if (!rt_tiles(cam, x[s], Hr, tiles_x, tiles_y, &x0, &x1, &y0, &y1)) return;
+    const float4 r = rt_record(cam, x[s]);
+    for (uint32_t ty = y0; ty <= y1; ++ty)
+        for (uint32_t tx = x0; tx <= x1; ++tx) {
+            const uint32_t q = atomicAdd(&cursor[ty * tiles_x + tx], 1u);
+            rec[q] = r;
+            rec_id[q] = make_uint2(orig, s);
+        }
+}
+
+// Without tiles: one list of every drawn sphere, shared by all tiles.
+__global__ __launch_bounds__(ABLOCK) void k_rt_all(RtCam cam, const double4* __restrict__ x,
+                                                   const double4* __restrict__ vel, const uint32_t* __restrict__ id,
+                                                   uint32_t n, int method, float lower_cutoff,
+                                                   const double* __restrict__ lin, uint32_t* __restrict__ cursor,
+                                                   float4* __restrict__ rec, uint2* __restrict__ rec_id) {
+    const uint32_t s = blockIdx.x * ABLOCK + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t orig = id[s];
+    if (!rt_drawn(method, lower_cutoff, lin[orig], vel[s].w)) return;
+    const uint32_t q = atomicAdd(cursor, 1u);
+    rec[q] = rt_record(cam, x[s]);
+    rec_id[q] = make_uint2(orig, s);
+}
+
+// One block per 16 x 16-pixel tile of rows [0, Hr): every pixel tests the
+// tile's records (staged through LDS 256 at a time) and shades its nearest
+// hit (rayIntersectSpheres, src/raytracer.cuh:55-137, and
+// rayIntersectSpheresColormap, :140-278).
+__global__ __launch_bounds__(RT_TILE* RT_TILE) void k_rt_trace(
+    RtCam cam, const float4* __restrict__ rec, const uint2* __restrict__ rec_id, const uint32_t* __restrict__ tstart,
+    uint32_t shared_n, uint32_t tiles_x, uint32_t Hr, int method, float max_val, const double4* __restrict__ x,
+    const double4* __restrict__ vel, const double* __restrict__ lin, uint8_t* __restrict__ rgb) {
+    __shared__ float4 s_rec[RT_TILE * RT_TILE];
+    __shared__ uint32_t s_id[RT_TILE * RT_TILE];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t i = (tile % tiles_x) * RT_TILE + (threadIdx.x % RT_TILE);
+    const uint32_t jr = (tile / tiles_x) * RT_TILE + (threadIdx.x / RT_TILE);
+    const bool active = i < cam.width && jr < Hr;
+    const uint32_t mempos = jr * cam.width + i;
+    const RF3 d = active ? ray_direction(cam, mempos) : rf3(0.f, 0.f, 0.f);
+    const float dd = rdot(d, d);
+    float tdist = 1e10f, t_best = 0.0f;
+    uint32_t best_id = 0xffffffffu, best_q = 0xffffffffu;
+    const uint32_t lo = shared_n ? 0u : tstart[tile], hi = shared_n ? shared_n : tstart[tile + 1];
+    for (uint32_t base = lo; base < hi; base += RT_TILE * RT_TILE) {
+        const uint32_t q = base + threadIdx.x;
+        if (q < hi) {
+            s_rec[threadIdx.x] = rec[q];
+            s_id[threadIdx.x] = rec_id[q].x;
+        }
+        __syncthreads();
+        const uint32_t m = min(hi - base, (uint32_t)(RT_TILE * RT_TILE));
+        if (active) {
+            for (uint32_t k = 0; k < m; ++k) {
+                const float4 r = s_rec[k];
+                const RF3 ec = rf3(r.x, r.y, r.z);
+                const float Delta = (2.0f * rdot(d, ec)) * (2.0f * rdot(d, ec)) - 4.0f * dd * r.w;
+                if (!(Delta > 0.0f)) continue;
+                const float t_minus = ((rdot(-d, ec) - sqrtf(rdot(d, ec) * rdot(d, ec) - dd * r.w)) / dd);
+                const float at = fabsf(t_minus);
+                const uint32_t oid = s_id[k];
+                if (at < tdist || (at == tdist && best_id != 0xffffffffu && oid < best_id)) {
+                    tdist = at;
+                    t_best = t_minus;
+                    best_id = oid;
+                    best_q = base + k;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    uint8_t* px = rgb + 3 * (size_t)mempos;
+    if (best_id == 0xffffffffu) {  // imageInit's background (:14-24)
+        px[0] = px[1] = px[2] = 255;
+        return;
+    }
+    const uint32_t s = rec_id[best_q].y;
+    const double4 xs = x[s];
+    const RF3 c = rf3((float)xs.x, (float)xs.y, (float)xs.z);
+    const RF3 p = cam.eye + t_best * d;
+    const RF3 n = rnormalize(2.0f * (p - c));
+    const float dotprod = fmaxf(0.0f, rdot(n, cam.light));
+    const float I_d = 40.0f, k_d = 5.0f, k_a = 10.0f, I_a = 5.0f;
+    if (method == 0) {
+        px[0] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * 0.48f);
+        px[1] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * 0.41f);
+        px[2] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * 0.27f);
+        return;
+    }
+    const float fixvel = (float)vel[s].w;
+    float ratio = (float)(lin[best_id] / max_val);
+    if (ratio < 0.01f) ratio = 0.01f;
+    if (ratio > 0.99f) ratio = 0.99f;
+    float redv = cb_red(ratio), greenv = cb_green(ratio), bluev = cb_blue(ratio);
+    if (fixvel > 0.f) redv = greenv = bluev = 0.5f;
+    px[0] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * redv);
+    px[1] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * greenv);
+    px[2] = (uint8_t)((k_d * I_d * dotprod + k_a * I_a) * bluev);
+}
+
+// Rows r >= width repeat row r mod width (the reference's row index is
+// (mempos / width) mod width).
+__global__ __launch_bounds__(ABLOCK) void k_rt_repeat_rows(uint8_t* __restrict__ rgb, uint32_t width, uint32_t height) {
+    const size_t row_bytes = 3 * (size_t)width;
+    const size_t total = row_bytes * height;
+    for (size_t b = row_bytes * width + blockIdx.x * (size_t)ABLOCK + threadIdx.x; b < total;
+         b += (size_t)gridDim.x * ABLOCK) {
+        const size_t r = b / row_bytes, off = b % row_bytes;
+        rgb[b] = rgb[(r % width) * row_bytes + off];
+    }
+}
+
 int check_handle(sph_dem* h, const char* fn) {
     if (!h) return SPH_E_INVALID;
     if (h->d.nsys > 1 || !h->members.empty()) {
@@ -534,6 +791,121 @@ extern "C" int sph_find_overlaps(sph_t* h, uint32_t* ij, double* delta_n, uint64
     A_TRY(hipGetLastError());
     A_TRY(hipMemcpyAsync(ij, pairs, (size_t)total * 8, hipMemcpyDeviceToHost, h->stream));
     A_TRY(hipMemcpyAsync(delta_n, dn, (size_t)total * 8, hipMemcpyDeviceToHost, h->stream));
+    A_TRY(hipStreamSynchronize(h->stream));
+    return SPH_OK;
+}
+
+extern "C" int sph_render(sph_t* h, int method, float max_val, float lower_cutoff, uint32_t width, uint32_t height,
+                          int cull, uint8_t* rgb) {
+    int rc;
+    if ((rc = check_handle(h, "sph_render"))) return rc;
+    if (method < 0 || method > 5 || width == 0 || height == 0 || !rgb ||
+        (uint64_t)width * height > (1ull << 24)) {
+        h->err = "sph_render: need method 0..5, an output buffer and 1 <= width x height <= 2^24 pixels";
+        return SPH_E_INVALID;
+    }
+    const sph::HostState& hs = h->hs;
+    const sph::PState& st = cur(h);
+    const uint32_t n = h->d.np;
+    Scratch s;
+    double bnd[7] = {0, 0, 0, 0, 0, 0, 0}, r_last;
+    if (n && (rc = bounds(h, st, n, s, bnd, &r_last))) return rc;
+    if (std::isnan(bnd[6])) {
+        h->err = "sph_render: non-finite position or radius";
+        return SPH_E_NAN;
+    }
+    // camera: render (src/raytracer.cuh:409-418) and cameraInit (:281-313)
+    RtCam cam;
+    cam.width = width;
+    cam.height = height;
+    const float imgw = (float)(hs.L[0] * 1.35f);
+    const double mx[3] = {std::max(0.0, bnd[3]), std::max(0.0, bnd[4]), std::max(0.0, bnd[5])};  // DEM::maxPos
+    const RF3 lookat = rf3((float)mx[0], (float)mx[1], (float)mx[2]) / 2.0f;
+    cam.eye = rf3((float)(hs.L[0] * 2.3f), (float)(hs.L[1] * -5.0f), (float)(hs.L[2] * 1.3f));
+    const float hw_ratio = (float)(height / width);
+    cam.pl = -0.5f * imgw;
+    cam.pr = 0.5f * imgw;
+    cam.pb = -0.5f * imgw * hw_ratio;
+    cam.pt = 0.5f * imgw * hw_ratio;
+    const RF3 view = cam.eye - lookat;
+    const RF3 up = rf3(0.0f, 0.0f, 1.0f);
+    cam.w = -view / rlength(view);
+    cam.u = rcross(up, cam.w) / rlength(rcross(up, cam.w));
+    cam.v = rcross(cam.w, cam.u);
+    cam.dfoc = rlength(view) * 0.8f;
+    cam.light = rnormalize(-1.0f * cam.eye * rf3(1.0f, 0.2f, 0.6f));
+    // colour values per original index (render, :445-503), in FP64
+    std::vector<double> lin(std::max<uint32_t>(n, 1), 0.0);
+    if (method != 0 && n) {
+        const int field = method == 1 ? SPH_P : method == 2 ? SPH_VEL : method == 3 ? SPH_ANGVEL
+                        : method == 4 ? SPH_XYSUM : SPH_ANGPOS;
+        const size_t comps = method == 1 ? 1 : method == 2 ? 4 : method == 4 ? 2 : 3;
+        std::vector<double> f(comps * n);
+        if ((rc = sph_get(h, field, f.data(), f.size() * 8))) return rc;
+        for (uint32_t i = 0; i < n; ++i) {
+            const double* v = &f[comps * i];
+            lin[i] = (method == 1 || method == 4) ? v[0] : std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+    }
+    double* dlin;
+    A_TRY(s.get(&dlin, lin.size()));
+    A_TRY(hipMemcpyAsync(dlin, lin.data(), lin.size() * 8, hipMemcpyHostToDevice, h->stream));
+    const uint32_t Hr = std::min(width, height);
+    const uint32_t tiles_x = (width + RT_TILE - 1) / RT_TILE, tiles_y = (Hr + RT_TILE - 1) / RT_TILE;
+    const size_t T = (size_t)tiles_x * tiles_y;
+    const uint32_t nb = std::max<uint32_t>(1, (n + ABLOCK - 1) / ABLOCK);
+    uint32_t *tcount, *tstart, *cursor;
+    unsigned long long* dtotal;
+    A_TRY(s.get(&tcount, T + 1));
+    A_TRY(s.get(&tstart, T + 1));
+    A_TRY(s.get(&cursor, T + 1));
+    A_TRY(s.get(&dtotal, 1));
+    k_zero_u32<<<fill_blocks(T + 1), ABLOCK, 0, h->stream>>>(tcount, T + 1);
+    k_zero_u32<<<1, ABLOCK, 0, h->stream>>>(reinterpret_cast<uint32_t*>(dtotal), 2);
+    unsigned long long total = 0;
+    // records of one tiling: 24 bytes each; beyond 2^26 of them (every
+    // sphere on many tiles) the single shared list is cheaper
+    const unsigned long long max_entries = 1ull << 26;
+    if (cull && n) {
+        k_rt_count<<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, Hr, tiles_x,
+                                                 (uint32_t)tiles_y, tcount, dtotal);
+        A_TRY(hipGetLastError());
+        A_TRY(hipMemcpyAsync(&total, dtotal, 8, hipMemcpyDeviceToHost, h->stream));
+        A_TRY(hipStreamSynchronize(h->stream));
+    }
+    const bool tiled = cull && n && total <= max_entries;
+    float4* rec;
+    uint2* rec_id;
+    uint32_t shared_n = 0;
+    if (tiled) {
+        if ((rc = exclusive_scan(h, tcount, tstart, T + 1, s))) return rc;
+        A_TRY(s.get(&rec, total));
+        A_TRY(s.get(&rec_id, total));
+        A_TRY(hipMemcpyAsync(cursor, tstart, (T + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
+        k_rt_fill<<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, Hr, tiles_x,
+                                                (uint32_t)tiles_y, cursor, rec, rec_id);
+    } else {
+        A_TRY(s.get(&rec, std::max<uint32_t>(n, 1)));
+        A_TRY(s.get(&rec_id, std::max<uint32_t>(n, 1)));
+        k_zero_u32<<<1, ABLOCK, 0, h->stream>>>(cursor, 1);
+        if (n)
+            k_rt_all<<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, cursor, rec,
+                                                   rec_id);
+        A_TRY(hipGetLastError());
+        A_TRY(hipMemcpyAsync(&shared_n, cursor, 4, hipMemcpyDeviceToHost, h->stream));
+        A_TRY(hipStreamSynchronize(h->stream));
+    }
+    uint8_t* img;
+    const size_t bytes = 3 * (size_t)width * height;
+    A_TRY(s.get(&img, bytes));
+    if (tiled || shared_n)
+        k_rt_trace<<<(uint32_t)T, RT_TILE * RT_TILE, 0, h->stream>>>(cam, rec, rec_id, tstart, shared_n, tiles_x, Hr,
+                                                                    method, max_val, st.x, st.vel, dlin, img);
+    else  // nothing drawn: imageInit's background only
+        k_fill_u8<<<fill_blocks(bytes), ABLOCK, 0, h->stream>>>(img, bytes, 255);
+    if (height > width) k_rt_repeat_rows<<<fill_blocks(bytes), ABLOCK, 0, h->stream>>>(img, width, height);
+    A_TRY(hipGetLastError());
+    A_TRY(hipMemcpyAsync(rgb, img, bytes, hipMemcpyDeviceToHost, h->stream));
     A_TRY(hipStreamSynchronize(h->stream));
     return SPH_OK;
 }

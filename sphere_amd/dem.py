@@ -148,6 +148,8 @@ EXPORTS = {
     "sph_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "sph_porosity": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "sph_find_overlaps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "sph_render": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_int,
+                             C.c_void_p]),
     "sph_ens_open_mem": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_uint32, C.c_double, C.c_int,
                                    C.POINTER(C.c_void_p)]),
     "sph_ens_open": (C.c_int, [C.POINTER(C.c_char_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
@@ -350,6 +352,17 @@ class DEM:
             self._check(self._lib.sph_find_overlaps(self._h, ij.ctypes.data_as(C.c_void_p),
                                                     dn.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
         return ij, dn
+
+    def render(self, method: int = 0, maxval: float = 1.0e3, lower_cutoff: float = 0.0, width: int = 800,
+               height: int = 800, cull: bool = True) -> np.ndarray:
+        """DEM::render (src/raytracer.cuh:374-553): the image as a
+        (height, width, 3) uint8 array in the reference's pixel order.
+        method 0 normal, 1 pressure, 2 velocity, 3 angular velocity,
+        4 x displacement, 5 angular position."""
+        img = np.empty((int(height), int(width), 3), dtype=np.uint8)
+        self._check(self._lib.sph_render(self._h, int(method), float(maxval), float(lower_cutoff), int(width),
+                                         int(height), 1 if cull else 0, img.ctypes.data_as(C.c_void_p)))
+        return img
 
     def set_precision(self, prec):
         if isinstance(prec, str):
