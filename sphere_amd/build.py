@@ -28,7 +28,8 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 SOURCES = ["kernels.hip", "engine.hip", "adjoint.hip", "analysis.hip", "binio.cpp", "generate.cpp"]
 # command-line front ends, each linked against the library next to it
-TOOLS = {"sphere": "sphere_main.cpp", "porosity": "porosity_main.cpp", "forcechains": "forcechains_main.cpp"}
+TOOLS = {"sphere": "sphere_main.cpp", "porosity": "porosity_main.cpp", "forcechains": "forcechains_main.cpp",
+         "sphere_status": "sphere_status_main.cpp"}
 
 
 def _compile(src: str) -> str:

@@ -10,7 +10,8 @@
 // then orders the pairs by (i, j) as the reference's double loop emits them.
 // Distances and volumes use the reference's expressions with FP contraction
 // off, so every delta_n is bit-identical to the double loop's; porosity sums
-// are taken in a fixed order (per-block tree, then blocks in order), which
+// are taken in a fixed order (a tree per block, then a fixed pattern over
+// the block sums), which
 // differs from the reference's running subtraction only by rounding.
 
 #include <hip/hip_runtime.h>
@@ -199,13 +200,16 @@ __global__ __launch_bounds__(ABLOCK) void k_porosity(const double4* __restrict__
     }
 }
 
-__global__ __launch_bounds__(ABLOCK) void k_porosity_final(const double* __restrict__ part, uint32_t nblk, int slices,
-                                                           double h_slice, double V_slice, double* __restrict__ z_pos,
-                                                           double* __restrict__ por) {
-    const int s = blockIdx.x * ABLOCK + threadIdx.x;
-    if (s >= slices) return;
+// One wavefront per slice: lane l adds the block sums l, l + 64, ... in
+// order, then the lanes combine in a fixed butterfly.
+__global__ __launch_bounds__(64) void k_porosity_final(const double* __restrict__ part, uint32_t nblk, int slices,
+                                                       double h_slice, double V_slice, double* __restrict__ z_pos,
+                                                       double* __restrict__ por) {
+    const int s = blockIdx.x;
     double sum = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) sum += part[(size_t)s * nblk + b];
+    for (uint32_t b = threadIdx.x; b < nblk; b += 64) sum += part[(size_t)s * nblk + b];
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (threadIdx.x != 0) return;
     const double lo = s * h_slice;
     z_pos[s] = lo + 0.5 * h_slice;
     por[s] = (V_slice - sum) / V_slice;
@@ -471,18 +475,22 @@ __device__ inline float4 rt_record(const RtCam& c, const double4& x) {
 }
 
 // Tiles (inclusive rectangle) the sphere can be hit from, or false for none.
-// The sphere is grown to R' = 1.001 R + 2e-3 |q| (q = c - eye): the FP32
-// discriminant errs by about 1e-7 (d.q)^2, i.e. it may accept rays up to
-// sqrt(R^2 + O(1e-6) |q|^2) from the centre.  Its box [q -+ R'] projects
-// through the eye (p_u = -D q_u / q_w, the line of pixel p) inside the
-// corner ratios when it does not straddle the eye plane; otherwise every
-// tile.  One pixel of margin on each side.
+// Delta / 4 = (d.ec)^2 - (d.d)(ec.ec - R^2) = (d.d) R^2 - |d x ec|^2, and
+// its FP32 evaluation errs by at most about 14 eps (d.d)(ec.ec) (eps =
+// 2^-24; d.ec does not cancel: the ray points at the sphere), so the test
+// can accept rays up to sqrt(R^2 + 8.4e-7 |q|^2) from the centre (q = c -
+// eye).  The sphere is grown to R' = 1.001 sqrt(R^2 + 2e-6 |q|^2) + 1e-6
+// |q| (the last term covers the rounding of e - c).  Its box [q -+ R']
+// projects through the eye (p_u = -D q_u / q_w, the line of pixel p) inside
+// the corner ratios when it does not straddle the eye plane; otherwise
+// every tile.  One pixel of margin on each side.
 __device__ inline bool rt_tiles(const RtCam& c, const double4& x, uint32_t Hr, uint32_t tiles_x, uint32_t tiles_y,
                                 uint32_t* t0x, uint32_t* t1x, uint32_t* t0y, uint32_t* t1y) {
     const double qx = (double)(float)x.x - c.eye.x, qy = (double)(float)x.y - c.eye.y,
                  qz = (double)(float)x.z - c.eye.z;
     const double qn = sqrt(qx * qx + qy * qy + qz * qz);
-    const double Rg = 1.001 * (double)(float)x.w + 2e-3 * qn + 1e-30;
+    const double R = (double)(float)x.w;
+    const double Rg = 1.001 * sqrt(R * R + 2e-6 * qn * qn) + 1e-6 * qn + 1e-30;
     const double qu = qx * c.u.x + qy * c.u.y + qz * c.u.z;
     const double qv = qx * c.v.x + qy * c.v.y + qz * c.v.z;
     const double qw = qx * c.w.x + qy * c.w.y + qz * c.w.z;
@@ -513,40 +521,81 @@ __device__ inline bool rt_tiles(const RtCam& c, const double4& x, uint32_t Hr, u
     return true;
 }
 
+// Binning with one LDS histogram of the T tiles per block when it fits
+// (RT_LDS_TILES), so that a block's spheres, which lie close together in the
+// cell-sorted state, cost one global atomic per tile they touch rather than
+// one per sphere and tile.
+constexpr uint32_t RT_LDS_TILES = 16384;
+
 // Pass 1: tiles per drawn sphere into tcount, their total into *total.
+template <bool LDS>
 __global__ __launch_bounds__(ABLOCK) void k_rt_count(RtCam cam, const double4* __restrict__ x,
                                                      const double4* __restrict__ vel, const uint32_t* __restrict__ id,
                                                      uint32_t n, int method, float lower_cutoff,
                                                      const double* __restrict__ lin, uint32_t Hr, uint32_t tiles_x,
                                                      uint32_t tiles_y, uint32_t* __restrict__ tcount,
                                                      unsigned long long* __restrict__ total) {
+    extern __shared__ uint32_t hist[];
+    __shared__ unsigned long long block_total;
+    const uint32_t T = tiles_x * tiles_y;
+    if (LDS) {
+        for (uint32_t t = threadIdx.x; t < T; t += ABLOCK) hist[t] = 0u;
+        if (threadIdx.x == 0) block_total = 0ull;
+        __syncthreads();
+    }
     const uint32_t s = blockIdx.x * ABLOCK + threadIdx.x;
-    if (s >= n) return;
-    if (!rt_drawn(method, lower_cutoff, lin[id[s]], vel[s].w)) return;
-    uint32_t x0, x1, y0, y1;
-    if (!rt_tiles(cam, x[s], Hr, tiles_x, tiles_y, &x0, &x1, &y0, &y1)) return;
-    for (uint32_t ty = y0; ty <= y1; ++ty)
-        for (uint32_t tx = x0; tx <= x1; ++tx) atomicAdd(&tcount[ty * tiles_x + tx], 1u);
-    atomicAdd(total, (unsigned long long)(x1 - x0 + 1) * (y1 - y0 + 1));
+    uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+    const bool ok = s < n && rt_drawn(method, lower_cutoff, lin[id[s]], vel[s].w) &&
+                    rt_tiles(cam, x[s], Hr, tiles_x, tiles_y, &x0, &x1, &y0, &y1);
+    if (ok) {
+        for (uint32_t ty = y0; ty <= y1; ++ty)
+            for (uint32_t tx = x0; tx <= x1; ++tx) atomicAdd(LDS ? &hist[ty * tiles_x + tx] : &tcount[ty * tiles_x + tx], 1u);
+        const unsigned long long area = (unsigned long long)(x1 - x0 + 1) * (y1 - y0 + 1);
+        atomicAdd(LDS ? &block_total : total, area);
+    }
+    if (LDS) {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < T; t += ABLOCK)
+            if (hist[t]) atomicAdd(&tcount[t], hist[t]);
+        if (threadIdx.x == 0 && block_total) atomicAdd(total, block_total);
+    }
 }
 
-// Pass 2: the records of each tile (any order inside a tile).
+// Pass 2: the records of each tile (any order inside a tile).  With LDS the
+// block reserves one range per tile it touches and hands out slots from it.
+template <bool LDS>
 __global__ __launch_bounds__(ABLOCK) void k_rt_fill(RtCam cam, const double4* __restrict__ x,
                                                     const double4* __restrict__ vel, const uint32_t* __restrict__ id,
                                                     uint32_t n, int method, float lower_cutoff,
                                                     const double* __restrict__ lin, uint32_t Hr, uint32_t tiles_x,
                                                     uint32_t tiles_y, uint32_t* __restrict__ cursor,
                                                     float4* __restrict__ rec, uint2* __restrict__ rec_id) {
+    extern __shared__ uint32_t hist[];
+    const uint32_t T = tiles_x * tiles_y;
+    if (LDS) {
+        for (uint32_t t = threadIdx.x; t < T; t += ABLOCK) hist[t] = 0u;
+        __syncthreads();
+    }
     const uint32_t s = blockIdx.x * ABLOCK + threadIdx.x;
-    if (s >= n) return;
+    uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+    const bool ok = s < n && rt_drawn(method, lower_cutoff, lin[id[s]], vel[s].w) &&
+                    rt_tiles(cam, x[s], Hr, tiles_x, tiles_y, &x0, &x1, &y0, &y1);
+    if (LDS) {
+        if (ok)
+            for (uint32_t ty = y0; ty <= y1; ++ty)
+                for (uint32_t tx = x0; tx <= x1; ++tx) atomicAdd(&hist[ty * tiles_x + tx], 1u);
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < T; t += ABLOCK)
+            if (hist[t]) hist[t] = atomicAdd(&cursor[t], hist[t]);  // the block's first slot in tile t
+        __syncthreads();
+    }
+    if (!ok) return;
     const uint32_t orig = id[s];
-    if (!rt_drawn(method, lower_cutoff, lin[orig], vel[s].w)) return;
-    uint32_t x0, x1, y0, y1;
-    if (!rt_tiles(cam, x[s], Hr, tiles_x, tiles_y, &x0, &x1, &y0, &y1)) return;
     const float4 r = rt_record(cam, x[s]);
     for (uint32_t ty = y0; ty <= y1; ++ty)
         for (uint32_t tx = x0; tx <= x1; ++tx) {
-            const uint32_t q = atomicAdd(&cursor[ty * tiles_x + tx], 1u);
+            const uint32_t t = ty * tiles_x + tx;
+            const uint32_t q = atomicAdd(LDS ? &hist[t] : &cursor[t], 1u);
             rec[q] = r;
             rec_id[q] = make_uint2(orig, s);
         }
@@ -709,8 +758,7 @@ extern "C" int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, doubl
     A_TRY(s.get(&dp, (size_t)slices));
     k_zero_f64<<<fill_blocks(parts), ABLOCK, 0, h->stream>>>(part, parts);
     if (nblk) k_porosity<<<nblk, ABLOCK, 0, h->stream>>>(st.x, n, slices, h_slice, rule, part);
-    k_porosity_final<<<(slices + ABLOCK - 1) / ABLOCK, ABLOCK, 0, h->stream>>>(part, nblk, slices, h_slice, V_slice,
-                                                                             dz, dp);
+    k_porosity_final<<<(uint32_t)slices, 64, 0, h->stream>>>(part, nblk, slices, h_slice, V_slice, dz, dp);
     A_TRY(hipGetLastError());
     A_TRY(hipMemcpyAsync(z_pos, dz, (size_t)slices * 8, hipMemcpyDeviceToHost, h->stream));
     A_TRY(hipMemcpyAsync(porosity, dp, (size_t)slices * 8, hipMemcpyDeviceToHost, h->stream));
@@ -867,8 +915,12 @@ extern "C" int sph_render(sph_t* h, int method, float max_val, float lower_cutof
     // sphere on many tiles) the single shared list is cheaper
     const unsigned long long max_entries = 1ull << 26;
     if (cull && n) {
-        k_rt_count<<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, Hr, tiles_x,
-                                                 (uint32_t)tiles_y, tcount, dtotal);
+        if (T <= RT_LDS_TILES)
+            k_rt_count<true><<<nb, ABLOCK, T * 4, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin,
+                                                               Hr, tiles_x, (uint32_t)tiles_y, tcount, dtotal);
+        else
+            k_rt_count<false><<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, Hr,
+                                                            tiles_x, (uint32_t)tiles_y, tcount, dtotal);
         A_TRY(hipGetLastError());
         A_TRY(hipMemcpyAsync(&total, dtotal, 8, hipMemcpyDeviceToHost, h->stream));
         A_TRY(hipStreamSynchronize(h->stream));
@@ -882,8 +934,12 @@ extern "C" int sph_render(sph_t* h, int method, float max_val, float lower_cutof
         A_TRY(s.get(&rec, total));
         A_TRY(s.get(&rec_id, total));
         A_TRY(hipMemcpyAsync(cursor, tstart, (T + 1) * 4, hipMemcpyDeviceToDevice, h->stream));
-        k_rt_fill<<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, Hr, tiles_x,
-                                                (uint32_t)tiles_y, cursor, rec, rec_id);
+        if (T <= RT_LDS_TILES)
+            k_rt_fill<true><<<nb, ABLOCK, T * 4, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin,
+                                                              Hr, tiles_x, (uint32_t)tiles_y, cursor, rec, rec_id);
+        else
+            k_rt_fill<false><<<nb, ABLOCK, 0, h->stream>>>(cam, st.x, st.vel, st.id, n, method, lower_cutoff, dlin, Hr,
+                                                           tiles_x, (uint32_t)tiles_y, cursor, rec, rec_id);
     } else {
         A_TRY(s.get(&rec, std::max<uint32_t>(n, 1)));
         A_TRY(s.get(&rec_id, std::max<uint32_t>(n, 1)));

@@ -59,3 +59,18 @@ def test_analysis_tools_parse_arguments_without_a_gpu():
         assert missing.returncode == 1 and "missing input binary file" in missing.stderr
     bad = subprocess.run([os.path.join(pkg, "porosity"), "-s", "0"], capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "number of slices must be a positive" in bad.stderr
+
+
+def test_sphere_status_reads_a_status_file(tmp_path):
+    """`sphere_status <sid>` (src/sphere_status.cpp) on the "%2.4e %2.4e %d"
+    file startTime writes."""
+    (tmp_path / "output").mkdir()
+    (tmp_path / "output" / "demo.status.dat").write_text("%2.4e %2.4e %d\n" % (0.0125, 12.5, 3))
+    exe = os.path.join(ROOT, "sphere_amd", "sphere_status")
+    out = subprocess.run([exe, "demo"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "Current simulation time:  0.0125 s" in out.stdout
+    assert "Percentage completed:     12.5%" in out.stdout
+    assert "demo.output00003.bin" in out.stdout
+    assert subprocess.run([exe], cwd=tmp_path, capture_output=True, timeout=60).returncode == 1
+    assert subprocess.run([exe, "absent"], cwd=tmp_path, capture_output=True, timeout=60).returncode == 1
