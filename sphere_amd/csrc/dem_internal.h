@@ -132,6 +132,8 @@ struct DemDev {
     uint32_t* skey;     // cell id per sorted position
     float4* xf;         // FP32 copy of a.x for the search's distance prefilter
     float search_eps;   // prefilter margin (m), above the FP32 rounding of a distance
+    float rmax_f;       // largest radius of any system (FP32, rounded up)
+    float cell_margin;  // widening of a cell's box when its spheres are bounded by it (m)
     uint32_t* nlist;    // mode 5: np * NLIST partners (their list index), list index = sorted position at the build
     uint32_t* ncnt;     // mode 5: list length per list index
     float4* xref;       // mode 5: position at the last list build, per list index

@@ -98,6 +98,7 @@ void update_search_eps(sph_dem* h) {
     }
     const double ulp = std::ldexp(1.0, std::ilogb(std::max(M, 1e-30)) - 23);
     h->d.search_eps = (float)(16.0 * ulp + 2e-6 * rmax);
+    h->d.rmax_f = (float)(rmax * (1.0 + 1e-6));
     // neighbour-list reach beyond r_a + r_b: a tenth of the smallest cell
     // (at least 2 r_max / 10), so lists hold the touching and nearly touching
     // spheres; a rebuild is due once a sphere has moved 0.45 skin.  Two
@@ -120,6 +121,9 @@ void update_search_eps(sph_dem* h) {
         if (m->periodic == 1 && m->num[1] < 5) shell_ok = false;
     }
     h->d.list_skin = (float)(0.1 * cmin);
+    // a sphere's FP32 position lies in its cell's box widened by the FP32
+    // rounding (below search_eps) and the cell-id rounding (relative 1e-12)
+    h->d.cell_margin = (float)(1e-3 * cmin + 4.0 * (double)h->d.search_eps);
     const double eps = (double)h->d.search_eps, skin = (double)h->d.list_skin;
     const bool tight = room - 2.0 * eps < 0.9 * skin;
     h->d.list_shell = tight && shell_ok ? 1u : 0u;

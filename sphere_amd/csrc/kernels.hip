@@ -858,6 +858,16 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
     }
 }
 
+// Distance along one axis from the coordinate v to the box of cell t (of n)
+// seen through the periodic shift dm (the image of a sphere at c is c - dm),
+// widened by mg.  Spheres of the edge cells may lie beyond the grid (cell
+// ids clamp), so those cells are open on their outer side.
+__device__ __forceinline__ float cell_gap(float v, int t, int n, float o, float cl, float dm, float mg) {
+    const float lo = t == 0 ? -INFINITY : o + (float)t * cl - dm - mg;
+    const float hi = t == n - 1 ? INFINITY : o + (float)(t + 1) * cl - dm + mg;
+    return fmaxf(0.0f, fmaxf(lo - v, v - hi));
+}
+
 // Second pass of a list build: the spheres two cells away (the outer shell of
 // the 5 x 5 x 5 cells around the particle's cell) within the list reach.  A
 // pair two cells apart is more than a cell apart; when a cell is narrower
@@ -865,13 +875,21 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
 // rebuild, so the host asks for this pass (list_shell) instead of shrinking
 // the rebuild distance to fit the 27 cells.  It does so only when every
 // periodic axis has at least 5 cells: no cell is then reached twice.
+// Cells whose widened box lies farther than the list reach with the largest
+// partner (reach2 = (r_a + eps + skin + r_max)^2, rounded up) hold no list
+// entry and are skipped.
+struct CellBox {
+    float o[3], cl[3], mg, reach2;
+};
+
 __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const float4* __restrict__ xf,
                                         uint32_t* list, float* gap, uint32_t np, uint32_t& lcnt, int gx, int gy, int gz, int nx,
                                         int ny, int nz, int periodic, float Lx, float Ly, uint32_t cbase,
-                                        float4 xaf, float ra_skin) {
+                                        float4 xaf, float ra_skin, CellBox cb3) {
     for (int dz = -2; dz <= 2; ++dz) {
         const int tz = gz + dz;
         if (tz < 0 || tz >= nz) continue;
+        const float gzz = cell_gap(xaf.z, tz, nz, cb3.o[2], cb3.cl[2], 0.0f, cb3.mg);
         for (int dy = -2; dy <= 2; ++dy) {
             int ty = gy + dy;
             float dmy = 0.0f;
@@ -880,6 +898,8 @@ __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const 
                 dmy = ty < 0 ? Ly : -Ly;
                 ty = ty < 0 ? ty + ny : ty - ny;
             }
+            const float gyy = cell_gap(xaf.y, ty, ny, cb3.o[1], cb3.cl[1], dmy, cb3.mg);
+            if (gyy * gyy + gzz * gzz > cb3.reach2) continue;
             for (int dx = -2; dx <= 2; ++dx) {
                 if (dx > -2 && dx < 2 && dy > -2 && dy < 2 && dz > -2 && dz < 2) continue;  // the 27 cells
                 int tx = gx + dx;
@@ -889,6 +909,8 @@ __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const 
                     dmx = tx < 0 ? Lx : -Lx;
                     tx = tx < 0 ? tx + nx : tx - nx;
                 }
+                const float gxx = cell_gap(xaf.x, tx, nx, cb3.o[0], cb3.cl[0], dmx, cb3.mg);
+                if (gxx * gxx + gyy * gyy + gzz * gzz > cb3.reach2) continue;
                 const uint32_t c = cbase + (uint32_t)tx + (uint32_t)ty * (uint32_t)nx + (uint32_t)tz * (uint32_t)(nx * ny);
                 const uint32_t hi = off[c + 1];
                 for (uint32_t sb = off[c]; sb < hi; ++sb) {
@@ -930,6 +952,17 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     uint32_t* gslot = d.cslot + (size_t)o * NC;
     uint32_t* list = BUILD ? d.nlist + s : nullptr;  // entry k at list[k * np]
     uint32_t lcnt = 0;
+    // rows of cells farther than any hit (or list entry) can reach are skipped
+    CellBox cb3;
+    for (int a = 0; a < 3; ++a) {
+        cb3.o[a] = (float)S.origo[a];
+        cb3.cl[a] = (float)S.cell_len[a];
+    }
+    cb3.mg = d.cell_margin;
+    {
+        const float reach = (BUILD ? ra_skin : ra_eps) + d.rmax_f;
+        cb3.reach2 = reach * reach * 1.001f;
+    }
     RowSlots st;
     st.mask = d.ro.cmask[d.perm[s]];
     st.fresh = 0; st.nc = 0; st.err = 0; st.dirty = 0; st.stop = false; st.loaded = false;
@@ -944,6 +977,12 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
                 if (S.periodic != 1) continue;
                 cy = ty < 0 ? 1u : 2u;
                 ty = ty < 0 ? ny - 1 : 0;
+            }
+            {
+                const float dmy = cy == 1u ? S.Lf[1] : (cy == 2u ? -S.Lf[1] : 0.0f);
+                const float gyy = cell_gap(xaf.y, ty, ny, cb3.o[1], cb3.cl[1], dmy, cb3.mg);
+                const float gzz = cell_gap(xaf.z, tz, nz, cb3.o[2], cb3.cl[2], 0.0f, cb3.mg);
+                if (gyy * gyy + gzz * gzz > cb3.reach2) continue;
             }
             const uint32_t row = S.cbase + (uint32_t)ty * S.num[0] + (S.num[0] * S.num[1]) * (uint32_t)tz;
             if (gx >= 1 && gx + 1 < nx && cy == 0) {
@@ -977,7 +1016,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     if constexpr (BUILD) {  // list index = this sorted position
         if (d.list_shell && !st.stop)
             list_shell(d.off, d.xf, list, d.nlistg + s, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0],
-                       S.Lf[1], S.cbase, xaf, ra_skin);
+                       S.Lf[1], S.cbase, xaf, ra_skin, cb3);
         d.ncnt[s] = lcnt < (uint32_t)NLIST ? lcnt : (uint32_t)NLIST;
         d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
         if (lcnt > (uint32_t)NLIST || st.stop) atomicOr(&d.nstate[2], 1u);  // list unusable: rebuild next step
