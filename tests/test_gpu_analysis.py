@@ -211,3 +211,28 @@ def test_forcechains_tool(tmp_path):
     arrows = [line for line in gp.stdout.splitlines() if line.startswith("set arrow ")]
     assert len(arrows) == int((f_n > lc).sum())
     assert gp.stdout.rstrip().splitlines()[-1].startswith("plot [")
+
+
+def test_reference_porosity_cases(tmp_path):
+    """The packings of tests/porosity_tests.py (simple cubic 10^3, a grid
+    under a wall) on the GPU: both rules against the oracle, and the exact
+    rule's slice mean against the bulk porosity."""
+    from test_oracle_analysis import _bulk, _packing
+
+    n = 10
+    idx = np.arange(n ** 3)
+    cubic = _packing(np.column_stack([idx % n, (idx // n) % n, idx // (n * n)]) * 2.0 + 1.0, np.ones(n ** 3),
+                     L=(20.0, 20.0, 20.0), sid="gpu-por-cubic")
+    rng = np.random.default_rng(4)
+    g = np.arange(5)
+    gx, gy, gz = np.meshgrid(g, g, np.arange(4), indexing="ij")
+    x = np.column_stack([gx.ravel() * 0.2 + 0.1, gy.ravel() * 0.2 + 0.1, gz.ravel() * 0.18 + 0.15])
+    x += rng.uniform(-0.04, 0.04, x.shape)
+    grid = _packing(x, rng.uniform(0.03, 0.05, len(x)), top=0.8, sid="gpu-por-grid")
+    for sb, top in ((cubic, 20.0), (grid, 0.8)):
+        orc = Oracle.from_spherebin(sb, tmp_path)
+        dem = DEM(sb, verbosity=0, checkVals=0)
+        _compare(dem, orc, slices=(1, 2, 4))
+        for slices in (1, 2, 4):
+            _, p = dem.porosity(slices, exact=True)
+            assert np.mean(p) == pytest.approx(_bulk(sb, top), abs=1e-12)

@@ -145,3 +145,61 @@ def test_find_overlaps_ignores_periodic_images(tmp_path):
     sb.periodic = 1
     ij, dn = Oracle.from_spherebin(sb, tmp_path).find_overlaps()
     assert len(ij) == 0
+
+
+def _bulk(sb, top):
+    """Spherebin.bulkPorosity (python/sphere.py:1579-1591)."""
+    V_total = sb.L[0] * sb.L[1] * top
+    return (V_total - np.sum(4.0 / 3.0 * math.pi * sb.radius ** 3)) / V_total
+
+
+def test_simple_cubic_mean_equals_bulk(tmp_path):
+    """tests/porosity_tests.py:38-59: 10^3 spheres of radius 1 in simple cubic
+    packing; the mean over 1, 2 and 4 slices equals the bulk porosity
+    1 - pi/6 for both rules (the planes at 5, 10, 15 pass through centres or
+    touching points, where the reference's branches are exact)."""
+    n = 10
+    idx = np.arange(n ** 3)
+    ix, iy, iz = idx % n, (idx // n) % n, idx // (n * n)
+    x = np.column_stack([ix, iy, iz]) * 2.0 + 1.0
+    sb = _packing(x, np.ones(n ** 3), L=(20.0, 20.0, 20.0), sid="por-cubic")
+    o = Oracle.from_spherebin(sb, tmp_path)
+    bulk = _bulk(sb, 20.0)
+    assert bulk == pytest.approx(1.0 - math.pi / 6.0, abs=1e-12)
+    for rule in (0, 1):
+        for slices in (1, 2, 4):
+            _, p = o.porosity(slices, rule)
+            assert np.mean(p) == pytest.approx(bulk, abs=1e-12), (rule, slices)
+
+
+def test_random_grid_mean_against_bulk(tmp_path):
+    """tests/porosity_tests.py:7-36 on a seeded grid of 100 spheres under a
+    wall: the exact rule's mean equals the bulk porosity; the reference's rule
+    falls short by 2 sum(cap) / V_total, a cap of the part below (above) the
+    plane for every sphere that crosses a slice plane with its centre above
+    (below) it."""
+    rng = np.random.default_rng(4)
+    g = np.arange(5)
+    gx, gy, gz = np.meshgrid(g, g, np.arange(4), indexing="ij")
+    x = np.column_stack([gx.ravel() * 0.2 + 0.1, gy.ravel() * 0.2 + 0.1, gz.ravel() * 0.18 + 0.15])
+    x += rng.uniform(-0.04, 0.04, x.shape)
+    r = rng.uniform(0.03, 0.05, len(x))
+    top = 0.8
+    sb = _packing(x, r, top=top, sid="por-grid")
+    o = Oracle.from_spherebin(sb, tmp_path)
+    bulk = _bulk(sb, top)
+    for slices in (1, 2, 4):
+        h = top / slices
+        _, p1 = o.porosity(slices, 1)
+        assert np.mean(p1) == pytest.approx(bulk, abs=1e-12)
+        extra = 0.0
+        for zc, ri in zip(x[:, 2], r):
+            for k in range(1, slices):
+                plane = k * h
+                if zc - ri < plane < zc:
+                    extra += 2.0 * _cap(plane - (zc - ri), ri)
+                elif zc < plane < zc + ri:
+                    extra += 2.0 * _cap(zc + ri - plane, ri)
+        _, p0 = o.porosity(slices, 0)
+        assert np.mean(p0) == pytest.approx(bulk - extra / (sb.L[0] * sb.L[1] * top), abs=1e-12)
+        assert (extra > 0.0) == (slices > 1)
