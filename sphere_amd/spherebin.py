@@ -394,6 +394,28 @@ class Spherebin:
                          + np.sum(0.5 * I * self.k_n * np.sum(ot ** 2, axis=1)))
         raise ValueError(f"unknown energy method {method!r}")
 
+    def bulkPorosity(self) -> float:
+        """(V_total - V_solid) / V_total with V_total = L0 L1 L2, or L0 L1 w_x[0]
+        under a wall (python/sphere.py:1579-1591)."""
+        if self.nw == 0:
+            V_total = self.L[0] * self.L[1] * self.L[2]
+        else:
+            V_total = self.L[0] * self.L[1] * self.w_x[0]
+            if V_total <= 0.0:
+                raise ValueError("Could not determine total volume")
+        V_solid = float(np.sum(4.0 / 3.0 * math.pi * self.radius ** 3))
+        return float((V_total - V_solid) / V_total)
+
+    def porosity(self, slices: int = 10, exact: bool = False):
+        """(porosity, depth) per horizontal slice, as python/sphere.py:1593-1627
+        returns them from the `porosity` tool; computed by DEM::porosity on
+        the GPU (exact=True: exact sphere volumes per slice)."""
+        from .dem import DEM
+
+        with DEM(self, verbosity=0, checkVals=0) as dem:
+            z, p = dem.porosity(slices, exact=exact)
+        return p, z
+
     def wall_stress(self, idx: int = 0) -> float:
         """Normal stress on a wall, w_force / (L0*L1) (python/sphere.py:2594,2724)."""
         return float(self.w_force[idx] / (self.L[0] * self.L[1]))

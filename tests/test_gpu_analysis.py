@@ -236,3 +236,16 @@ def test_reference_porosity_cases(tmp_path):
         for slices in (1, 2, 4):
             _, p = dem.porosity(slices, exact=True)
             assert np.mean(p) == pytest.approx(_bulk(sb, top), abs=1e-12)
+
+
+def test_spherebin_porosity_matches_the_handle(tmp_path):
+    """Spherebin.porosity returns (porosity, depth) as python/sphere.py's
+    porosity() does, from the same GPU computation."""
+    sb = _random_overlapping(n=500, sid="sb-porosity")
+    p, z = sb.porosity(slices=6)
+    orc = Oracle.from_spherebin(sb, tmp_path)
+    wz, wp = orc.porosity(6, 0)
+    np.testing.assert_array_equal(z, wz)
+    np.testing.assert_allclose(p, wp, rtol=POROSITY_RTOL)
+    pe, _ = sb.porosity(slices=6, exact=True)
+    assert np.mean(pe) == pytest.approx(sb.bulkPorosity(), abs=1e-12)

@@ -203,3 +203,12 @@ def test_random_grid_mean_against_bulk(tmp_path):
         _, p0 = o.porosity(slices, 0)
         assert np.mean(p0) == pytest.approx(bulk - extra / (sb.L[0] * sb.L[1] * top), abs=1e-12)
         assert (extra > 0.0) == (slices > 1)
+
+
+def test_spherebin_bulk_porosity():
+    """Spherebin.bulkPorosity against its definition (python/sphere.py:1579-1591)."""
+    sb = _packing([[0.5, 0.5, 0.5], [0.2, 0.2, 0.2]], [0.1, 0.05], L=(1.0, 2.0, 3.0), sid="bulk")
+    V = 1.0 * 2.0 * 3.0
+    assert sb.bulkPorosity() == pytest.approx((V - _ball(0.1) - _ball(0.05)) / V, rel=1e-14)
+    sb2 = _packing([[0.5, 0.5, 0.5]], [0.1], L=(1.0, 2.0, 3.0), top=0.75, sid="bulk-wall")
+    assert sb2.bulkPorosity() == pytest.approx((1.5 - _ball(0.1)) / 1.5, rel=1e-14)
