@@ -800,8 +800,11 @@ __device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ s
         cpos = 31 - __clz(empty);
         st.fresh |= 1u << cpos;
     }
-    sw[cpos * st.ld] = bo | (code << 28);
-    st.dirty |= 1u << cpos;
+    const uint32_t w = bo | (code << 28);
+    if (sw[cpos * st.ld] != w || !((st.mask >> cpos) & 1u)) {  // unchanged kept slots are not written back
+        sw[cpos * st.ld] = w;
+        st.dirty |= 1u << cpos;
+    }
     st.mask |= 1u << cpos;
     if (++st.nc > (uint32_t)NC) { st.err |= FLAG_CONTACT_OVERFLOW; st.stop = true; }
 }
@@ -1165,9 +1168,12 @@ __device__ __forceinline__ void apply_hits(const DemDev& d, uint32_t s, uint64_t
             const int i = __ffs(m) - 1;
             if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo) { cpos = i; break; }
         }
-        if (cpos >= 0) {
-            sw[cpos * APPLY_BLOCK] = bo | (code << 28);
-            dirty |= 1u << cpos;
+        if (cpos >= 0) {  // kept; written back only if its image changed
+            const uint32_t w = bo | (code << 28);
+            if (sw[cpos * APPLY_BLOCK] != w) {
+                sw[cpos * APPLY_BLOCK] = w;
+                dirty |= 1u << cpos;
+            }
         } else {
             k[nnew++] = v;  // compacted in place (nnew <= h)
         }
@@ -1359,9 +1365,12 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
                 const int i = __ffs(m) - 1;
                 if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) { cpos = i; break; }
             }
-            if (cpos >= 0) {
-                sw[cpos * APPLY_BLOCK] = bo[k] | ((uint32_t)(v >> 32) & 15u) << 28;
-                dirty |= 1u << cpos;
+            if (cpos >= 0) {  // kept; written back only if its image changed
+                const uint32_t w = bo[k] | ((uint32_t)(v >> 32) & 15u) << 28;
+                if (sw[cpos * APPLY_BLOCK] != w) {
+                    sw[cpos * APPLY_BLOCK] = w;
+                    dirty |= 1u << cpos;
+                }
             } else if (nnew < (uint32_t)NC) {
                 hk[nnew++] = v;
             }
