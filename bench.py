@@ -43,11 +43,10 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 1M spheres (FP64 vs mixed); adjoint/forward step time ratio"
 STAGES = ("bonds", "cellid", "sort", "reorder", "search", "force", "walls")
 STAGE_KERNELS = {
-    "bonds": "k_bonds", "cellid": "k_fill_u32/k_count (ids from k_force)", "sort": "k_scan_*/k_scatter/k_cellsort",
+    "bonds": "k_bonds", "cellid": "(ids, counts and ranks from k_force)", "sort": "k_scan_reduce/k_scan_down/k_scatter/k_cellsort",
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
-PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>", "force": "k_force<double, false, false>",
-                  "cellid": "k_count"}
+PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>", "force": "k_force<double, false, false>"}
 SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply",
                   5: "k_list_search (k_topo_rows<true> on rebuild steps)"}
 SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false, false>", 3: "k_topo_wave", 4: "k_topo_half<false>",
@@ -64,11 +63,11 @@ def kernel_bytes(n, c, b, w, k):
     neighbour and bond-endpoint gathers counted as cache resident."""
     return {
         "bonds": 136 * b,                          # pair 8, delta 32, omega 32 read; delta, omega 64 write
-        "cellid": 12 * n,                          # count: key read, rank write, one atomic per run (ids come from force)
+        "cellid": 0,                               # keys, counts and ranks are written by the force kernel (below)
         "sort": 16 * n + 8 * k,                    # key + index read and write, cell bounds
         "reorder": 184 * n,                        # key, index, 11 doubles gathered and written
         "search": 32 * n + 32 * c,                 # own x,y,z,r; old + new slot, image per active slot
-        # interact (196N + 76C + 8WN) + integrate (388N) + next cell id and count (8N)
+        # interact (196N + 76C + 8WN) + integrate (388N) + next cell id and rank (8N)
         "force": 592 * n + 76 * c + 8 * w * n,
         "walls": 8 * w * n,
     }

@@ -629,7 +629,9 @@ std::vector<Region> state_regions(sph_dem* h) {
         r.push_back({p->cmask, n * 4});
     }
     r.push_back({d.pos_of, n * 4});
-    r.push_back({d.key, n * 4});
+    r.push_back({d.key, n * 4});  // the next step's keys with their counts and ranks
+    r.push_back({d.count, (size_t)d.K * 4});
+    r.push_back({d.rank, n * 4});
     r.push_back({d.cslot, n * nc * 4});
     if (d.cdt) r.push_back({d.cdt, n * nc * 3 * 8});
     if (d.cdt32) r.push_back({d.cdt32, n * nc * 3 * 4});

@@ -171,8 +171,9 @@ struct DemDev {
 // kernel launchers (kernels.hip); all enqueue on `s`
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 void launch_bonds(const DemDev& d, hipStream_t s);
-void launch_cellid(const DemDev& d, hipStream_t s);  // keys of the stored positions
-void launch_count(const DemDev& d, hipStream_t s);   // per-cell counts and ranks from the keys
+// keys of the stored positions, with their per-cell counts and ranks (the
+// counts must start at zero: k_scan_down clears them after use)
+void launch_cellid(const DemDev& d, hipStream_t s);
 void launch_scan(const DemDev& d, hipStream_t s);
 void launch_scatter(const DemDev& d, hipStream_t s);
 void launch_cellsort(const DemDev& d, hipStream_t s);

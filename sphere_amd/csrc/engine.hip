@@ -555,8 +555,8 @@ void sph::enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail,
     if (timed) (void)hipEventRecord(h->ev[0], s);
     sph::launch_bonds(d, s);
     if (timed) (void)hipEventRecord(h->ev[1], s);
-    // cell ids come from the previous step's force kernel (or prepare_keys)
-    sph::launch_count(d, s);
+    // cell ids, per-cell counts and ranks come from the previous step's
+    // force kernel (or prepare_keys)
     if (timed) (void)hipEventRecord(h->ev[2], s);
     sph::launch_scan(d, s);
     sph::launch_scatter(d, s);
@@ -583,6 +583,7 @@ int prepare_keys(sph_dem* h) {
     DemDev d = h->d;
     d.ro = own_set(own(h));
     sph::launch_fill_u32(h->d.flags_next, 1, 0u, h->stream);
+    sph::launch_fill_u32(h->d.count, h->d.K, 0u, h->stream);
     sph::launch_cellid(d, h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(h->stream));

@@ -136,34 +136,18 @@ __device__ __forceinline__ uint32_t cell_key(const Sys& S, double x, double y, d
     return S.cbase + (((uint32_t)iz * S.num[1]) * S.num[0] + (uint32_t)iy * S.num[0] + (uint32_t)ix);
 }
 
-// Cell ids of the stored positions, for a state that did not come out of a
-// step (after open or a position update); steps compute the next step's ids
-// in the force kernel.
-template <bool B>
-__global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.np) return;
-    const Sys& S = sys_at<B>(d, i);
-    const double4 x = d.ro.x[i];
-    uint32_t err = 0;
-    d.key[i] = cell_key(S, x.x, x.y, x.z, err);
-    if (err) atomicOr(d.flags_next, err);
-}
-
 // Per-cell counts and each particle's rank among its cell's members (in any
-// order: the cell sort orders members by original index afterwards).
-// Storage is the previous step's sorted order, so a wavefront's keys come in
-// runs; one atomic per run.
-__global__ __launch_bounds__(BLOCK) void k_count(DemDev d) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// order: the cell sort orders members by original index afterwards), taken
+// where the keys are computed (k_force for the next step, k_cellid after an
+// upload).  Storage is the previous step's sorted order, so a wavefront's
+// keys come in runs; one atomic per run.  Every lane of the wavefront below
+// the grid's end calls it (the active lanes are a prefix).
+__device__ __forceinline__ void count_key(const DemDev& d, uint32_t i, uint32_t key) {
     const uint32_t lane = threadIdx.x & 63u;
-    const bool act = i < d.np;
-    const uint32_t key = act ? d.key[i] : 0xffffffffu;
     const uint32_t prev = __shfl_up(key, 1, 64);
-    const bool head = act && (lane == 0u || prev != key);
+    const bool head = lane == 0u || prev != key;
     const uint64_t heads = __ballot(head);
-    const uint64_t acts = __ballot(act);
-    if (!act) return;
+    const uint64_t acts = __ballot(true);
     const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 for lane 63)
     const uint32_t start = 63u - (uint32_t)__clzll((long long)(heads & upto));
     uint32_t base = 0;
@@ -174,6 +158,22 @@ __global__ __launch_bounds__(BLOCK) void k_count(DemDev d) {
     }
     base = __shfl(base, (int)start, 64);
     d.rank[i] = base + (lane - start);
+}
+
+// Cell ids of the stored positions, for a state that did not come out of a
+// step (after open or a position update); steps compute the next step's ids
+// in the force kernel.
+template <bool B>
+__global__ __launch_bounds__(BLOCK) void k_cellid(DemDev d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.np) return;
+    const Sys& S = sys_at<B>(d, i);
+    const double4 x = d.ro.x[i];
+    uint32_t err = 0;
+    const uint32_t key = cell_key(S, x.x, x.y, x.z, err);
+    d.key[i] = key;
+    if (err) atomicOr(d.flags_next, err);
+    count_key(d, i, key);
 }
 
 // -------------------------------------------------------------------- scan
@@ -202,6 +202,18 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
 
 __global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
     __shared__ uint32_t lds[BLOCK / 64];
+    // the cell-id errors belong to this step; so does a neighbour-list
+    // rebuild requested by the last step's force kernel (or a failed build)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t f = *d.flags_next;
+        if (f) {
+            atomicOr(d.flags, f);
+            *d.flags_next = 0u;
+        }
+        d.nstate[0] = d.nstate[1] | d.nstate[2];
+        d.nstate[1] = 0u;
+        d.nstate[2] = 0u;
+    }
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     uint32_t sum = 0;
 #pragma unroll
@@ -212,34 +224,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
     if (threadIdx.x == 0) d.scan_bsum[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_top(DemDev d) {
-    __shared__ uint32_t lds[BLOCK / 64];
-    // the cell-id errors belong to this step; so does a neighbour-list
-    // rebuild requested by the last step's force kernel (or a failed build)
-    if (threadIdx.x == 0) {
-        const uint32_t f = *d.flags_next;
-        if (f) {
-            atomicOr(d.flags, f);
-            *d.flags_next = 0u;
-        }
-        d.nstate[0] = d.nstate[1] | d.nstate[2];
-        d.nstate[1] = 0u;
-        d.nstate[2] = 0u;
-    }
-    uint32_t carry = 0;
-    for (uint32_t b0 = 0; b0 < d.scan_blocks; b0 += BLOCK) {
-        const uint32_t i = b0 + threadIdx.x;
-        const uint32_t v = i < d.scan_blocks ? d.scan_bsum[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, lds, total);
-        if (i < d.scan_blocks) d.scan_bsum[i] = carry + ex;
-        carry += total;
-    }
-}
-
-// off[c] = first sorted position of cell c, off[K] = np
+// off[c] = first sorted position of cell c, off[K] = np.  Each block adds
+// the totals of the blocks before it (integer sums: any order).
 __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
     __shared__ uint32_t lds[BLOCK / 64];
+    uint32_t before = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += BLOCK) before += d.scan_bsum[b];
+    uint32_t prefix;
+    (void)block_exclusive_scan(before, lds, prefix);
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     uint32_t c[SCAN_PER_THREAD];
     uint32_t sum = 0;
@@ -249,7 +241,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
         sum += c[j];
     }
     uint32_t total;
-    uint32_t off = block_exclusive_scan(sum, lds, total) + d.scan_bsum[blockIdx.x];
+    uint32_t off = block_exclusive_scan(sum, lds, total) + prefix;
 #pragma unroll
     for (int j = 0; j < SCAN_PER_THREAD; ++j) {
         if (base + j < d.K) {
@@ -264,7 +256,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
 __global__ __launch_bounds__(BLOCK) void k_scatter(DemDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
-    d.perm[d.off[d.key[i]] + d.rank[i]] = i;
+    const uint32_t q = d.off[d.key[i]] + d.rank[i];
+    if (q < d.np) d.perm[q] = i;  // (counts and keys agree by construction; never write past perm)
 }
 
 // Order one cell's members by original index: odd-even transposition network
@@ -322,7 +315,11 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     d.xf[q] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
     const uint32_t id = d.ro.id[p];
     d.sid[q] = id;
-    d.pos_of[id] = q;
+    // pos_of is the inverse of the storage order (the last step's sorted
+    // order, or the original order after an upload): a particle that keeps
+    // its position already has its entry, and most do from one step to the
+    // next, which spares the scattered write
+    if (p != q) d.pos_of[id] = q;
     d.skey[q] = d.key[p];
     if (d.search_mode == 5) {
         const uint32_t li = d.ro.lidx[p];
@@ -1803,8 +1800,10 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
 
     // cell id of the new position for the next step's sort
     uint32_t cerr = 0;
-    d.key[s] = cell_key(S, px, py, pz, cerr);
+    const uint32_t nkey = cell_key(S, px, py, pz, cerr);
+    d.key[s] = nkey;
     if (cerr) atomicOr(d.flags_next, cerr);
+    count_key(d, s, nkey);
 
     // neighbour lists (search mode 5): the list index travels with the
     // particle; request a rebuild once it moved list_lim since the build
@@ -1921,11 +1920,8 @@ void launch_cellid(const DemDev& d, hipStream_t s) {
     if (d.systems) k_cellid<true><<<nblk(d.np), BLOCK, 0, s>>>(d);
     else k_cellid<false><<<nblk(d.np), BLOCK, 0, s>>>(d);
 }
-// (the counts start at zero: k_scan_down clears them after use)
-void launch_count(const DemDev& d, hipStream_t s) { k_count<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_scan(const DemDev& d, hipStream_t s) {
     k_scan_reduce<<<d.scan_blocks, BLOCK, 0, s>>>(d);
-    k_scan_top<<<1, BLOCK, 0, s>>>(d);
     k_scan_down<<<d.scan_blocks, BLOCK, 0, s>>>(d);
 }
 void launch_scatter(const DemDev& d, hipStream_t s) { k_scatter<<<nblk(d.np), BLOCK, 0, s>>>(d); }
