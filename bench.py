@@ -384,11 +384,15 @@ def main():
         sys.exit(2)
 
     from sphere_amd import configs
-    from sphere_amd.dem import bandwidth_probe
+    from sphere_amd.dem import bandwidth_probe, flops_probe
 
     if args.probe_only:
         gbs = bandwidth_probe(1 << 30, 20)
         print(json.dumps({"probe": "k_copy_probe", "bytes_per_launch": 2 * (1 << 30), "GBs": gbs}))
+        for fp64 in (True, False):
+            tf = flops_probe(fp64, 4096, 4096, 10)
+            print(json.dumps({"probe": "k_flops_probe<%s>" % ("double" if fp64 else "float"),
+                              "flops_per_launch": 16 * 4096 * 4096 * 256, "TFLOPs": tf}))
         return
 
     prof_steps = min(args.steps, 50)

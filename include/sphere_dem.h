@@ -204,6 +204,13 @@ int sph_set_force_kernel(sph_t* h, int variant);
  * (k_copy_probe) and returns the achieved read+write rate in GB/s. */
 int sph_bandwidth_probe(uint64_t bytes, int reps, double* gbs);
 
+/* Measurement helper with no counterpart in the reference: `blocks` blocks
+ * of 256 threads, each thread running iters x 8 independent fused
+ * multiply-adds (16 x iters floating-point operations) in FP64 (fp64 != 0)
+ * or FP32, `reps` times (k_flops_probe); returns the achieved TFLOP/s.  It
+ * calibrates the VALU FLOP counters of a profiler against known work. */
+int sph_flops_probe(int fp64, uint32_t blocks, uint32_t iters, int reps, double* tflops);
+
 /* Adjoint of the time step, with no counterpart in the reference: gradients
  * of a time-mean force objective
  *     J = 1/nsteps sum_{steps} sum_i w_i . F_i

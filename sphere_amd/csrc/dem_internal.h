@@ -181,6 +181,8 @@ void launch_reorder(const DemDev& d, hipStream_t s);
 void launch_contact(const DemDev& d, int precision, hipStream_t s);  // search + forces
 void launch_topology(const DemDev& d, hipStream_t s);
 void launch_forces(const DemDev& d, int precision, hipStream_t s);
+// FP64 or FP32 fused multiply-adds of known count (16 * iters per thread, BLOCK threads per block)
+void launch_flops_probe(void* out, uint32_t blocks, uint32_t iters, bool fp64, hipStream_t s);
 void launch_walls(const DemDev& d, hipStream_t s);
 void launch_copy_probe(const double4* src, double4* dst, uint64_t n, hipStream_t s);
 

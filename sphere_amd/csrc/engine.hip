@@ -1465,6 +1465,33 @@ int sph_bandwidth_probe(uint64_t bytes, int reps, double* gbs) {
     return rc;
 }
 
+int sph_flops_probe(int fp64, uint32_t blocks, uint32_t iters, int reps, double* tflops) {
+    if (blocks == 0 || iters == 0 || reps < 1 || !tflops) return SPH_E_INVALID;
+    const size_t elem = fp64 ? 8 : 4;
+    void* out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPH_OK;
+    if (hipMalloc(&out, (size_t)blocks * sph::BLOCK * elem) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = SPH_E_HIP;
+    } else {
+        sph::launch_flops_probe(out, blocks, iters, fp64 != 0, st);  // warm up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) sph::launch_flops_probe(out, blocks, iters, fp64 != 0, st);
+        (void)hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SPH_E_HIP;
+        else *tflops = 16.0 * iters * (double)blocks * sph::BLOCK * reps / (ms * 1e-3) / 1e12;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 }  // extern "C"
 
 // ----------------------------------------------------------------- ensembles

@@ -1906,6 +1906,24 @@ __global__ __launch_bounds__(BLOCK) void k_copy_probe(const double4* __restrict_
         dst[i] = src[i];
 }
 
+// Counter calibration: every thread runs `iters` rounds of 8 independent
+// fused multiply-adds, i.e. exactly 16 * iters floating-point operations,
+// and stores one value so that nothing is removed.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_flops_probe(T* __restrict__ out, uint32_t iters, T a, T b) {
+    T acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = (T)(threadIdx.x + k);
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = fma(acc[k], a, b);
+    }
+    T sum = acc[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) sum += acc[k];  // (7 adds, outside the counted work's 16 * iters)
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
 inline uint32_t nblk(uint64_t n, uint32_t b = BLOCK) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
@@ -1969,6 +1987,10 @@ void launch_contact(const DemDev& d, int precision, hipStream_t s) {
 }
 void launch_copy_probe(const double4* src, double4* dst, uint64_t n, hipStream_t s) {
     k_copy_probe<<<2048, BLOCK, 0, s>>>(src, dst, n);
+}
+void launch_flops_probe(void* out, uint32_t blocks, uint32_t iters, bool fp64, hipStream_t s) {
+    if (fp64) k_flops_probe<double><<<blocks, BLOCK, 0, s>>>(static_cast<double*>(out), iters, 0.999999, 1e-7);
+    else k_flops_probe<float><<<blocks, BLOCK, 0, s>>>(static_cast<float*>(out), iters, 0.999999f, 1e-7f);
 }
 void launch_walls(const DemDev& d, hipStream_t s) {
     if (d.systems) {

@@ -140,6 +140,7 @@ EXPORTS = {
     "sph_set_search": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_set_force_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "sph_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "sph_flops_probe": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "sph_dry_run": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t,
                               C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
     "sph_pack_rsa": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
@@ -539,4 +540,15 @@ def bandwidth_probe(nbytes: int = 1 << 30, reps: int = 20) -> float:
     rc = lib.sph_bandwidth_probe(int(nbytes), int(reps), C.byref(out))
     if rc != SPH_OK:
         _raise(rc, "bandwidth probe failed")
+    return out.value
+
+
+def flops_probe(fp64: bool = True, blocks: int = 4096, iters: int = 4096, reps: int = 10) -> float:
+    """TFLOP/s of k_flops_probe: 16 x iters fused-multiply-add FLOPs per
+    thread, 256 threads per block."""
+    lib = load_library()
+    out = C.c_double(0.0)
+    rc = lib.sph_flops_probe(1 if fp64 else 0, int(blocks), int(iters), int(reps), C.byref(out))
+    if rc != SPH_OK:
+        _raise(rc, "flops probe failed")
     return out.value
