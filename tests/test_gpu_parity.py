@@ -296,3 +296,15 @@ def test_contact_model_3_is_refused():
     sb.contactmodel = 3
     with pytest.raises(Unsupported, match="Hertz"):
         DEM(sb, verbosity=0)
+
+
+@pytest.mark.parametrize("workload", ["c1-compression", "dense-crystal"])
+def test_full_size_steps_match_oracle(tmp_path, workload):
+    """The benchmark's 1,048,576-sphere inputs at full size: every array of
+    the step bit-identical to the oracle after 1 and 3 steps (the random
+    compression packing with its stress wall, and the FCC crystal with twelve
+    contacts per sphere)."""
+    _say(f"full size parity {workload}")
+    sb = configs.random_compression() if workload == "c1-compression" else configs.dense_lattice()
+    errs = run_pair(sb, tmp_path, [1, 2])
+    assert not errs, errs
