@@ -216,3 +216,29 @@ def test_ensemble_refuses_bonds_and_grid_walking_kernels():
     with pytest.raises(Unsupported):
         ens.get(0, "cell_keys")
     ens.close()
+
+
+@pytest.mark.gpu
+def test_calibration_ensemble_members_match_single_handles():
+    """A 48-member slice of the calibration ensemble (uneven counts up to
+    32,768, k_n and mu spread, mixed precision, autotuned grid), 120 steps:
+    a sample of members bit-identical to their own handles on the same
+    regridded inputs."""
+    from sphere_amd.dem import DEM, Ensemble
+
+    _say("calibration ensemble sample")
+    ms = configs.calibration_ensemble(members=48, steps=1000)
+    ens = Ensemble(ms, cell_factor=0.0)
+    ens.set_precision("mixed")
+    ens.step(120)
+    info = ens.info()
+    assert info.members == 48
+    errs = []
+    for k in (0, 7, 23, 47):
+        d = DEM(regridded(ms[k], info.cell_factor), verbosity=0)
+        d.set_precision("mixed")
+        d.step(120)
+        errs += _compare(ens, k, d, _fields(ms[k]))
+        d.close()
+    ens.close()
+    assert not errs, errs[:8]
