@@ -150,3 +150,26 @@ def test_bonded_input_is_unsupported():
     with pytest.raises(Unsupported):
         dem.adjoint_forward(5, np.zeros((2, 3)))
     dem.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_at_full_size_matches_central_differences():
+    """The benchmark's adjoint workload at its full size (the 65,536-sphere
+    dense shear cell, top layer driven at 0.1 m/s under 10 kPa), over a
+    10-step horizon in 3 segments, against central differences."""
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    _say("fp64 adjoint vs central differences, 65,536 spheres, 10 steps")
+    sb = configs.fcc_shear(gamma_n=THETA0[2], sid="adj-fcc-shear-full")
+    assert sb.np == 65536
+    w = shear_stress_weights(sb)
+    dem = DEM(sb, verbosity=0)
+    dem.set_params(k_n=THETA0[0], k_t=THETA0[1], gamma_n=THETA0[2], mu_s=THETA0[3], mu_d=THETA0[3])
+    J = dem.adjoint_forward(10, w, seg_len=4)
+    g = dem.adjoint_backward()
+    dem.close()
+    fd = _fd(sb, w, 10)
+    err = _relerr(g, fd)
+    _say(f"J={J:.6e} adjoint={g} fd={fd} relative error={err}")
+    assert np.all(err <= 1e-4), f"relative error {err}"
