@@ -308,3 +308,27 @@ def test_full_size_steps_match_oracle(tmp_path, workload):
     sb = configs.random_compression() if workload == "c1-compression" else configs.dense_lattice()
     errs = run_pair(sb, tmp_path, [1, 2])
     assert not errs, errs
+
+
+def test_mixed_precision_stress_error_at_full_size():
+    """Mixed precision (FP32 pair forces) on the 1,048,576-sphere dense
+    crystal, 300 steps: the normal stress on the top wall (its per-particle
+    contact forces summed over the wall area) within 1e-6 relative of the
+    FP64 run, positions within 1e-4 of the smallest radius."""
+    from sphere_amd.dem import DEM
+
+    _say("mixed precision error at full size")
+    sb = configs.dense_lattice()
+    area = float(sb.L[0] * sb.L[1])
+    out = {}
+    for prec in ("fp64", "mixed"):
+        dem = DEM(sb, verbosity=0)
+        dem.set_precision(prec)
+        dem.step(300)
+        out[prec] = (float(np.sum(dem.get("wall_pp")[0])) / area, dem.get("x")[:, :3].copy())
+        dem.close()
+    s64, x64 = out["fp64"]
+    smx, xmx = out["mixed"]
+    assert s64 > 1e4  # the crystal presses on its wall
+    assert abs(smx - s64) / abs(s64) <= 1e-6, (s64, smx)
+    assert np.abs(xmx - x64).max() <= 1e-4 * float(sb.radius.min())
