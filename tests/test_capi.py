@@ -74,3 +74,15 @@ def test_sphere_status_reads_a_status_file(tmp_path):
     assert "demo.output00003.bin" in out.stdout
     assert subprocess.run([exe], cwd=tmp_path, capture_output=True, timeout=60).returncode == 1
     assert subprocess.run([exe, "absent"], cwd=tmp_path, capture_output=True, timeout=60).returncode == 1
+
+
+def test_sphere_render_flags_parse_without_a_gpu():
+    """`sphere -m <method>` (src/main.cpp:97-131): an unknown method is
+    refused before any input is opened; -h lists the raytracer's flags."""
+    exe = os.path.join(ROOT, "sphere_amd", "sphere")
+    out = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "--render" in out.stdout and "angpos" in out.stdout
+    bad = subprocess.run([exe, "-m", "colour", "1", "x.bin"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "Visualization method not understood" in bad.stderr
+    short = subprocess.run([exe, "-m", "vel"], capture_output=True, text=True, timeout=60)
+    assert short.returncode == 1 and "needs a maximum value" in short.stderr
