@@ -47,6 +47,7 @@ STAGE_KERNELS = {
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
 PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>", "force": "k_force<double, false, false>"}
+FORCE_PROFILE = {1: "k_force<double, false, false>", 2: "k_force<double, false, true>"}
 SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply",
                   5: "k_list_search (k_topo_rows<true> on rebuild steps)"}
 SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false, false>", 3: "k_topo_wave", 4: "k_topo_half<false>",
@@ -73,6 +74,15 @@ def kernel_bytes(n, c, b, w, k):
     }
 
 
+def roofline_entry(kernel, algorithmic_bytes, ms, traffic=None):
+    """HBM roofline of one kernel: algorithmic bytes per launch over its
+    measured time against the 8 TB/s peak."""
+    achieved = algorithmic_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+            "algorithmic_bytes_per_launch": algorithmic_bytes, "ms": ms}
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
@@ -84,10 +94,10 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+def measured_traffic(kernel, summary="c1_1M_pmc_traffic.md"):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "c1_1M_pmc_traffic.md")
+    path = os.path.join(ROOT, "profiles", summary)
     if not kernel or not os.path.exists(path):
         return None
     for line in open(path):
@@ -141,7 +151,8 @@ def timed_run(bin_path, precision, steps, warmup, prof_steps):
     dem.close()
     stage_ms = {k: tim[f"{k}_ms"] / max(1, tim["steps"]) for k in STAGES}
     return {"ms": ms, "contacts": c, "stage_ms": stage_ms, "tile": [tim["tile_x"], tim["tile_y"]],
-            "search_mode": int(tim["search_mode"]), "state": state, "steps_total": warmup + steps + prof_steps}
+            "search_mode": int(tim["search_mode"]), "force_variant": int(tim["force_variant"]), "state": state,
+            "steps_total": warmup + steps + prof_steps}
 
 
 def mixed_error(ref, mix, area, r_min):
@@ -313,7 +324,7 @@ def ensemble_run(members, steps, warmup, sample_every=32, sample_steps=100):
     ens = Ensemble(members, cell_factor=0.0)
     t_open = time.perf_counter() - t0
     ens.set_precision("mixed")
-    ens.step(warmup)  # the first call also tunes the search kernel and the block sizes
+    ens.step(warmup)  # the first call also tunes the search kernel and the block sizes (on a copy)
     t0 = time.perf_counter()
     ens.step(steps)
     ms = (time.perf_counter() - t0) * 1e3 / steps
@@ -356,10 +367,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    # (the first sph_step of at least 36 steps, 54 for the ensemble, times the
-    # search and force kernel variants; a warm-up of 50 keeps that out of the
-    # timed steps, the ensemble's warm-up is at least 60)
-    ap.add_argument("--warmup", type=int, default=50)
+    # (the first sph_step call of any length times the search and force
+    # kernel variants on a copy of the state and restores it, so every step
+    # of every call runs the chosen kernels whatever the warm-up)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-steps", type=int, default=4)
@@ -412,7 +423,8 @@ def main():
     log(f"{args.precision}: {ms:.4f} ms/step, C/N={c / n:.3f}, stages {stage_ms}")
     kb = kernel_bytes(n, c, sb.nb0, sb.nw, K)
     stage_kernels = dict(STAGE_KERNELS, search=SEARCH_KERNELS.get(main_run["search_mode"], "?"))
-    profile_kernel = dict(PROFILE_KERNEL, search=SEARCH_PROFILE.get(main_run["search_mode"]))
+    profile_kernel = dict(PROFILE_KERNEL, search=SEARCH_PROFILE.get(main_run["search_mode"]),
+                          force=FORCE_PROFILE[main_run["force_variant"]])
     per_kernel = {k: {"ms": stage_ms[k], "kernels": stage_kernels[k], "algorithmic_bytes": kb[k],
                       "GBs": (kb[k] / (stage_ms[k] * 1e-3) / 1e9) if stage_ms[k] > 0 and kb[k] else None}
                   for k in STAGES}
@@ -460,6 +472,14 @@ def main():
                  "contacts_per_particle": d64["contacts"] / dn, "fp64": {"value": dn / (d64["ms"] * 1e-3),
                  "ms_per_step": d64["ms"], "stage_ms": d64["stage_ms"],
                  "search_kernel": SEARCH_KERNELS.get(d64["search_mode"])}}
+        kd = kernel_bytes(dn, d64["contacts"], 0, sbd.nw, int(np.prod(sbd.num.astype(np.int64))))
+        dsk = SEARCH_PROFILE.get(d64["search_mode"])
+        dense["fp64"]["roofline"] = {
+            "force": roofline_entry(FORCE_PROFILE[d64["force_variant"]], kd["force"], d64["stage_ms"]["force"],
+                                    measured_traffic(FORCE_PROFILE[d64["force_variant"]], "dense_1M_pmc_traffic.md")),
+            "search": roofline_entry(dsk, kd["search"], d64["stage_ms"]["search"],
+                                     measured_traffic(dsk, "dense_1M_pmc_traffic.md")),
+        }
         log(f"dense fp64: {d64['ms']:.4f} ms/step, C/N={d64['contacts'] / dn:.2f}")
         if not args.no_mixed:
             dmx = timed_run(dpath, "mixed", args.steps, args.warmup, prof_steps)
@@ -481,13 +501,13 @@ def main():
         t0 = time.perf_counter()
         members = configs.calibration_ensemble(members=args.ensemble_members, steps=args.ensemble_steps)
         log(f"generated the ensemble in {time.perf_counter() - t0:.1f}s")
-        result["ensemble"] = ensemble_run(members, args.ensemble_steps, max(args.warmup, 60))
+        result["ensemble"] = ensemble_run(members, args.ensemble_steps, max(args.warmup, 1))
         del members
     if not args.no_cpu:
         rate, secs = cpu_baseline(bin_path, n, args.cpu_steps)
         result["cpu_baseline"] = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                                   "sample": f"same 1,048,576-sphere input, {args.cpu_steps} steps after one untimed step ({secs:.1f} s)",
-                                  "cpu": cpu_model()}
+                                  "cpu": cpu_model(), "host_nproc": os.cpu_count()}
     print(json.dumps(result), flush=True)
 
 

@@ -106,6 +106,7 @@ typedef struct sph_timing {
     uint32_t tile_x, tile_y;     /* cells per contact-search block */
     int32_t search_mode;         /* contact-search kernel in use (sph_set_search) */
     int32_t search_auto;         /* 1 when the kernel is chosen by timing */
+    int32_t force_variant;       /* k_force in use: 1 per-slot velocity gathers, 2 gathered ahead (sph_set_force_kernel) */
 } sph_timing_t;
 
 /* DEM::DEM + readbin + checkValues + allocateGlobalDeviceMemory +

@@ -600,53 +600,6 @@ __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
 
 inline uint32_t nb(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
-// ------------------------------------------------------------ host driver
-// Device copy of everything a step reads (see enqueue_step).
-struct Checkpoint {
-    std::vector<void*> bufs;
-    int own_cur = 0;
-    double t_cur = 0.0;
-    uint64_t steps_done = 0;
-};
-
-struct Region {
-    void* p;
-    size_t bytes;
-};
-
-std::vector<Region> state_regions(sph_dem* h) {
-    const DemDev& d = h->d;
-    const size_t n = h->hs.np, nc = NC;
-    std::vector<Region> r;
-    for (const PState* p : {&d.a, &d.b}) {
-        r.push_back({p->x, n * 32});
-        r.push_back({p->vel, n * 32});
-        r.push_back({p->angvel, n * 32});
-        r.push_back({p->pr0, n * 32});
-        r.push_back({p->pr1, n * 32});
-        r.push_back({p->pr2, n * 32});
-        r.push_back({p->id, n * 4});
-        r.push_back({p->cmask, n * 4});
-    }
-    r.push_back({d.pos_of, n * 4});
-    r.push_back({d.key, n * 4});  // the next step's keys with their counts and ranks
-    r.push_back({d.count, (size_t)d.K * 4});
-    r.push_back({d.rank, n * 4});
-    r.push_back({d.cslot, n * nc * 4});
-    if (d.cdt) r.push_back({d.cdt, n * nc * 3 * 8});
-    if (d.cdt32) r.push_back({d.cdt32, n * nc * 3 * 4});
-    r.push_back({d.wnx, NWALL * 32});
-    r.push_back({d.wmvfd, NWALL * 32});
-    r.push_back({d.wvel0, NWALL * 8});
-    r.push_back({d.t_cur, 8});
-    r.push_back({d.flags, 8});
-    r.push_back({d.force, n * 32});
-    r.push_back({d.torque, n * 32});
-    r.push_back({d.pres, n * 8});
-    if (h->hs.nw) r.push_back({d.wall_pp, (size_t)h->hs.nw * n * 8});
-    return r;
-}
-
 }  // namespace
 
 struct AdjointState {
@@ -668,9 +621,6 @@ struct AdjointState {
     bool have_forward = false;
     ~AdjointState() {
         for (void* p : allocs) (void)hipFree(p);
-        for (auto& c : cps)
-            for (void* p : c.bufs) (void)hipFree(p);
-        for (void* p : end.bufs) (void)hipFree(p);
     }
 };
 
@@ -686,44 +636,6 @@ template <class T> int adj_alloc(sph_dem* h, AdjointState* s, T** p, size_t coun
     }
     s->allocs.push_back(q);
     *p = (T*)q;
-    return SPH_OK;
-}
-
-int save_state(sph_dem* h, Checkpoint& c) {
-    const auto regs = state_regions(h);
-    if (c.bufs.empty()) {
-        for (const Region& r : regs) {
-            void* q = nullptr;
-            if (hipMalloc(&q, r.bytes) != hipSuccess) {
-                h->err = "adjoint: checkpoint allocation failed";
-                return SPH_E_HIP;
-            }
-            c.bufs.push_back(q);
-        }
-    }
-    for (size_t k = 0; k < regs.size(); ++k)
-        if (hipMemcpyAsync(c.bufs[k], regs[k].p, regs[k].bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
-            h->err = "adjoint: checkpoint copy failed";
-            return SPH_E_HIP;
-        }
-    c.own_cur = h->own_cur;
-    c.t_cur = h->hs.t_cur;
-    c.steps_done = h->steps_done;
-    return SPH_OK;
-}
-
-int restore_state(sph_dem* h, const Checkpoint& c) {
-    const auto regs = state_regions(h);
-    for (size_t k = 0; k < regs.size(); ++k)
-        if (hipMemcpyAsync(regs[k].p, c.bufs[k], regs[k].bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
-            h->err = "adjoint: checkpoint restore failed";
-            return SPH_E_HIP;
-        }
-    h->own_cur = c.own_cur;
-    h->hs.t_cur = c.t_cur;
-    h->steps_done = c.steps_done;
-    // neighbour lists (search mode 5) describe another state: rebuild
-    launch_fill_u32(h->d.nstate + 1, 1, 1u, h->stream);
     return SPH_OK;
 }
 
@@ -766,6 +678,11 @@ int sph_adjoint_forward(sph_t* h, uint64_t nsteps, uint32_t seg_len, const doubl
         h->err = "sph_adjoint_forward: only contact model 2 is differentiated";
         return SPH_E_UNSUPPORTED;
     }
+    if (h->hs.mu_s != h->hs.mu_d) {
+        // theta holds one friction coefficient mu = mu_s = mu_d
+        h->err = "sph_adjoint_forward: the gradient is taken for mu = mu_s = mu_d; set them equal";
+        return SPH_E_INVALID;
+    }
     const size_t n = h->hs.np;
     h->adj = std::unique_ptr<AdjointState, void (*)(AdjointState*)>(new AdjointState(), sph::free_adj);
     AdjointState* a = h->adj.get();
@@ -783,10 +700,7 @@ int sph_adjoint_forward(sph_t* h, uint64_t nsteps, uint32_t seg_len, const doubl
         return SPH_E_HIP;
     }
     // pick the contact-search kernel on a throw-away copy of the state
-    if (h->search_auto && !h->tuned) {
-        if ((rc = sph::save_state(h, a->end)) || (rc = sph::tune_search(h)) || (rc = sph::restore_state(h, a->end)))
-            return rc;
-    }
+    if ((rc = sph::autotune(h))) return rc;
     const uint64_t nseg = (nsteps + seg_len - 1) / seg_len;
     a->cps.resize(nseg);
     sph::StepHooks hooks;

@@ -244,6 +244,8 @@ void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
     d.tiles_y = (ny + by - 1) / by;
 }
 
+int check_bond_pairs(sph_dem* h, const uint32_t* bonds, size_t nb, uint32_t np);
+
 int validate_system(sph_dem* h, const HostState& s) {
     if (s.contactmodel == 3) {
         // src/contactmodels.cuh:565-566,596-597 raise the (negative) overlap
@@ -277,14 +279,7 @@ int validate_system(sph_dem* h, const HostState& s) {
         h->err = "periodic y needs at least 3 cells";
         return SPH_E_INVALID;
     }
-    for (uint32_t k = 0; k < s.nb0; ++k) {
-        if (s.bonds[2 * k] < s.np && s.bonds[2 * k + 1] >= s.np) {
-            h->err = "bond " + std::to_string(k) + " names particle " + std::to_string(s.bonds[2 * k + 1]) +
-                     " >= np";
-            return SPH_E_INVALID;
-        }
-    }
-    return SPH_OK;
+    return check_bond_pairs(h, s.bonds.data(), s.nb0, s.np);
 }
 
 int validate_for_device(sph_dem* h) {
@@ -320,6 +315,52 @@ struct Snapshot {
     std::vector<uint32_t> id;       // storage position -> original index
     std::vector<double> x, vel, angvel, pr0, pr1, pr2;  // 4 per particle, original order
 };
+
+// Per-particle bond lists in ascending bond index, first-index side first
+// (entry = k << 1 | side), over the intact bonds of `bonds` (pairs, first <
+// np), replacing the handle's lists.
+int upload_bond_lists(sph_dem* h, const std::vector<uint32_t>& bonds) {
+    DemDev& d = h->d;
+    const size_t n = h->hs.np, nb = h->hs.nb0;
+    std::vector<uint32_t> off(n + 1, 0), ent;
+    for (size_t k = 0; k < nb; ++k) {
+        const uint32_t bi = bonds[2 * k], bj = bonds[2 * k + 1];
+        if (!(bi < h->hs.np)) continue;
+        ++off[bi + 1];
+        ++off[bj + 1];
+    }
+    for (size_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    ent.assign(off[n] ? off[n] : 1, 0);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (size_t k = 0; k < nb; ++k) {
+        const uint32_t bi = bonds[2 * k], bj = bonds[2 * k + 1];
+        if (!(bi < h->hs.np)) continue;
+        ent[cur[bi]++] = (uint32_t)(k << 1);
+        ent[cur[bj]++] = (uint32_t)(k << 1) | 1u;
+    }
+    if (d.bcsr_ent) {
+        auto it = std::find(h->allocs.begin(), h->allocs.end(), (void*)d.bcsr_ent);
+        if (it != h->allocs.end()) h->allocs.erase(it);
+        (void)hipFree(d.bcsr_ent);
+        d.bcsr_ent = nullptr;
+    }
+    if (dalloc(h, &d.bcsr_ent, ent.size())) return SPH_E_HIP;
+    HIP_TRY(hipMemcpy(d.bcsr_off, off.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.bcsr_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+    return SPH_OK;
+}
+
+// The bond check of validate_system on a pair list: an intact bond (first
+// index < np) must name a second particle < np.
+int check_bond_pairs(sph_dem* h, const uint32_t* bonds, size_t nb, uint32_t np) {
+    for (size_t k = 0; k < nb; ++k) {
+        if (bonds[2 * k] < np && bonds[2 * k + 1] >= np) {
+            h->err = "bond " + std::to_string(k) + " names particle " + std::to_string(bonds[2 * k + 1]) + " >= np";
+            return SPH_E_INVALID;
+        }
+    }
+    return SPH_OK;
+}
 
 int upload_all(sph_dem* h) {
     HostState& s = h->hs;
@@ -440,26 +481,8 @@ int upload_all(sph_dem* h) {
         HIP_TRY(hipMemcpy(d.bonds, s.bonds.data(), nb * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.bdelta, bd.data(), nb * 32, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.bomega, bo.data(), nb * 32, hipMemcpyHostToDevice));
-        // per-particle bond lists in ascending bond index, first-index side first
-        std::vector<uint32_t> off(n + 1, 0), ent;
-        for (size_t k = 0; k < nb; ++k) {
-            const uint32_t bi = s.bonds[2 * k], bj = s.bonds[2 * k + 1];
-            if (!(bi < s.np)) continue;
-            ++off[bi + 1];
-            ++off[bj + 1];
-        }
-        for (size_t i = 0; i < n; ++i) off[i + 1] += off[i];
-        ent.assign(off[n] ? off[n] : 1, 0);
-        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
-        for (size_t k = 0; k < nb; ++k) {
-            const uint32_t bi = s.bonds[2 * k], bj = s.bonds[2 * k + 1];
-            if (!(bi < s.np)) continue;
-            ent[cur[bi]++] = (uint32_t)(k << 1);
-            ent[cur[bj]++] = (uint32_t)(k << 1) | 1u;
-        }
-        if (dalloc(h, &d.bcsr_ent, ent.size())) return SPH_E_HIP;
-        HIP_TRY(hipMemcpy(d.bcsr_off, off.data(), (n + 1) * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(d.bcsr_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+        int rc;
+        if ((rc = upload_bond_lists(h, s.bonds))) return rc;
     }
     return SPH_OK;
 }
@@ -843,6 +866,115 @@ int sph::tune_search(sph_dem* h, uint64_t* ran) {
 
 namespace {
 
+struct Region {
+    void* p;
+    size_t bytes;
+};
+
+std::vector<Region> state_regions(sph_dem* h) {
+    const DemDev& d = h->d;
+    const size_t n = h->hs.np, nc = sph::NC;
+    const size_t ns = h->members.empty() ? 1 : h->members.size();
+    std::vector<Region> r;
+    for (const sph::PState* p : {&d.a, &d.b}) {
+        r.push_back({p->x, n * 32});
+        r.push_back({p->vel, n * 32});
+        r.push_back({p->angvel, n * 32});
+        r.push_back({p->pr0, n * 32});
+        r.push_back({p->pr1, n * 32});
+        r.push_back({p->pr2, n * 32});
+        r.push_back({p->id, n * 4});
+        r.push_back({p->cmask, n * 4});
+    }
+    r.push_back({d.pos_of, n * 4});
+    r.push_back({d.key, n * 4});  // the next step's keys with their counts and ranks
+    r.push_back({d.count, (size_t)d.K * 4});
+    r.push_back({d.rank, n * 4});
+    r.push_back({d.cslot, n * nc * 4});
+    if (d.cdt) r.push_back({d.cdt, n * nc * 3 * 8});
+    if (d.cdt32) r.push_back({d.cdt32, n * nc * 3 * 4});
+    r.push_back({d.wnx, 6 * ns * 32});
+    r.push_back({d.wmvfd, 6 * ns * 32});
+    r.push_back({d.wvel0, 6 * ns * 8});
+    r.push_back({d.t_cur, ns * 8});
+    r.push_back({d.flags, 8});
+    r.push_back({d.force, n * 32});
+    r.push_back({d.torque, n * 32});
+    r.push_back({d.pres, n * 8});
+    if (d.nw) r.push_back({d.wall_pp, (size_t)d.nw * n * 8});
+    if (h->hs.nb0) {
+        const size_t nb = h->hs.nb0;
+        r.push_back({d.bonds, nb * 8});
+        r.push_back({d.bdelta, nb * 32});
+        r.push_back({d.bomega, nb * 32});
+    }
+    return r;
+}
+
+}  // namespace
+
+sph::Checkpoint::~Checkpoint() {
+    for (void* p : bufs) (void)hipFree(p);
+}
+
+int sph::save_state(sph_dem* h, Checkpoint& c) {
+    const auto regs = state_regions(h);
+    if (c.bufs.size() != regs.size()) {
+        for (void* p : c.bufs) (void)hipFree(p);
+        c.bufs.clear();
+        for (const Region& r : regs) {
+            void* q = nullptr;
+            if (hipMalloc(&q, r.bytes) != hipSuccess) {
+                h->err = "state copy: hipMalloc of " + std::to_string(r.bytes) + " bytes failed";
+                return SPH_E_HIP;
+            }
+            c.bufs.push_back(q);
+        }
+    }
+    for (size_t k = 0; k < regs.size(); ++k)
+        HIP_TRY(hipMemcpyAsync(c.bufs[k], regs[k].p, regs[k].bytes, hipMemcpyDeviceToDevice, h->stream));
+    c.own_cur = h->own_cur;
+    c.t_cur = h->hs.t_cur;
+    c.steps_done = h->steps_done;
+    return SPH_OK;
+}
+
+int sph::restore_state(sph_dem* h, const Checkpoint& c) {
+    const auto regs = state_regions(h);
+    if (c.bufs.size() != regs.size()) {
+        h->err = "state copy: the checkpoint does not match the handle";
+        return SPH_E_STATE;
+    }
+    for (size_t k = 0; k < regs.size(); ++k)
+        HIP_TRY(hipMemcpyAsync(regs[k].p, c.bufs[k], regs[k].bytes, hipMemcpyDeviceToDevice, h->stream));
+    h->own_cur = c.own_cur;
+    h->hs.t_cur = c.t_cur;
+    h->steps_done = c.steps_done;
+    // neighbour lists (search mode 5) describe another state: rebuild
+    request_list_rebuild(h);
+    return SPH_OK;
+}
+
+int sph::autotune(sph_dem* h) {
+    if (!tune_steps(h) || (h->tuned && h->steps_since_tune < 4096)) return SPH_OK;
+    Checkpoint c;
+    int rc;
+    if ((rc = save_state(h, c))) return rc;
+    // a simulation error inside the trial steps (the state blowing up within
+    // them) is the trial's, not the caller's: the kernels chosen so far are
+    // kept and the restored state is stepped as asked
+    rc = tune_search(h);
+    if (rc == SPH_E_HIP) return rc;
+    h->tuned = true;
+    h->steps_since_tune = 0;
+    destroy_graph(h);
+    if ((rc = restore_state(h, c))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SPH_OK;
+}
+
+namespace {
+
 int run_steps(sph_dem* h, uint64_t n) {
     if (n == 0) return SPH_OK;
     int rc;
@@ -868,15 +1000,11 @@ int run_steps(sph_dem* h, uint64_t n) {
         t.tile_y = h->d.tile_y;
         t.search_mode = h->d.search_mode;
         t.search_auto = h->search_auto ? 1 : 0;
+        t.force_variant = h->d.force_vw ? 2 : 1;
         h->timing = t;
         return check_flags(h);
     }
-    const uint64_t ts = tune_steps(h);
-    if (ts && n >= 2 * ts && (!h->tuned || h->steps_since_tune >= 4096)) {
-        uint64_t ran = 0;
-        if ((rc = sph::tune_search(h, &ran))) return rc;
-        n -= ran;
-    }
+    if ((rc = sph::autotune(h))) return rc;
     h->steps_since_tune += n;
     if (h->use_graph && (rc = ensure_graph(h))) return rc;
     // step i > 0 runs step i-1's wall update beside its cell list and search;
@@ -993,34 +1121,34 @@ int sph_readbin(sph_t* h, const char* path) {
         delete t;
         return rc;
     }
-    const int precision = h->precision, profiling = h->profiling, verbosity = h->verbosity, vals = h->check_vals;
-    const bool use_graph = h->use_graph, search_auto = h->search_auto;
-    const int search_mode = h->d.search_mode;
-    const uint32_t tpb_force = h->d.tpb_force, tpb_search = h->d.tpb_search, force_vw = h->d.force_vw;
+    // the new simulation is opened completely on the scratch handle, with
+    // this handle's settings, and swapped in only once it is ready: any
+    // error (an allocation that fails for a larger file, say) leaves h as it
+    // was
+    t->d.search_mode = h->d.search_mode;
+    t->d.tpb_force = h->d.tpb_force;
+    t->d.tpb_search = h->d.tpb_search;
+    t->d.force_vw = h->d.force_vw;
+    t->force_auto = h->force_auto;
+    rc = open_common(t, h->verbosity, h->check_vals);
+    if (!rc) {
+        t->use_graph = h->use_graph;
+        t->search_auto = h->search_auto && t->hs.contactmodel != 1;
+        t->profiling = h->profiling;
+        if (t->d.search_mode == 5) request_list_rebuild(t);
+        if (h->precision != SPH_PREC_FP64) rc = sph_set_precision(t, h->precision);
+    }
+    if (rc) {
+        h->err = t->err;
+        free_all(t);
+        delete t;
+        return rc;
+    }
     h->adj.reset();
-    free_all(h);
-    h->events_made = false;
-    h->hs = std::move(t->hs);
+    std::swap(*h, *t);
+    free_all(t);
     delete t;
-    h->d = DemDev{};
-    h->d.search_mode = search_mode;
-    h->d.tpb_force = tpb_force;
-    h->d.tpb_search = tpb_search;
-    h->d.force_vw = force_vw;
-    h->own_cur = 0;
-    h->precision = SPH_PREC_FP64;
-    h->steps_done = 0;
-    h->tuned = false;
-    h->steps_since_tune = 0;
-    h->timing = sph_timing_t{};
-    rc = open_common(h, verbosity, vals);
-    if (rc) return rc;
-    h->use_graph = use_graph;
-    h->search_auto = search_auto && h->hs.contactmodel != 1;
-    h->profiling = profiling;
-    if (search_mode == 5) request_list_rebuild(h);
-    if (precision != SPH_PREC_FP64) rc = sph_set_precision(h, precision);
-    return rc;
+    return SPH_OK;
 }
 
 void sph_close(sph_t* h) {
@@ -1316,7 +1444,17 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
             HIP_TRY(hipMemcpy(d.pres, pv.data(), n * 8, hipMemcpyHostToDevice));
             return SPH_OK;
         }
-        case SPH_BONDS: return copy(d.bonds);
+        case SPH_BONDS: {
+            // new pairs: checked like a .bin's, then the per-particle bond
+            // lists the force kernel gathers through are rebuilt from them
+            const uint32_t* pairs = static_cast<const uint32_t*>(src);
+            int rc;
+            if ((rc = check_bond_pairs(h, pairs, h->hs.nb0, h->hs.np))) return rc;
+            std::vector<uint32_t> b(pairs, pairs + 2 * (size_t)h->hs.nb0);
+            if ((rc = copy(d.bonds))) return rc;
+            h->hs.bonds = b;
+            return upload_bond_lists(h, b);
+        }
         case SPH_BONDS_DELTA: return copy(d.bdelta);
         case SPH_BONDS_OMEGA: return copy(d.bomega);
         case SPH_WALLS_NX: return copy(d.wnx);
@@ -1590,7 +1728,7 @@ int trial_ms(const std::vector<HostState>& ms, double factor, double& ms_step, s
     int rc = ens_open(std::move(t), 0, &e, err);
     if (rc) return rc;
     sph_dem* h = e->h;
-    rc = run_steps(h, 2 * tune_steps(h));
+    rc = run_steps(h, 4);  // tunes the kernels on a copy, then warms up
     hipEvent_t a = nullptr, b = nullptr;
     if (!rc && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) rc = SPH_E_HIP;
     float ms10 = 0.0f;

@@ -75,6 +75,35 @@ void enqueue_step(sph_dem* h, bool timed, int parity, bool lead, bool tail, cons
 // of the search and force kernels.  *ran receives the number of steps run.
 int tune_search(sph_dem* h, uint64_t* ran = nullptr);
 
+// Device copy of every array a step reads or a caller can observe (both
+// buffer sets, contact history, cell keys with their counts and ranks, the
+// last step's outputs, walls, bonds, time and error flags), with the host
+// counters that go with it.  Buffers are allocated by the first save_state
+// and freed with the checkpoint.
+struct Checkpoint {
+    std::vector<void*> bufs;
+    int own_cur = 0;
+    double t_cur = 0.0;
+    uint64_t steps_done = 0;
+    Checkpoint() = default;
+    Checkpoint(const Checkpoint&) = delete;
+    Checkpoint& operator=(const Checkpoint&) = delete;
+    Checkpoint(Checkpoint&& o) noexcept
+        : bufs(std::move(o.bufs)), own_cur(o.own_cur), t_cur(o.t_cur), steps_done(o.steps_done) {
+        o.bufs.clear();
+    }
+    ~Checkpoint();
+};
+int save_state(sph_dem* h, Checkpoint& c);
+// Copies the checkpoint back and requests a neighbour-list rebuild.
+int restore_state(sph_dem* h, const Checkpoint& c);
+
+// When the kernel choice is automatic and due (never tuned, or 4096 steps
+// since the last tuning): saves the state, runs tune_search on it and
+// restores it, so the caller's steps all run with the chosen kernels and
+// tuning leaves no trace in the results.
+int autotune(sph_dem* h);
+
 }  // namespace sph
 
 #endif

@@ -153,6 +153,20 @@ def test_bonded_input_is_unsupported():
 
 
 @pytest.mark.gpu
+def test_distinct_static_and_dynamic_friction_is_refused():
+    """theta holds one friction coefficient, mu = mu_s = mu_d: a handle with
+    mu_s != mu_d is refused rather than given d/dmu_s + d/dmu_d."""
+    from sphere_amd.dem import DEM, SphereInvalid
+
+    sb = configs.two_spheres(sid="mu-split")
+    sb.mu_s, sb.mu_d = 0.5, 0.4
+    dem = DEM(sb, verbosity=0)
+    with pytest.raises(SphereInvalid, match="mu_s"):
+        dem.adjoint_forward(5, np.zeros((2, 3)))
+    dem.close()
+
+
+@pytest.mark.gpu
 def test_adjoint_at_full_size_matches_central_differences():
     """The benchmark's adjoint workload at its full size (the 65,536-sphere
     dense shear cell, top layer driven at 0.1 m/s under 10 kPa), over a

@@ -181,6 +181,66 @@ def test_bonded_solid_full_size(tmp_path):
     assert not errs, errs
 
 
+def test_set_bonds_checks_pairs_and_rebuilds_the_bond_lists(tmp_path):
+    """sph_set(SPH_BONDS) mid-run: a pair naming a particle >= np is refused
+    and leaves the bonds as they were; a valid new pair list (one bond
+    broken by hand, one re-pointed at another neighbour) is what the
+    following steps use, bond forces included -- GPU and oracle, given the
+    same pairs, stay bit-identical."""
+    from oracle_lib import Oracle
+    from parity import all_fields, compare_all
+    from sphere_amd.dem import DEM, SphereInvalid
+
+    _say("sph_set bonds")
+    sb = configs.fcc_bonded(cells=(3, 3, 4), sigma_b=1e9, tau_b=1e9)
+    orc = Oracle.from_spherebin(sb, tmp_path)
+    dem = DEM(sb, verbosity=0)
+    dem.step(20)
+    assert orc.step(20) == 0
+    pairs = dem.get("bonds")
+    assert np.array_equal(pairs, orc.get("bonds"))
+    bad = pairs.copy()
+    bad[3, 1] = sb.np
+    with pytest.raises(SphereInvalid):
+        dem.set("bonds", bad)
+    assert np.array_equal(dem.get("bonds"), pairs)
+    new = pairs.copy()
+    new[0, 0] = sb.nb0  # broken
+    i = int(new[1, 0])
+    others = [int(j) for j in new[new[:, 0] == i][:, 1] if int(j) != int(new[1, 1])]
+    new[1, 1] = others[0]  # bond 1 now joins i to another of its neighbours
+    dem.set("bonds", new)
+    orc.set("bonds", new)
+    dem.step(30)
+    assert orc.step(30) == 0
+    errs = compare_all(dem, orc, all_fields(orc))
+    dem.close()
+    orc.close()
+    assert not errs, errs
+
+
+def test_first_call_tunes_on_a_copy(tmp_path):
+    """With the kernel choice automatic, the first call of any length times
+    the kernels on a copy of the state and restores it: a 2-step call runs 2
+    steps of the chosen kernels, and the results stay the oracle's."""
+    from sphere_amd.dem import DEM
+
+    _say("first call tunes on a copy")
+    sb = configs.lattice_settle(nside=(6, 6, 5), spacing=2.15e-2, jitter=5e-4, periodic=1, gamma_n=0.0,
+                                gamma_t=10.0, sid="tune-copy")
+    errs = run_pair(sb, tmp_path, [2, 1, 40])
+    assert not errs, errs
+    dem = DEM(sb, verbosity=0)
+    dem.step(2)
+    info = dem.info()
+    assert info.steps_done == 2
+    assert abs(dem.get("time")[1] - 2 * sb.time_dt) < 1e-18
+    dem.set_profiling(True)
+    dem.step(1)
+    assert dem.timing()["search_mode"] in (2, 4, 5)
+    dem.close()
+
+
 SEARCH_VARIANTS = [
     ("rows", lambda d: d.set_search(2)),
     ("rows-vw", lambda d: (d.set_search(2), d.set_force_kernel(2))),

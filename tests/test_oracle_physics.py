@@ -266,3 +266,193 @@ def test_stress_wall_static_equilibrium(tmp_path):
     assert abs(fz - F) < 1e-9 * F
     assert abs(o.get("walls_nx")[0, 3] - (2 * r - 2 * d)) < 1e-9 * d
     assert abs(o.get("wall_pp")[0, 0] - F) < 1e-9 * F
+
+
+# ---- contactLinear_wall tangential friction (src/contactmodels.cuh:47-80)
+
+def _floor_sphere(sid, r=0.01, g=0.0, periodic=1):
+    sb = _box(1, sid=sid, periodic=periodic)
+    sb.radius[0] = r
+    sb.g = np.array([0.0, 0.0, g])
+    return sb
+
+
+def test_sliding_sphere_on_floor_rolls_at_five_sevenths(tmp_path):
+    """A sphere launched sliding without spin on the floor: Coulomb friction
+    mu_wd N decelerates it at mu_wd g while its torque about the centre
+    spins it up, until it rolls.  Angular momentum about the contact point
+    is conserved, so it rolls at 5/7 of the launch speed with angular
+    velocity v / r, reached at t = 2 v0 / (7 mu_wd g) (textbook sliding-to-
+    rolling).  Pins the friction's sign, the dynamic coefficient above 1 mm/s
+    and the torque's sign and lever arm (-r n x f_t)."""
+    r, g, v0 = 0.01, -9.81, 0.5
+    sb = _floor_sphere("roll", r=r, g=g)
+    m = _mass(r)
+    sb.x[0] = (0.5, 0.5, r - m * abs(g) / sb.k_n)  # at the equilibrium overlap
+    sb.vel[0] = (v0, 0.0, 0.0)
+    sb.gamma_wn = 30.0           # under critical (2 sqrt(k m) = 66)
+    sb.gamma_wt = 1e3            # viscous force far above the Coulomb cap while sliding
+    sb.mu_wd, sb.mu_ws = 0.3, 0.6
+    sb.time_dt = 0.05 * math.sqrt(m / sb.k_n)
+    o = Oracle.from_spherebin(sb, tmp_path)
+    t_roll = 2 * v0 / (7 * sb.mu_wd * abs(g))
+    half = int(0.5 * t_roll / sb.time_dt)
+    assert o.step(half) == 0
+    t = half * sb.time_dt
+    v = o.get("vel")[0, 0]
+    # leapfrog velocity at step n carries the half step: v0 - a dt (n + 1/2)
+    want = v0 - sb.mu_wd * abs(g) * sb.time_dt * (half + 0.5)
+    assert abs(v - want) < 2e-3 * (v0 - v)
+    w = o.get("angvel")[0]
+    assert w[1] > 0 and abs(w[0]) < 1e-12 and abs(w[2]) < 1e-12
+    # spin-up: torque mu N r over I = 2/5 m r^2 -> omega = 5 mu g t / (2 r)
+    assert abs(w[1] - 2.5 * sb.mu_wd * abs(g) * sb.time_dt * (half + 0.5) / r) < 2e-3 * w[1]
+    assert o.step(int(1.5 * t_roll / sb.time_dt)) == 0
+    v = o.get("vel")[0, :3]
+    w = o.get("angvel")[0]
+    assert abs(v[0] - 5.0 / 7.0 * v0) < 1e-3 * v0
+    assert abs(w[1] * r - v[0]) < 1e-3 * v0
+    assert abs(v[1]) < 1e-12 and abs(w[0]) < 1e-9 and abs(w[2]) < 1e-9
+
+
+@pytest.mark.parametrize("vt,gamma_wt,regime", [
+    (0.0011, 1e5, "dynamic"),   # just above 1 mm/s: capped at mu_wd N
+    (0.0009, 1e5, "static"),    # just below: capped at mu_ws N
+    (0.01, 1.0, "viscous"),     # gamma_wt |v_t| below the cap
+])
+def test_floor_friction_regimes_one_step(tmp_path, vt, gamma_wt, regime):
+    """One step of a sphere pressed into the floor (overlap 1e-4 m, no
+    gravity, no normal damping) sliding along (cos 30, sin 30, 0): the
+    tangential force opposes the slip with magnitude gamma_wt |v_t| below the
+    Coulomb cap and the cap above it, mu_wd N for |v_t| > 1 mm/s and mu_ws N
+    below; the torque is (contact point - centre) x f_t."""
+    r, ov = 0.01, 1e-4
+    sb = _floor_sphere("fric-" + regime, r=r)
+    sb.x[0] = (0.5, 0.5, r - ov)
+    d = np.array([math.cos(math.pi / 6), math.sin(math.pi / 6), 0.0])
+    sb.vel[0] = vt * d
+    sb.gamma_wn, sb.gamma_wt = 0.0, gamma_wt
+    sb.mu_wd, sb.mu_ws = 0.3, 0.6
+    o = Oracle.from_spherebin(sb, tmp_path)
+    assert o.step(1) == 0
+    F, T = o.get("force")[0], o.get("torque")[0]
+    N = sb.k_n * ov
+    assert abs(F[2] - N) < 1e-12 * N
+    ft = F.copy()
+    ft[2] = 0.0
+    mag = {"dynamic": sb.mu_wd * N, "static": sb.mu_ws * N, "viscous": gamma_wt * vt}[regime]
+    np.testing.assert_allclose(ft, -mag * d, rtol=1e-12, atol=1e-18)
+    arm = np.array([0.0, 0.0, -r])
+    np.testing.assert_allclose(T, np.cross(arm, ft), rtol=1e-12, atol=1e-18)
+
+
+def test_moving_wall_adds_its_scalar_velocity_to_every_component(tmp_path):
+    """A sphere at rest under a top wall moving at w (wall velocity wmode 2):
+    the reference adds the scalar w to all three components of the contact
+    velocity (operator+(Float3, Float), src/vector_arithmetic.h:61,
+    src/contactmodels.cuh:33), so the tangential slip is (w, w, 0) -- the
+    sphere feels friction along -(1, 1, 0) sign(w) although nothing slides
+    sideways -- and the normal damping sees v_n = -w."""
+    r, ov, w = 0.01, 1e-4, 0.02
+    sb = _box(1, sid="wvel", periodic=1)
+    sb.set_walls(1)
+    sb.wmode = np.array([2], dtype=np.int32)
+    sb.w_n[0] = (0.0, 0.0, -1.0)
+    sb.radius[0] = r
+    sb.x[0] = (0.5, 0.5, 0.5)
+    sb.w_x[0] = 0.5 + r - ov
+    sb.w_vel[0] = w
+    sb.w_m[0] = 1.0
+    sb.gamma_wn, sb.gamma_wt = 2.0, 3.0
+    sb.mu_wd = sb.mu_ws = 0.5
+    o = Oracle.from_spherebin(sb, tmp_path)
+    assert o.step(1) == 0
+    F, T = o.get("force")[0], o.get("torque")[0]
+    N = sb.k_n * ov + sb.gamma_wn * w      # f_n = (-k_n delta - gamma_wn v_n) n with v_n = -w
+    slip = math.sqrt(2.0) * w
+    mag = min(sb.gamma_wt * slip, sb.mu_wd * N)
+    ft = -mag * np.array([1.0, 1.0, 0.0]) / math.sqrt(2.0)
+    np.testing.assert_allclose(F, ft + np.array([0.0, 0.0, -N]), rtol=1e-12, atol=1e-18)
+    assert F[0] == F[1] and F[0] < 0
+    np.testing.assert_allclose(T, np.cross(np.array([0.0, 0.0, r]), ft), rtol=1e-12, atol=1e-18)
+    assert abs(o.get("wall_pp")[0, 0] - N) < 1e-12 * N
+
+
+# ---- integrate's fixvel branch (src/integration.cuh:65-77)
+
+def test_fixvel_free_flight_keeps_velocities_to_the_bit(tmp_path):
+    """A fixvel sphere in free flight under gravity with a sideways
+    component: no horizontal acceleration, gravity cancelled on z, no angular
+    acceleration -- velocity and angular velocity stay bit-identical, the
+    position advances by v dt per step; a free sphere beside it falls."""
+    sb = _box(2, L=4.0, num=10, sid="fixvel")
+    sb.g = np.array([1.5, -2.0, -9.81])
+    sb.radius[:] = 0.01
+    sb.x[0] = (1.0, 1.0, 2.0)
+    sb.x[1] = (3.0, 3.0, 2.0)
+    sb.vel[:] = (0.3, -0.2, 0.1)
+    sb.angvel[:] = (1.0, 2.0, 3.0)
+    sb.fixvel[0] = 1.0
+    o = Oracle.from_spherebin(sb, tmp_path)
+    n = 400
+    assert o.step(n) == 0
+    v, w, x = o.get("vel"), o.get("angvel"), o.get("x")
+    assert np.array_equal(v[0, :3], sb.vel[0]) and v[0, 3] == 1.0
+    assert np.array_equal(w[0], sb.angvel[0])
+    np.testing.assert_allclose(x[0, :3], sb.x[0] + n * sb.time_dt * sb.vel[0], rtol=1e-12)
+    np.testing.assert_allclose(o.get("angpos")[0], n * sb.time_dt * sb.angvel[0], rtol=1e-12)
+    np.testing.assert_allclose(v[1, :3], sb.vel[1] + sb.g * sb.time_dt * (n + 0.5), rtol=1e-12)
+
+
+def test_fixvel_sphere_moves_in_z_by_contact_force_only(tmp_path):
+    """A fixvel sphere pressed into the floor under gravity: its z
+    acceleration is the contact force over its mass with gravity removed
+    (free to dilate), its horizontal velocity is held although the floor
+    pushes it sideways by friction, and it does not spin up."""
+    r, ov = 0.01, 1e-4
+    sb = _floor_sphere("fixvel-z", r=r, g=-9.81)
+    sb.x[0] = (0.5, 0.5, r - ov)
+    sb.vel[0] = (0.05, 0.0, 0.0)
+    sb.fixvel[0] = 1.0
+    sb.gamma_wn, sb.gamma_wt = 0.0, 1e3
+    sb.mu_wd = sb.mu_ws = 0.5
+    o = Oracle.from_spherebin(sb, tmp_path)
+    assert o.step(1) == 0
+    F = o.get("force")[0]
+    N = sb.k_n * ov
+    assert abs(F[2] - N) < 1e-12 * N and F[0] < 0
+    m = _mass(r)
+    v = o.get("vel")[0]
+    # one leapfrog step from rest in z with acceleration N / m: 1.5 a dt
+    assert abs(v[2] - 1.5 * N / m * sb.time_dt) < 1e-12 * abs(v[2])
+    assert v[0] == 0.05 and v[1] == 0.0
+    assert not o.get("angvel")[0].any()
+
+
+# ---- periodic wrap before the cell id (src/sorting.cuh:7-19, src/integration.cuh:80-95)
+
+def test_sphere_past_the_periodic_boundary_is_hashed_wrapped(tmp_path):
+    """A sphere whose integrated position lies just past x = L (integrate
+    wraps the position before moving it, so it can end up to v dt outside)
+    is hashed at its wrapped position, cell (0, y, z) of its own row, and
+    meets the sphere just inside x = 0.  The reference hashes the unwrapped
+    position: floor(x / cell) = num_x puts it in cell (0, y + 1, z) of the
+    next row, whose neighbourhood misses that partner for this step; this
+    build defines the hash on the wrapped position instead."""
+    r = 0.01
+    sb = _box(2, L=1.0, num=10, periodic=1, sid="wrap")
+    sb.radius[:] = r
+    sb.x[0] = (1.0 + 1e-3, 0.55, 0.55)
+    sb.x[1] = (0.015, 0.55, 0.55)
+    o = Oracle.from_spherebin(sb, tmp_path)
+    o.stage("bonds")
+    o.stage("cellid")
+    keys = o.get("cell_keys")
+    own_row = 5 * 100 + 5 * 10 + 0
+    assert keys[0] == own_row and keys[1] == own_row
+    assert keys[0] != own_row + 10  # the reference's next-row cell
+    for st in ("sort", "reorder", "topology"):
+        o.stage(st)
+    c = o.get("contacts")
+    assert 1 in c[0] and 0 in c[1]
+    assert abs(o.get("x")[0, 0] - 1e-3) < 1e-15
