@@ -387,6 +387,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the copy-kernel bandwidth probe (counter calibration)")
+    ap.add_argument("--profile-dense", choices=["fp64", "mixed"],
+                    help="run only the dense crystal in this precision (kernel profiles); prints its timings")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -408,6 +410,16 @@ def main():
 
     prof_steps = min(args.steps, 50)
     tmp = tempfile.mkdtemp(prefix="sphere-bench-")
+    if args.profile_dense:
+        sbd = configs.dense_lattice()
+        dpath = os.path.join(tmp, "dense.bin")
+        sbd.writebin(dpath)
+        run = timed_run(dpath, args.profile_dense, args.steps, args.warmup, prof_steps)
+        print(json.dumps({"workload": "dense crystal", "n": sbd.np, "precision": args.profile_dense,
+                          "ms_per_step": run["ms"], "contacts_per_particle": run["contacts"] / sbd.np,
+                          "stage_ms": run["stage_ms"], "search_mode": run["search_mode"],
+                          "force_variant": run["force_variant"]}), flush=True)
+        return
     t0 = time.perf_counter()
     sb = configs.random_compression(n=args.n)
     log(f"generated {args.n} spheres in {time.perf_counter() - t0:.1f}s, grid {[int(v) for v in sb.num]}")
