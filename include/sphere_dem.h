@@ -37,6 +37,13 @@ enum sph_status {
     SPH_E_UNSUPPORTED = -8       /* input uses a feature outside this path     */
 };
 
+/* Particles with fixvel (the 4th component of SPH_VEL) > 0 keep their x and
+ * y velocity, feel no gravity and do not spin up (src/integration.cuh:65-77;
+ * the reference's inputs use fixvel = 1).  Extension with no reference
+ * counterpart: fixvel >= SPH_FIXVEL_HOLD_Z also keeps the z velocity, a
+ * velocity-controlled particle layer (tension and compression tests). */
+#define SPH_FIXVEL_HOLD_Z 2.0
+
 /* Field ids for sph_get / sph_set.  Shapes in entries of f64 or u32,
  * original particle order, the reference's own layout. */
 enum sph_field {

@@ -24,6 +24,10 @@
 //    sums wall 0 for every wall);
 //  * bond forces are added in ascending bond index (src/cohesion.cuh:220-226
 //    races), a broken bond's first index becomes max(nb0, np);
+//  * extension (no reference counterpart): fixvel >= 2 also holds the z
+//    velocity (acc_z = 0), a velocity-controlled particle layer for tension
+//    tests; fixvel in (0, 2) is the reference's fixed horizontal velocity
+//    (src/integration.cuh:65-77), which the reference's inputs use (1);
 //  * bondsLinear reads the positions before the periodic wrap, as the
 //    reference does (it runs before integrate); running it first in the step
 //    is otherwise equivalent because no stage before it in the reference
@@ -775,6 +779,7 @@ static void stage_integrate(orc_t* s) {
         acc.x += s->g[0]; acc.y += s->g[1]; acc.z += s->g[2];
         if (vel.w > 0.0) {
             acc.x = 0.0; acc.y = 0.0; acc.z -= s->g[2];
+            if (vel.w >= ORC_FIXVEL_HOLD_Z) acc.z = 0.0;  // extension (header note)
             angacc = v3(0.0, 0.0, 0.0);
         }
         wrap_position(s, x);

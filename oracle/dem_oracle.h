@@ -29,7 +29,17 @@
  *                                  the C library's exp to 2 ulp
  *   bondsLinear .................. two-particle bond energy conservation and
  *                                  axial frequency sqrt(2 k_n A / m)
- *   integrate .................... leapfrog free-fall closed form
+ *   contactLinear_wall friction .. sliding-to-rolling on the floor (5/7
+ *                                  v0 at t = 2 v0 / (7 mu_wd g)), the
+ *                                  viscous / static / dynamic caps and the
+ *                                  1 mm/s switch, torque = arm x f_t, the
+ *                                  moving wall's scalar velocity added to
+ *                                  all three components
+ *   integrate .................... leapfrog free-fall closed form; fixvel
+ *                                  free flight bit-invariant velocities,
+ *                                  z moved by the contact force alone;
+ *                                  fixvel >= 2 (extension) holds vel_z too;
+ *                                  periodic wrap before the cell id
  *   summation / integrateWalls ... velocity wall pos0 + v t, stress wall
  *                                  equilibrium
  *   whole step ................... zero-g no-contact bit invariance and the
@@ -51,6 +61,10 @@
 
 #include <stddef.h>
 #include <stdint.h>
+
+/* fixvel at or above this value holds the z velocity too (an extension of
+ * the reference's fixed horizontal velocity, src/integration.cuh:65-77). */
+#define ORC_FIXVEL_HOLD_Z 2.0
 
 #ifdef __cplusplus
 extern "C" {

@@ -16,7 +16,7 @@ import math
 
 import numpy as np
 
-from .spherebin import Spherebin
+from .spherebin import FIXVEL_HOLD_Z, Spherebin
 
 BASE_SEED = 20261202
 
@@ -183,9 +183,11 @@ def fcc_bonded(cells=(4, 4, 8), radius=1e-2, jitter_frac=0.01, bond_reach=1.05, 
     Bonds join every pair with centre distance <= 1.05 (r_i + r_j).  The bond
     stiffness enters through the contact k_n = k_t (the reference scales it by
     the bond area, src/cohesion.cuh:124-177).  The top and bottom layers are
-    pulled apart with fixed velocities (vel.w = fixvel > 0 keeps vel_x, vel_y
-    and removes gravity; the vertical velocity is integrated).  No periodic
-    axes, no walls beyond the reference defaults.
+    pulled apart at +-pull_velocity / 2 by holding their whole velocity:
+    fixvel = FIXVEL_HOLD_Z (2), this build's extension of the reference's
+    fixed horizontal velocity (fixvel = 1 keeps vel_x and vel_y only and
+    lets the bonds decelerate the layers in z, src/integration.cuh:65-77).
+    No periodic axes, no walls beyond the reference defaults.
     """
     rng = np.random.default_rng(seed)
     a = 2.0 * math.sqrt(2.0) * radius
@@ -221,7 +223,7 @@ def fcc_bonded(cells=(4, 4, 8), radius=1e-2, jitter_frac=0.01, bond_reach=1.05, 
     zlo, zhi = z.min(), z.max()
     bottom = z < zlo + 0.5 * a
     top = z > zhi - 0.5 * a
-    sb.fixvel[bottom | top] = 1.0
+    sb.fixvel[bottom | top] = FIXVEL_HOLD_Z
     sb.vel[top, 2] = pull_velocity / 2.0
     sb.vel[bottom, 2] = -pull_velocity / 2.0
     sb.time_dt = _dt(sb, 0.17) * 0.5

@@ -342,6 +342,7 @@ __global__ __launch_bounds__(256) void k_adj_lf(DemDev d, AdjDev a) {
     const uint32_t o = t.id[s];
     const double ra = t.x[s].w;
     const bool fix = t.vel[s].w > 0.0;
+    const bool hold_z = t.vel[s].w >= SPH_FIXVEL_HOLD_Z;  // extension: no z acceleration either
     const double dt = d.sys.dt;
     const double m = 4.0 / 3.0 * PI_REF * ra * ra * ra * d.sys.rho;
     const double inertia = 2.0 / 5.0 * m * ra * ra;
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(256) void k_adj_lf(DemDev d, AdjDev a) {
     const V3<double> La = Lv0t * dt + Lv * (0.5 * dt);
     const V3<double> Lw0t = Lw0 + Lw;
     const V3<double> Lal = Lw0t * dt + Lw * (0.5 * dt);
-    V3<double> LF = fix ? mk(0.0, 0.0, La.z / m) : La / m;
+    V3<double> LF = fix ? mk(0.0, 0.0, hold_z ? 0.0 : La.z / m) : La / m;
     const V3<double> LT = fix ? mk(0.0, 0.0, 0.0) : Lal / inertia;
     const double4 wo = a.wobj[o];
     LF = LF + mk(wo.x, wo.y, wo.z) * a.inv_n;
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
         t.vel[q] = d.ro.vel[p];
         t.angvel[q] = d.ro.angvel[p];
         t.id[q] = d.sid[q];
-        t.pos_of[q] = d.pos_of[q];
+        t.pos_of[d.sid[q]] = (uint32_t)q;  // this step's sorted position of each particle
         const uint32_t mask = d.tmask[q];
         t.tmask[q] = mask;
         t.tfresh[q] = d.tfresh[q];

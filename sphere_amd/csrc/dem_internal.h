@@ -61,12 +61,14 @@ struct PState {
     uint32_t *id;     // original index
     uint32_t *cmask;  // occupied contact slots
     uint32_t *lidx;   // search mode 5: the particle's neighbour list (its sorted position at the build)
+    float4 *vf, *wf;  // mixed mode: FP32 copies of vel and angvel (xyz) for the pair force's partner gathers
 };
 
 // The records of one buffer set, as a step reads (ro) or writes (wo) them.
 struct OwnSet {
     double4 *x, *vel, *angvel, *pr0, *pr1, *pr2;
     uint32_t *id, *cmask, *lidx;
+    float4 *vf, *wf;
 };
 
 // Box, cell grid, parameters and walls of one system.  A handle steps either
@@ -108,7 +110,8 @@ struct DemDev {
     PState a, b;        // storage of the two buffer sets
     OwnSet ro, wo;      // this step's input (at perm[s]) and output (at s)
     uint32_t* sid;      // original index per sorted position of this step
-    uint32_t* pos_of;   // original index -> sorted position of this step
+    uint32_t* pos_of;   // original index -> storage position (the previous step's sorted position)
+                        // until k_pos_of after the force kernel sets this step's
 
     // per-step outputs, sorted order of the step
     double4 *force;     // fx, fy, fz, es_dot
@@ -184,6 +187,9 @@ void launch_forces(const DemDev& d, int precision, hipStream_t s);
 // FP64 or FP32 fused multiply-adds of known count (16 * iters per thread, BLOCK threads per block)
 void launch_flops_probe(void* out, uint32_t blocks, uint32_t iters, bool fp64, hipStream_t s);
 void launch_walls(const DemDev& d, hipStream_t s);
+// mixed mode: the input set's FP32 velocity copies from its FP64 records (after
+// any change of the state outside a step; steps keep them)
+void launch_f32_copies(const DemDev& d, hipStream_t s);
 void launch_copy_probe(const double4* src, double4* dst, uint64_t n, hipStream_t s);
 
 }  // namespace sph

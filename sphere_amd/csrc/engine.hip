@@ -65,7 +65,7 @@ void destroy_graph(sph_dem* h) {
 }
 
 sph::OwnSet own_set(const sph::PState& p) {
-    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx};
+    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx, p.vf, p.wf};
 }
 
 // the buffer set holding the current private records
@@ -649,9 +649,20 @@ int check_flags(sph_dem* h) {
 
 int ensure_mixed_buffers(sph_dem* h) {
     if (h->d.cdt32) return SPH_OK;
-    ALLOC(h->d.cdt32, (size_t)h->hs.np * sph::NC * 3);
-    HIP_TRY(hipMemset(h->d.cdt32, 0, (size_t)h->hs.np * sph::NC * 3 * 4));
+    const size_t n = h->hs.np;
+    ALLOC(h->d.cdt32, n * sph::NC * 3);
+    HIP_TRY(hipMemset(h->d.cdt32, 0, n * sph::NC * 3 * 4));
+    ALLOC(h->d.a.vf, n); ALLOC(h->d.a.wf, n); ALLOC(h->d.b.vf, n); ALLOC(h->d.b.wf, n);
     return SPH_OK;
+}
+
+// Mixed mode: the FP32 velocity copies of the current records, after any
+// change of the state outside a step (steps in mixed mode keep them).
+void refresh_f32(sph_dem* h) {
+    if (h->precision != SPH_PREC_MIXED || !h->d.a.vf) return;
+    DemDev d = h->d;
+    d.ro = own_set(own(h));
+    sph::launch_f32_copies(d, h->stream);
 }
 
 // Downloads every particle array into the host mirror (for writebin).
@@ -952,6 +963,7 @@ int sph::restore_state(sph_dem* h, const Checkpoint& c) {
     h->steps_done = c.steps_done;
     // neighbour lists (search mode 5) describe another state: rebuild
     request_list_rebuild(h);
+    refresh_f32(h);
     return SPH_OK;
 }
 
@@ -1390,7 +1402,25 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
     return SPH_E_INVALID;
 }
 
+}  // extern "C"
+
+namespace {
+int set_field(sph_t* h, int f, const void* src, size_t bytes);
+}  // namespace
+
+extern "C" {
+
 int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
+    const int rc = set_field(h, f, src, bytes);
+    if (rc == SPH_OK) refresh_f32(h);
+    return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+int set_field(sph_t* h, int f, const void* src, size_t bytes) {
     if (!h || !field_is_known(f)) return SPH_E_INVALID;
     const size_t need = sph_field_bytes(h, f);
     if (bytes != need) {
@@ -1474,6 +1504,10 @@ int sph_set(sph_t* h, int f, const void* src, size_t bytes) {
     return SPH_E_INVALID;
 }
 
+}  // namespace
+
+extern "C" {
+
 int sph_info(const sph_t* h, sph_info_t* out) {
     if (!h || !out) return SPH_E_INVALID;
     out->np = h->hs.np; out->nb0 = h->hs.nb0; out->nw = h->hs.nw; out->nc = sph::NC;
@@ -1526,6 +1560,7 @@ int sph_set_precision(sph_t* h, int precision) {
         HIP_TRY(hipMemcpy(h->d.cdt, a.data(), cnt * 8, hipMemcpyHostToDevice));
     }
     h->precision = precision;
+    refresh_f32(h);
     destroy_graph(h);
     return SPH_OK;
 }

@@ -315,11 +315,6 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
     d.xf[q] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
     const uint32_t id = d.ro.id[p];
     d.sid[q] = id;
-    // pos_of is the inverse of the storage order (the last step's sorted
-    // order, or the original order after an upload): a particle that keeps
-    // its position already has its entry, and most do from one step to the
-    // next, which spares the scattered write
-    if (p != q) d.pos_of[id] = q;
     d.skey[q] = d.key[p];
     if (d.search_mode == 5) {
         const uint32_t li = d.ro.lidx[p];
@@ -541,6 +536,26 @@ __device__ __forceinline__ void decode_image(const Sys& S, uint32_t code, double
     dx = cx == 1u ? S.L[0] : (cx == 2u ? -S.L[0] : 0.0);
     dy = cy == 1u ? S.L[1] : (cy == 2u ? -S.L[1] : 0.0);
 }
+
+// A partner's velocity records as the pair force of arithmetic T reads
+// them: the FP64 records, or (mixed mode) their FP32 copies, which the
+// previous step's force kernel wrote next to them -- the same values the
+// pair force would round them to, in half the bytes.
+template <class T> struct PartnerVel;
+template <> struct PartnerVel<double> {
+    using type = double4;
+    static __device__ __forceinline__ double4 vel(const DemDev& d, uint32_t p) { return d.ro.vel[p]; }
+    static __device__ __forceinline__ double4 angvel(const DemDev& d, uint32_t p) { return d.ro.angvel[p]; }
+    static __device__ __forceinline__ double4 zero() { return make_double4(0.0, 0.0, 0.0, 0.0); }
+    static __device__ __forceinline__ V3<double> v3(double4 a) { return xyz(a); }
+};
+template <> struct PartnerVel<float> {
+    using type = float4;
+    static __device__ __forceinline__ float4 vel(const DemDev& d, uint32_t p) { return d.ro.vf[p]; }
+    static __device__ __forceinline__ float4 angvel(const DemDev& d, uint32_t p) { return d.ro.wf[p]; }
+    static __device__ __forceinline__ float4 zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+    static __device__ __forceinline__ V3<float> v3(float4 a) { return mk<float>(a.x, a.y, a.z); }
+};
 
 template <class T> struct SlotStore;
 template <> struct SlotStore<double> {
@@ -1606,9 +1621,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     acc.T = mk(0.0, 0.0, 0.0);
     acc.es_dot = 0.0; acc.ev_dot = 0.0; acc.p = 0.0;
     // Slots in groups of FORCE_BATCH: the group's dependent gathers (slot
-    // word -> partner's sorted position -> where its record is stored -> the
-    // record) are issued together, then the slots are processed one by one in
-    // slot order.
+    // word -> where the partner's record is stored, pos_of still holding the
+    // storage positions -> the record) are issued together, then the slots
+    // are processed one by one in slot order.
     for (uint32_t m = mask; m;) {
         int si[FORCE_BATCH];
         uint32_t sw[FORCE_BATCH], pb[FORCE_BATCH];
@@ -1622,16 +1637,16 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         for (int k = 0; k < FORCE_BATCH; ++k) sw[k] = k < cnt ? slot[si[k]] : 0u;
 #pragma unroll
         for (int k = 0; k < FORCE_BATCH; ++k) pb[k] = k < cnt ? d.pos_of[sw[k] & SLOT_IDX_MASK] : 0u;
-#pragma unroll
-        for (int k = 0; k < FORCE_BATCH; ++k) pb[k] = k < cnt ? d.perm[pb[k]] : 0u;
-        double4 xbs[FORCE_BATCH], vbs[VW ? FORCE_BATCH : 1], wbs[VW ? FORCE_BATCH : 1];
+        using PV = PartnerVel<T>;
+        double4 xbs[FORCE_BATCH];
+        typename PV::type vbs[VW ? FORCE_BATCH : 1], wbs[VW ? FORCE_BATCH : 1];
 #pragma unroll
         for (int k = 0; k < FORCE_BATCH; ++k) xbs[k] = k < cnt ? d.ro.x[pb[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
         if constexpr (VW) {
 #pragma unroll
             for (int k = 0; k < FORCE_BATCH; ++k) {
-                vbs[k] = k < cnt ? d.ro.vel[pb[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
-                wbs[k] = k < cnt ? d.ro.angvel[pb[k]] : make_double4(0.0, 0.0, 0.0, 0.0);
+                vbs[k] = k < cnt ? PV::vel(d, pb[k]) : PV::zero();
+                wbs[k] = k < cnt ? PV::angvel(d, pb[k]) : PV::zero();
             }
         }
 #pragma unroll
@@ -1648,9 +1663,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
             if (delta_n < 0.0) {
                 const size_t mp = (size_t)o * NC + i;
                 V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
-                const double4 vb = VW ? vbs[VW ? k : 0] : d.ro.vel[pb[k]];
-                const double4 wb = VW ? wbs[VW ? k : 0] : d.ro.angvel[pb[k]];
-                contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), cvt<T>(xyz(vb)), cvt<T>(xyz(wb)), (T)ra,
+                const typename PV::type vb = VW ? vbs[VW ? k : 0] : PV::vel(d, pb[k]);
+                const typename PV::type wb = VW ? wbs[VW ? k : 0] : PV::angvel(d, pb[k]);
+                contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), PV::v3(vb), PV::v3(wb), (T)ra,
                                   (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
                 SlotStore<T>::store(d, mp, dt0);
             } else {
@@ -1772,6 +1787,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     ax += S.g[0]; ay += S.g[1]; az += S.g[2];
     if (va4.w > 0.0) {
         ax = 0.0; ay = 0.0; az -= S.g[2];
+        if (va4.w >= SPH_FIXVEL_HOLD_Z) az = 0.0;  // extension: the z velocity held as well
         bx = 0.0; by = 0.0; bz = 0.0;
     }
     double px = xa.x, py = xa.y, pz = xa.z;
@@ -1794,6 +1810,10 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     d.wo.pr2[s] = make_double4(ap.z, ea.x, ea.y, ea.z);
     d.wo.angvel[s] = make_double4(wx, wy, wz, ea.w);
     d.wo.vel[s] = make_double4(vx, vy, vz, va4.w);
+    if constexpr (sizeof(T) == 4) {
+        d.wo.vf[s] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
+        d.wo.wf[s] = make_float4((float)wx, (float)wy, (float)wz, 0.0f);
+    }
     d.wo.x[s] = make_double4(px, py, pz, radius);
     d.wo.id[s] = o;
     d.wo.cmask[s] = mask;
@@ -1819,6 +1839,28 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
         d.ldisp[li] = sqrtf(dd2);
         if (dd2 > d.list_lim * d.list_lim) atomicOr(&d.nstate[1], 1u);
     }
+}
+
+// FP32 copies of the input set's velocity records (mixed mode), by storage
+// position.
+__global__ __launch_bounds__(BLOCK) void k_f32_copies(DemDev d) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.np) return;
+    const double4 v = d.ro.vel[q], w = d.ro.angvel[q];
+    d.ro.vf[q] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.0f);
+    d.ro.wf[q] = make_float4((float)w.x, (float)w.y, (float)w.z, 0.0f);
+}
+
+// After the force kernel: pos_of[id] = s for the particles whose sorted
+// position changed this step (most keep theirs from one step to the next,
+// which spares the scattered write).  Until this kernel pos_of holds the
+// storage positions -- the previous step's sorted positions, where the
+// input set's records are -- which is what the force kernel and the next
+// step's bond kernel gather partner records through.
+__global__ __launch_bounds__(BLOCK) void k_pos_of(DemDev d) {
+    const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
+    if (q >= d.np) return;
+    if (d.perm[q] != q) d.pos_of[d.sid[q]] = q;
 }
 
 // ------------------------------------------------------------------- walls
@@ -1980,6 +2022,7 @@ void launch_forces(const DemDev& d, int precision, hipStream_t s) {
         if (d.force_vw) forces<false, true>(d, mixed, s);
         else forces<false, false>(d, mixed, s);
     }
+    k_pos_of<<<nblk(d.np), BLOCK, 0, s>>>(d);
 }
 void launch_contact(const DemDev& d, int precision, hipStream_t s) {
     launch_topology(d, s);
@@ -1992,6 +2035,7 @@ void launch_flops_probe(void* out, uint32_t blocks, uint32_t iters, bool fp64, h
     if (fp64) k_flops_probe<double><<<blocks, BLOCK, 0, s>>>(static_cast<double*>(out), iters, 0.999999, 1e-7);
     else k_flops_probe<float><<<blocks, BLOCK, 0, s>>>(static_cast<float*>(out), iters, 0.999999f, 1e-7f);
 }
+void launch_f32_copies(const DemDev& d, hipStream_t s) { k_f32_copies<<<nblk(d.np), BLOCK, 0, s>>>(d); }
 void launch_walls(const DemDev& d, hipStream_t s) {
     if (d.systems) {
         if (d.nw) k_wallsum<true><<<dim3(d.nwall_blocks, d.nw), BLOCK, 0, s>>>(d);

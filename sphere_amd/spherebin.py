@@ -36,6 +36,9 @@ import numpy as np
 
 MAXWALLS = 6
 PI_REF = 3.14159265358979  # src/constants.h:14
+# fixvel > 0 keeps a particle's x and y velocity (src/integration.cuh:65-77);
+# this build's extension: fixvel >= FIXVEL_HOLD_Z keeps its z velocity too
+FIXVEL_HOLD_Z = 2.0
 
 
 class BinFormatError(ValueError):

@@ -82,3 +82,23 @@ def run_pair(sb, tmpdir, steps_list, precision="fp64", setup=None):
     dem.close()
     orc.close()
     return errors
+
+
+# every field that carries state from one step to the next (per-step
+# outputs and sort fields are recomputed by the next step)
+HANDOVER_FIELDS = ["x", "vel", "angvel", "force", "torque", "angpos", "xysum", "vel0", "angvel0", "es_dot", "es",
+                   "ev_dot", "ev", "p", "contacts", "distmod", "delta_t"]
+
+
+def hand_over(dem, orc):
+    """Copies the GPU handle's whole state into the oracle (which must have
+    been opened on the same input), so that the oracle continues the GPU's
+    trajectory from this step on."""
+    fields = list(HANDOVER_FIELDS)
+    if orc.nb0:
+        fields += BOND_FIELDS
+    if orc.nw:
+        fields += ["walls_nx", "walls_mvfd", "walls_vel0"]
+    for f in fields:
+        orc.set(f, dem.get(f))
+    orc.set("time", dem.get("time"))

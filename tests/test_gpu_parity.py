@@ -12,6 +12,11 @@ from sphere_amd import configs
 pytestmark = pytest.mark.gpu
 
 
+# steps before the hand-over to the oracle, and steps checked after it, of
+# the bonded solid's breakage window
+BONDED_WINDOW = (4800, 200)
+
+
 def _say(msg):
     print(msg, flush=True)
     sys.stdout.flush()
@@ -179,6 +184,39 @@ def test_bonded_solid_full_size(tmp_path):
     assert sb.np == 262144 and sb.nb0 > 1_400_000
     errs = run_pair(sb, tmp_path, [1, 9, 15])
     assert not errs, errs
+
+
+def test_bonded_solid_breakage_window_full_size(tmp_path):
+    """The benchmark's bonded solid at full size (262,144 spheres, about 1.5M
+    bonds, strengths 1e5 Pa, top and bottom layers held at +-0.005 m/s):
+    the GPU runs the first BONDED_WINDOW[0] steps, hands its whole state to
+    the oracle, and both run on through the window in which bonds break --
+    breakage indices and every array bit for bit."""
+    from oracle_lib import Oracle
+    from parity import all_fields, compare_all, hand_over
+    from sphere_amd.dem import DEM
+
+    start, length = BONDED_WINDOW
+    _say(f"bonded solid breakage window {start}..{start + length}")
+    sb = configs.fcc_bonded(cells=(32, 32, 64), sigma_b=1e5, tau_b=1e5)
+    dem = DEM(sb, verbosity=0)
+    dem.step(start)
+    orc = Oracle.from_spherebin(sb, tmp_path)
+    hand_over(dem, orc)
+    broken0 = int((dem.get("bonds")[:, 0] >= sb.np).sum())
+    done = 0
+    while done < length:
+        k = min(50, length - done)
+        dem.step(k)
+        assert orc.step(k) == 0
+        done += k
+        errs = compare_all(dem, orc, all_fields(orc))
+        assert not errs, (start + done, errs)
+    broken = int((orc.get("bonds")[:, 0] >= sb.np).sum())
+    _say(f"broken bonds {broken0} -> {broken}")
+    dem.close()
+    orc.close()
+    assert broken > broken0
 
 
 def test_set_bonds_checks_pairs_and_rebuilds_the_bond_lists(tmp_path):

@@ -456,3 +456,26 @@ def test_sphere_past_the_periodic_boundary_is_hashed_wrapped(tmp_path):
     c = o.get("contacts")
     assert 1 in c[0] and 0 in c[1]
     assert abs(o.get("x")[0, 0] - 1e-3) < 1e-15
+
+
+def test_fixvel_hold_z_extension_keeps_the_whole_velocity(tmp_path):
+    """This build's extension: fixvel >= 2 holds vel_z as well (a velocity-
+    controlled layer); fixvel = 1 beside it is the reference's rule and is
+    pushed in z by the same floor contact."""
+    from sphere_amd.spherebin import FIXVEL_HOLD_Z
+
+    r, ov = 0.01, 1e-4
+    sb = _box(2, sid="hold-z", periodic=1)
+    sb.g = np.array([0.0, 0.0, -9.81])
+    sb.radius[:] = r
+    sb.x[0] = (0.3, 0.5, r - ov)
+    sb.x[1] = (0.7, 0.5, r - ov)
+    sb.vel[:] = (0.02, -0.01, 0.003)
+    sb.fixvel[:] = (FIXVEL_HOLD_Z, 1.0)
+    o = Oracle.from_spherebin(sb, tmp_path)
+    assert o.step(50) == 0
+    v = o.get("vel")
+    assert np.array_equal(v[0, :3], sb.vel[0])
+    np.testing.assert_allclose(o.get("x")[0, :3], sb.x[0] + 50 * sb.time_dt * sb.vel[0], rtol=1e-12)
+    assert np.array_equal(v[1, :2], sb.vel[1, :2]) and v[1, 2] > sb.vel[1, 2]
+    assert o.get("force")[0, 2] > 0  # the floor pushes, the held layer ignores it
