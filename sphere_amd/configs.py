@@ -268,6 +268,95 @@ def shear_cell(n=65536, box=None, stress=10e3, shear_velocity=0.1, gravity=-9.81
     return sb
 
 
+def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.5, mu=0.5, steps=2000,
+                            seed=BASE_SEED + 3, sid="c3-shear", settle_chunk=5000, max_settle_steps=400000,
+                            wall_steps=20000, log=None) -> Spherebin:
+    """The differentiable shear test's input: shear_cell's seeded lognormal
+    packing (config-1 radii, x/y periodic), compacted on the GPU under its
+    10 kPa top wall, then set up for shearing.
+
+    1. Settling under gravity with contact model 1 (contactLinearViscous,
+       src/contactmodels.cuh:114-253, whose normal dashpot damps: model 2's
+       feeds energy in on approach), gamma = 100 N s/m and no friction (a
+       frictionless packing jams with about six contacts per sphere), the top
+       wall parked at the top of the grid, until no sphere moves faster than
+       2 cm/s.
+    2. The top wall placed just above the packing and held at `stress`
+       (wmode 1, the reference's consolidate, python/sphere.py:1243-1258)
+       for `wall_steps` steps.
+    3. Contact model 2 with friction `mu` and a small normal dashpot
+       `gamma_n` (the adjoint's parameters), frictionless undamped walls, and
+       the shear set-up of shear_cell: the top layer driven at
+       `shear_velocity` in x and the bottom layer held (fixvel,
+       python/sphere.py:1297-1356).
+
+    Runs on the GPU (sphere_amd.dem).  `log(msg)` receives progress lines.
+    """
+    from .dem import DEM
+
+    say = log or (lambda msg: None)
+    sb = shear_cell(n=n, stress=stress, shear_velocity=shear_velocity, steps=steps, seed=seed, sid=sid)
+    r = np.asarray(sb.radius)
+    sb.fixvel[:] = 0.0
+    sb.vel[:] = 0.0
+    sb.contactmodel = 1
+    sb.gamma_n = sb.gamma_t = 100.0
+    sb.gamma_wn, sb.gamma_wt = 100.0, 0.0
+    sb.mu_s = sb.mu_d = 0.0
+    sb.mu_ws = sb.mu_wd = 0.0
+    sb.wmode = np.array([0], dtype=np.int32)
+    sb.w_x[0] = float(sb.L[2])
+    dem = DEM(sb, verbosity=0)
+    done, vmax = 0, math.inf
+    while done < max_settle_steps:
+        dem.step(settle_chunk)
+        done += settle_chunk
+        vmax = float(np.max(np.linalg.norm(dem.get("vel")[:, :3], axis=1)))
+        if done % (10 * settle_chunk) == 0:
+            say(f"settling: {done} steps, fastest sphere {vmax:.3g} m/s")
+        if done >= 40000 and vmax < 0.02:
+            break
+    say(f"settled after {done} steps (fastest sphere {vmax:.3g} m/s)")
+    sb = dem.to_spherebin()
+    dem.close()
+    top = float(np.max(sb.x[:, 2] + sb.radius))
+    sb.wmode = np.array([1], dtype=np.int32)
+    sb.w_x[0] = top + 0.1 * float(np.median(r))
+    sb.w_vel[0] = 0.0
+    sb.w_devs[0] = float(stress)
+    dem = DEM(sb, verbosity=0)
+    dem.step(wall_steps)
+    ij, _ = dem.findOverlaps()
+    cn = 2.0 * len(ij) / sb.np
+    wall = dem.get("walls_nx")[0, 3]
+    say(f"under the wall: {wall_steps} steps, wall at {wall:.4f} m (packing top {top:.4f}), C/N = {cn:.2f}")
+    sb = dem.to_spherebin()
+    dem.close()
+    sb.contactmodel = 2
+    sb.gamma_n, sb.gamma_t = float(gamma_n), 100.0
+    sb.mu_s = sb.mu_d = float(mu)
+    sb.gamma_wn = sb.gamma_wt = 0.0
+    sb.mu_ws = sb.mu_wd = 0.0
+    z = sb.x[:, 2]
+    rr = sb.radius
+    zmin, zmax = float(np.min(z - rr)), float(np.max(z + rr))
+    d_bot = 2.0 * float(np.max(rr[z < zmin + 0.3 * (zmax - zmin)]))
+    d_top = 2.0 * float(np.max(rr[z > zmin + 0.7 * (zmax - zmin)]))
+    bot = z < zmin + d_bot
+    topl = z > zmax - d_top
+    sb.vel[:] = 0.0
+    sb.angvel[:] = 0.0
+    sb.fixvel[:] = 0.0
+    sb.fixvel[bot | topl] = 1.0
+    sb.vel[topl, 0] = shear_velocity
+    sb.time_current = 0.0
+    sb.time_total = steps * sb.time_dt
+    sb.time_file_dt = sb.time_total
+    sb.time_step_count = 0
+    sb.sid = sid
+    return sb
+
+
 def dense_lattice(cells=(64, 64, 64), radius=1e-2, overlap=2e-5, jitter=1e-6, gravity=0.0, steps=2000,
                   seed=BASE_SEED + 5, sid="dense-fcc") -> Spherebin:
     """Confined FCC crystal, 4 x 64^3 = 1,048,576 spheres, every nearest

@@ -419,6 +419,166 @@ class Spherebin:
             z, p = dem.porosity(slices, exact=exact)
         return p, z
 
+    # ------------------------------------------- experiment set-up (seeded)
+    # The reference draws from the global, unseeded numpy.random; these take
+    # a seed or a numpy Generator so that a set-up is reproducible.
+
+    @staticmethod
+    def _rng(seed=None, rng=None) -> np.random.Generator:
+        return rng if rng is not None else np.random.default_rng(seed)
+
+    def generateRadii(self, psd: str = "logn", radius_mean: float = 440e-6, radius_variance: float = 8.8e-9,
+                      histogram: bool = False, seed=None, rng=None):
+        """python/sphere.py:789-822: radii from a log-normal distribution of
+        the given mean and variance (mu = ln(m^2 / sqrt(v + m^2)), sigma =
+        sqrt(ln(v / m^2 + 1))), or ``psd='uni'``: uniform in [mean -
+        variance, mean + variance] (the reference uses the variance as the
+        half-width).  The histogram plot is not drawn."""
+        g = self._rng(seed, rng)
+        if psd == "logn":
+            mu = math.log(radius_mean ** 2 / math.sqrt(radius_variance + radius_mean ** 2))
+            sigma = math.sqrt(math.log(radius_variance / radius_mean ** 2 + 1.0))
+            self.radius = g.lognormal(mu, sigma, self.np)
+        elif psd == "uni":
+            self.radius = g.uniform(radius_mean - radius_variance, radius_mean + radius_variance, self.np)
+        else:
+            raise ValueError(f"unknown particle size distribution {psd!r}")
+
+    def initRandomGridPos(self, g=(0.0, 0.0, -9.80665), gridnum=(12, 12, 32), periodic: int = 1,
+                          contactmodel: int = 2, seed=None, rng=None):
+        """python/sphere.py:1058-1115: every particle in its own cell of a
+        coarse grid of gridnum/2 cells of 4.2 r_max, offset at random inside
+        the cell without touching its faces; the world is then the smallest
+        whole number of such cells holding every sphere.  Radii must be set
+        first.  As in the reference, a particle's y cell index wraps with the
+        x cell count."""
+        gen = self._rng(seed, rng)
+        self.g = np.asarray(g, dtype=np.float64).copy()
+        self.periodic = int(periodic)
+        coarse = np.floor(np.asarray(gridnum, dtype=np.float64) / 2.0)
+        r_max = float(np.max(self.radius))
+        cellsize = 2.1 * r_max * 2
+        if (coarse[0] - 1) * (coarse[1] - 1) * (coarse[2] - 1) < self.np:
+            raise ValueError("Error! The grid is not sufficiently large.")
+        i = np.arange(self.np, dtype=np.float64)
+        gridpos = np.stack([i % coarse[0], np.floor(i / coarse[0]) % coarse[0],
+                            np.floor(i / (coarse[0] * coarse[1]))], axis=1)
+        r = (self.radius * 1.05)[:, None]
+        u = gen.random_sample((self.np, 3)) if hasattr(gen, "random_sample") else gen.random((self.np, 3))
+        self.x = gridpos * cellsize + ((cellsize - r) - r) * u + r
+        self.contactmodel = int(contactmodel)
+        top = np.max(self.x + self.radius[:, None], axis=0)
+        self.num = np.ceil(top / cellsize).astype(np.uint32)
+        self.L = self.num.astype(np.float64) * cellsize
+
+    def adjustWall(self, idx: int, adjust: float = 1.1):
+        """python/sphere.py:1214-1240: grid and wall `idx` fitted to the
+        particles along the wall's axis (0: z, 1-2: x, 3-4: y); the wall
+        sits at the particles' extreme and weighs rho np pi (cell/2)^3."""
+        dim = {0: 2, 1: 0, 2: 0, 3: 1, 4: 1}.get(idx)
+        if dim is None:
+            raise ValueError("adjustWall: idx value not understood")
+        xmin = float(np.min(self.x[:, dim] - self.radius))
+        xmax = float(np.max(self.x[:, dim] + self.radius))
+        cellsize = float(self.L[0] / self.num[0])
+        self.num[dim] = np.uint32(math.ceil(((xmax - xmin) * adjust + xmin) / cellsize))
+        self.L[dim] = (xmax - xmin) * adjust + xmin
+        self.w_x[idx] = xmax if idx in (0, 1, 3) else xmin
+        self.w_m[idx] = self.rho * self.np * math.pi * (cellsize / 2.0) ** 3
+
+    def adjustUpperWall(self, z_adjust: float = 1.1):
+        """python/sphere.py:1199-1212: one fixed upper wall (normal -z) at the
+        top of the packing, the grid 10 % higher."""
+        self.set_walls(1)
+        self.wmode = np.zeros(1, dtype=np.int32)
+        self.w_n = np.array([[0.0, 0.0, -1.0]])
+        self.adjustWall(0, adjust=z_adjust)
+
+    def consolidate(self, deviatoric_stress: float = 10e3, periodic: int = 1):
+        """python/sphere.py:1243-1257: zero the kinematics, fit the upper
+        wall to the packing and hold it at `deviatoric_stress` (wmode 1).
+        (`periodic` is accepted and unused, as in the reference.)"""
+        self.zeroKinematics()
+        self.adjustUpperWall()
+        self.wmode = np.array([1], dtype=np.int32)
+        self.w_devs = np.ones(1) * float(deviatoric_stress)
+
+    def uniaxialStrainRate(self, wvel: float = -0.001, periodic: int = 1):
+        """python/sphere.py:1259-1271: zero the kinematics, fit the upper
+        wall and move it at the fixed velocity `wvel` (wmode 2)."""
+        self.zeroKinematics()
+        self.adjustUpperWall()
+        self.wmode = np.array([2], dtype=np.int32)
+        self.w_vel = np.array([float(wvel)])
+
+    def shear(self, shear_strain_rate: float = 1.0, periodic: int = 1):
+        """python/sphere.py:1297-1355: one more cell of grid height for
+        dilation, zero kinematics, the lowest particles (centres below z_min
+        + the largest diameter among those below 0.3 (z_max - z_min)) held
+        at zero horizontal velocity and the highest ones (above z_max - the
+        largest diameter among those above 0.7 (z_max - z_min)) moved at
+        (z_max - z_min) * shear_strain_rate in x, with fixvel = 1; frictionless
+        undamped walls.  The 0.3 / 0.7 selections compare absolute heights,
+        as the reference does."""
+        z = self.x[:, 2]
+        z_min = float(np.min(z - self.radius))
+        z_max = float(np.max(z + self.radius))
+        cellsize = float(self.L[0] / self.num[0])
+        self.num[2] += 1
+        self.L[2] = self.num[2] * cellsize
+        self.zeroKinematics()
+        d_bot = float(np.max(self.radius[z < (z_max - z_min) * 0.3])) * 2.0
+        low = z < (z_min + d_bot)
+        self.fixvel[low] = 1.0
+        self.angvel[low] = 0.0
+        self.vel[low, 0] = 0.0
+        self.vel[low, 1] = 0.0
+        d_top = float(np.max(self.radius[z > (z_max - z_min) * 0.7])) * 2.0
+        high = z > (z_max - d_top)
+        self.fixvel[high] = 1.0
+        self.angvel[high] = 0.0
+        self.vel[high, 0] = (z_max - z_min) * shear_strain_rate
+        self.vel[high, 1] = 0.0
+        if self.nw > 0:
+            self.gamma_wn = 0.0
+            self.gamma_wt = 0.0
+            self.mu_ws = 0.0
+            self.mu_wd = 0.0
+
+    def run(self, verbose: bool = True, hideinputfile: bool = False, dry: bool = False, root: str = ".") -> int:
+        """python/sphere.py:1629-1655: run the `sphere` executable on
+        ``<root>/input/<sid>.bin`` from ``root`` (outputs in
+        ``<root>/output/``); the input is written first.  Returns the exit
+        status, with the reference's warning when it is not 0.  Valgrind,
+        cuda-memcheck and the Darcy solver are not part of this path."""
+        import subprocess
+
+        exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sphere")
+        os.makedirs(os.path.join(root, "input"), exist_ok=True)
+        self.writebin(os.path.join(root, "input", f"{self.sid}.bin"))
+        cmd = [exe]
+        if not verbose:
+            cmd.append("-q")
+        if dry:
+            cmd.append("--dry")
+        cmd.append(os.path.join("input", f"{self.sid}.bin"))
+        out = subprocess.DEVNULL if hideinputfile else None
+        status = subprocess.call(cmd, cwd=root, stdout=out)
+        if status != 0:
+            print("Warning: the sphere run returned with status " + str(status))
+        return status
+
+    def readlast(self, verbose: bool = True, root: str = "."):
+        """python/sphere.py:783-787: read the last output file of this sid,
+        the file number taken from ``<root>/output/<sid>.status.dat``."""
+        last = status(self.sid, root)
+        fn = os.path.join(root, "output", "{0}.output{1:0=5}.bin".format(self.sid, last))
+        other = Spherebin.readbin(fn)
+        other.sid = self.sid
+        self.__dict__.update(other.__dict__)
+        if verbose:
+            print("Read " + fn)
+
     def wall_stress(self, idx: int = 0) -> float:
         """Normal stress on a wall, w_force / (L0*L1) (python/sphere.py:2594,2724)."""
         return float(self.w_force[idx] / (self.L[0] * self.L[1]))
@@ -431,3 +591,10 @@ class Spherebin:
         """
         sel = (self.fixvel > 0.0) & (self.vel[:, 0] > 0.0)
         return float(-np.sum(self.force[sel, 0]) / (self.L[0] * self.L[1]))
+
+
+def status(project: str, root: str = ".") -> int:
+    """python/sphere.py:2892-2907: the last output file number written to
+    ``<root>/output/<project>.status.dat`` (its third field)."""
+    with open(os.path.join(root, "output", f"{project}.status.dat")) as fh:
+        return int(fh.read().split()[2])
