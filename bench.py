@@ -311,12 +311,29 @@ def analysis_run(dpath, reps=5, with_cpu=True):
     return out
 
 
-def ensemble_run(members, steps, warmup, sample_every=32, sample_steps=100):
+def regrid(sb, factor):
+    """A member on the grid sph_ens_open gives it for cell_factor >= 1
+    (cells of factor x 2 r_max, at least 3 on a periodic axis)."""
+    import math
+
+    sb = sb.copy()
+    if factor >= 1.0:
+        edge = factor * 2.0 * float(np.max(sb.radius))
+        num = []
+        for a in range(3):
+            per = (a == 0 and sb.periodic in (1, 2)) or (a == 1 and sb.periodic == 1)
+            num.append(max(math.floor(sb.L[a] / edge), 3 if per else 1))
+        sb.num = np.array(num, dtype=np.uint32)
+    return sb
+
+
+def ensemble_run(members, steps, warmup, sample_every=1, sample_steps=None):
     """The calibration ensemble in one handle (one launch of each kernel per
     step for every member, graph-replayed), mixed precision, grid and block
     sizes from the autotuner; against the same members run one at a time,
-    each in its own handle (a sample of every `sample_every`-th member,
-    extrapolated by member count)."""
+    each in its own handle for the same steps (every member by default;
+    `sample_every` > 1 extrapolates a sample by member count)."""
+    sample_steps = sample_steps or steps
     from sphere_amd.dem import DEM, Ensemble
 
     n_total = sum(m.np for m in members)
@@ -339,15 +356,15 @@ def ensemble_run(members, steps, warmup, sample_every=32, sample_steps=100):
     sample = members[::sample_every]
     per = []
     for sb in sample:
-        d = DEM(sb, verbosity=0)
+        d = DEM(regrid(sb, info.cell_factor), verbosity=0)
         d.set_precision("mixed")
         d.step(warmup)
         t0 = time.perf_counter()
         d.step(sample_steps)
         per.append((time.perf_counter() - t0) * 1e3 / sample_steps)
         d.close()
-    ms_one = float(np.mean(per)) * len(members)
-    log(f"one at a time: {np.mean(per):.4f} ms/step per member (sample of {len(sample)}), {ms_one:.2f} ms per "
+    ms_one = float(np.sum(per)) * sample_every
+    log(f"one at a time: {np.mean(per):.4f} ms/step per member ({len(sample)} members timed), {ms_one:.2f} ms per "
         f"ensemble step")
     return {"workload": f"{len(members)} packings, {n_total:,} spheres (counts uniform in [1024, 32768]), k_n uniform "
                         "in [1e5, 1e7], mu uniform in [0.2, 0.8], C1 radii, settling under a 10 kPa top wall, mixed "
@@ -358,8 +375,8 @@ def ensemble_run(members, steps, warmup, sample_every=32, sample_steps=100):
                           "search_kernel": SEARCH_KERNELS.get(info.search_mode),
                           "force_block": info.force_block, "search_block": info.search_block},
             "one_at_a_time": {"value": n_total / (ms_one * 1e-3), "ms_per_ensemble_step": ms_one,
-                              "ms_per_member_step": float(np.mean(per)), "sample_members": len(sample),
-                              "sample_steps": sample_steps},
+                              "ms_per_member_step": float(np.mean(per)), "members_timed": len(sample),
+                              "steps": sample_steps, "grid": "each member on the ensemble's autotuned grid"},
             "speedup_vs_one_at_a_time": ms_one / ms}
 
 

@@ -16,7 +16,9 @@
 //    negative floor to unsigned);
 //  * a contact that finds neither its own slot nor an empty one, or more than
 //    NC overlaps, raises ORC_F_CONTACT_OVERFLOW and ends the particle's search
-//    (src/contactsearch.cuh:252-284 writes past the slot array);
+//    (src/contactsearch.cuh:252-284 writes the partner into slot nc, the
+//    count of overlaps found so far: over another contact when nc < NC, one
+//    past the particle's slots when nc = NC);
 //  * delta_t starts at zero and a tangential history that stays untouched
 //    keeps its projected value (src/contactmodels.cuh:357,487-491 leave it
 //    uninitialised);

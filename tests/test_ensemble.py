@@ -242,3 +242,49 @@ def test_calibration_ensemble_members_match_single_handles():
         d.close()
     ens.close()
     assert not errs, errs[:8]
+
+
+@pytest.mark.gpu
+def test_calibration_ensemble_full_size(tmp_path):
+    """The calibration ensemble at its stated size: 512 members (uneven
+    counts in [1,024, 32,768], about 8.5M spheres), autotuned grid, 1,000
+    mixed-precision steps in one handle -- every member bit-identical to its
+    own handle on the regridded input after the same steps.  Then the same
+    ensemble in FP64 for 1,000 steps, its two largest members against the
+    CPU oracle."""
+    from oracle_lib import Oracle
+    from parity import STATE_FIELDS, first_mismatch
+    from sphere_amd.dem import DEM, Ensemble
+
+    steps = 1000
+    ms = configs.calibration_ensemble(members=512, steps=steps)
+    n_total = sum(m.np for m in ms)
+    _say(f"calibration ensemble full size: {len(ms)} members, {n_total} spheres")
+    ens = Ensemble(ms, cell_factor=0.0)
+    factor = ens.info().cell_factor
+    ens.set_precision("mixed")
+    ens.step(steps)
+    errs = []
+    for k, m in enumerate(ms):
+        d = DEM(regridded(m.copy(), factor), verbosity=0)
+        d.set_precision("mixed")
+        d.step(steps)
+        errs += _compare(ens, k, d, _fields(m))
+        d.close()
+        if k % 128 == 127:
+            _say(f"  {k + 1} members compared")
+    ens.close()
+    assert not errs, errs[:8]
+    _say("FP64 ensemble against the oracle, two largest members")
+    ens = Ensemble(ms, cell_factor=factor)
+    ens.step(steps)
+    for k in np.argsort([m.np for m in ms])[-2:]:
+        orc = Oracle.from_spherebin(regridded(ms[k].copy(), factor), tmp_path)
+        assert orc.step(steps) == 0
+        for f in STATE_FIELDS + ["contacts", "delta_t", "walls_nx", "walls_mvfd", "wall_pp"]:
+            mm = first_mismatch(f, ens.get(int(k), f), orc.get(f))
+            if mm:
+                errs.append(f"member {k}: {mm}")
+        orc.close()
+    ens.close()
+    assert not errs, errs[:8]

@@ -182,6 +182,10 @@ def test_slot_rule_first_match_wins(tmp_path):
 
 
 def test_slot_table_full_is_overflow(tmp_path):
+    """Every slot holds another partner and the new overlap finds no slot of
+    its own: the reference would write it into slot nc (here 0, the first
+    overlap of the search), overwriting a stored contact; this build stops
+    with CONTACT_OVERFLOW instead."""
     o, sb = _pair_chain(tmp_path, nb=2)
     c = o.get("contacts")
     c[0, :] = 2          # partner 2 fills every slot: partner 1 finds no room
@@ -192,8 +196,9 @@ def test_slot_table_full_is_overflow(tmp_path):
 
 
 def test_more_than_nc_overlaps_is_overflow(tmp_path):
-    """A large sphere among > 32 small ones (the reference writes past its
-    slot array, src/contactsearch.cuh:283-284)."""
+    """A large sphere among > 32 small ones (the reference writes the 33rd
+    partner into slot nc = 32, one past its slot array, then stops,
+    src/contactsearch.cuh:252-284)."""
     n = 41
     sb = Spherebin(n, nw=0, sid="crowd")
     sb.set_walls(0)
