@@ -37,6 +37,13 @@
 
 #include "dem_oracle.h"
 
+// Planted mistakes for the pin tests' own check (tests/test_oracle_mutants.py
+// builds the oracle with -DORC_MUTANT=k and expects the closed-form pins to
+// fail); 0, the default, is the restatement itself.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -486,7 +493,8 @@ double contact_linear_wall(orc_t* s, Acc& acc, const D4& linvel4, V3 angvel, dou
     const V3 linvel = v3(linvel4.x, linvel4.y, linvel4.z);
     // operator+(Float3, Float) adds the scalar wall velocity to every component
     const V3 vel0 = add3(linvel, smul(radius_a, cross3(n, angvel)));
-    const V3 vel = v3(vel0.x + wvel, vel0.y + wvel, vel0.z + wvel);
+    V3 vel = v3(vel0.x + wvel, vel0.y + wvel, vel0.z + wvel);
+    if (ORC_MUTANT == 4) vel = v3(vel0.x, vel0.y, vel0.z + wvel);
     const double vel_n = dot3(vel, n);
     const V3 vel_t = sub3(vel, muls(n, dot3(vel, n)));
     const double vel_t_length = len3(vel_t);
@@ -498,13 +506,14 @@ double contact_linear_wall(orc_t* s, Acc& acc, const D4& linvel4, V3 angvel, dou
     if (vel_t_length > 0.0) {
         const double f_t_visc = s->gamma_wt * vel_t_length;
         double f_t_limit;
-        if (vel_t_length > (double)0.001f) f_t_limit = s->mu_wd * f_n_length;
+        const bool dynamic = ORC_MUTANT == 2 ? vel_t_length < (double)0.001f : vel_t_length > (double)0.001f;
+        if (dynamic) f_t_limit = s->mu_wd * f_n_length;
         else f_t_limit = s->mu_ws * f_n_length;
-        if (f_t_visc < f_t_limit) f_t = divs(smul(-1.0 * f_t_visc, vel_t), vel_t_length);
-        else f_t = divs(smul(-1.0 * f_t_limit, vel_t), vel_t_length);
+        if (f_t_visc < f_t_limit) f_t = divs(smul((ORC_MUTANT == 1 ? 1.0 : -1.0) * f_t_visc, vel_t), vel_t_length);
+        else f_t = divs(smul((ORC_MUTANT == 5 ? 1.0 : -1.0) * f_t_limit, vel_t), vel_t_length);
     }
     acc.F = add3(acc.F, add3(f_n, f_t));
-    acc.T = add3(acc.T, add3(smul(-radius_a, cross3(n, f_t)), T_res));
+    acc.T = add3(acc.T, add3(smul(ORC_MUTANT == 3 ? radius_a : -radius_a, cross3(n, f_t)), T_res));
     acc.p += f_n_length / (4.0 * PI * radius_a * radius_a);
     return dot3(f_n, n);
 }
@@ -779,10 +788,11 @@ static void stage_integrate(orc_t* s) {
                        torque.y * 1.0 / (2.0 / 5.0 * m * radius * radius),
                        torque.z * 1.0 / (2.0 / 5.0 * m * radius * radius));
         acc.x += s->g[0]; acc.y += s->g[1]; acc.z += s->g[2];
-        if (vel.w > 0.0) {
-            acc.x = 0.0; acc.y = 0.0; acc.z -= s->g[2];
-            if (vel.w >= ORC_FIXVEL_HOLD_Z) acc.z = 0.0;  // extension (header note)
-            angacc = v3(0.0, 0.0, 0.0);
+        if (vel.w > 0.0 && ORC_MUTANT != 6) {
+            acc.x = 0.0; acc.y = 0.0;
+            if (ORC_MUTANT != 7) acc.z -= s->g[2];
+            if (vel.w >= ORC_FIXVEL_HOLD_Z && ORC_MUTANT != 8) acc.z = 0.0;  // extension (header note)
+            if (ORC_MUTANT != 9) angacc = v3(0.0, 0.0, 0.0);
         }
         wrap_position(s, x);
         vel0.x += acc.x * dt; vel0.y += acc.y * dt; vel0.z += acc.z * dt;

@@ -14,6 +14,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "build", "liboracle_dem.so")
+# a planted-mistake build of the oracle, for the pin tests' own check
+MUTANT_LIB = os.environ.get("SPH_ORACLE_MUTANT_LIB")
 
 FIELDS = {
     "x": 0, "vel": 1, "angvel": 2, "force": 3, "torque": 4, "angpos": 5, "xysum": 6,
@@ -40,8 +42,11 @@ def ensure_built():
 def lib():
     global _lib
     if _lib is None:
-        ensure_built()
-        L = C.CDLL(LIB)
+        if MUTANT_LIB:
+            L = C.CDLL(MUTANT_LIB)
+        else:
+            ensure_built()
+            L = C.CDLL(LIB)
         L.orc_open.restype = C.c_void_p
         L.orc_open.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_size_t]
         L.orc_close.argtypes = [C.c_void_p]
