@@ -114,6 +114,7 @@ typedef struct sph_timing {
     int32_t search_mode;         /* contact-search kernel in use (sph_set_search) */
     int32_t search_auto;         /* 1 when the kernel is chosen by timing */
     int32_t force_variant;       /* k_force in use: 1 per-slot velocity gathers, 2 gathered ahead (sph_set_force_kernel) */
+    uint32_t list_builds;        /* neighbour-list builds (search mode 5) since the handle was opened */
 } sph_timing_t;
 
 /* DEM::DEM + readbin + checkValues + allocateGlobalDeviceMemory +

@@ -268,27 +268,38 @@ def shear_cell(n=65536, box=None, stress=10e3, shear_velocity=0.1, gravity=-9.81
     return sb
 
 
-def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.5, mu=0.5, steps=2000,
-                            seed=BASE_SEED + 3, sid="c3-shear", settle_chunk=5000, max_settle_steps=400000,
-                            wall_steps=20000, log=None) -> Spherebin:
+def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.5, gamma_t=10.0, mu=0.5,
+                            steps=2000, seed=BASE_SEED + 3, sid="c3-shear", fall_steps=40000, settle_steps=100000,
+                            settle_gamma=10.0, wall_steps=20000, wall_mass_scale=1e-3, chunk=5000,
+                            log=None) -> Spherebin:
     """The differentiable shear test's input: shear_cell's seeded lognormal
     packing (config-1 radii, x/y periodic), compacted on the GPU under its
     10 kPa top wall, then set up for shearing.
 
     1. Settling under gravity with contact model 1 (contactLinearViscous,
        src/contactmodels.cuh:114-253, whose normal dashpot damps: model 2's
-       feeds energy in on approach), gamma = 100 N s/m and no friction (a
-       frictionless packing jams with about six contacts per sphere), the top
-       wall parked at the top of the grid, until no sphere moves faster than
-       2 cm/s.
+       feeds energy in on approach) and no friction (a frictionless packing
+       jams with about six contacts per sphere), the top wall parked at the
+       top of the grid: `fall_steps` steps of free fall onto the floor, then
+       every `chunk` steps up to `settle_steps` every sphere is stopped
+       (velocities and half-step velocities zeroed), so that the small
+       spheres the collapse throws up fall back and come to rest.  The
+       dashpot is `settle_gamma` = 10 N s/m: the explicit update of a dashpot
+       is stable only while dt * (sum of the dashpots on a sphere) / m stays
+       below 2, and with the smallest spheres (m = 1.4e-3 kg) touching a
+       dozen others at dt = 6.3e-6 s, 100 N s/m is past that.
     2. The top wall placed just above the packing and held at `stress`
        (wmode 1, the reference's consolidate, python/sphere.py:1243-1258)
-       for `wall_steps` steps.
+       for `wall_steps` steps, its mass the reference's default times
+       `wall_mass_scale` (0.001, as the reference's own shear example scales
+       it, python/shear-test.py:77), so that it reaches the packing early.
     3. Contact model 2 with friction `mu` and a small normal dashpot
-       `gamma_n` (the adjoint's parameters), frictionless undamped walls, and
-       the shear set-up of shear_cell: the top layer driven at
-       `shear_velocity` in x and the bottom layer held (fixvel,
-       python/sphere.py:1297-1356).
+       `gamma_n` (the adjoint's parameters), a tangential dashpot `gamma_t`
+       within the same stability limit (10 N s/m: the tangential force also
+       spins the sphere up, 3.5 times its effect on the sliding speed alone),
+       frictionless undamped walls, and the shear set-up of shear_cell: the
+       top layer driven at `shear_velocity` in x and the bottom layer held
+       (fixvel, python/sphere.py:1297-1356).
 
     Runs on the GPU (sphere_amd.dem).  `log(msg)` receives progress lines.
     """
@@ -297,26 +308,48 @@ def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.
     say = log or (lambda msg: None)
     sb = shear_cell(n=n, stress=stress, shear_velocity=shear_velocity, steps=steps, seed=seed, sid=sid)
     r = np.asarray(sb.radius)
+    dt = float(sb.time_dt)
+
+    def fresh_clock(s, nsteps):
+        s.time_current = 0.0
+        s.time_total = (nsteps + 1) * dt
+        s.time_file_dt = s.time_total
+        s.time_step_count = 0
+
+    def fastest(d):
+        return float(np.max(np.linalg.norm(d.get("vel")[:, :3], axis=1)))
+
+    def stop_all(d):
+        v = d.get("vel")
+        v[:, :3] = 0.0
+        d.set("vel", v)
+        z3 = np.zeros((sb.np, 3))
+        d.set("vel0", z3)
+        d.set("angvel", z3)
+        d.set("angvel0", z3)
+
     sb.fixvel[:] = 0.0
     sb.vel[:] = 0.0
     sb.contactmodel = 1
-    sb.gamma_n = sb.gamma_t = 100.0
-    sb.gamma_wn, sb.gamma_wt = 100.0, 0.0
+    sb.gamma_n = sb.gamma_t = float(settle_gamma)
+    sb.gamma_wn, sb.gamma_wt = float(settle_gamma), 0.0
     sb.mu_s = sb.mu_d = 0.0
     sb.mu_ws = sb.mu_wd = 0.0
     sb.wmode = np.array([0], dtype=np.int32)
     sb.w_x[0] = float(sb.L[2])
+    fresh_clock(sb, settle_steps)
     dem = DEM(sb, verbosity=0)
-    done, vmax = 0, math.inf
-    while done < max_settle_steps:
-        dem.step(settle_chunk)
-        done += settle_chunk
-        vmax = float(np.max(np.linalg.norm(dem.get("vel")[:, :3], axis=1)))
-        if done % (10 * settle_chunk) == 0:
-            say(f"settling: {done} steps, fastest sphere {vmax:.3g} m/s")
-        if done >= 40000 and vmax < 0.02:
-            break
-    say(f"settled after {done} steps (fastest sphere {vmax:.3g} m/s)")
+    done = 0
+    while done < settle_steps:
+        k = min(chunk, settle_steps - done)
+        dem.step(k)
+        done += k
+        if done % (4 * chunk) == 0 or done == settle_steps:
+            x = dem.get("x")
+            say(f"settling: {done} steps, fastest sphere {fastest(dem):.3g} m/s, "
+                f"highest top {float(np.max(x[:, 2] + x[:, 3])):.4f} m")
+        if done >= fall_steps:
+            stop_all(dem)
     sb = dem.to_spherebin()
     dem.close()
     top = float(np.max(sb.x[:, 2] + sb.radius))
@@ -324,16 +357,23 @@ def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.
     sb.w_x[0] = top + 0.1 * float(np.median(r))
     sb.w_vel[0] = 0.0
     sb.w_devs[0] = float(stress)
+    sb.w_m[0] = float(sb.w_m[0]) * wall_mass_scale
+    fresh_clock(sb, wall_steps)
     dem = DEM(sb, verbosity=0)
-    dem.step(wall_steps)
+    done = 0
+    while done < wall_steps:
+        k = min(chunk, wall_steps - done)
+        dem.step(k)
+        done += k
     ij, _ = dem.findOverlaps()
     cn = 2.0 * len(ij) / sb.np
     wall = dem.get("walls_nx")[0, 3]
-    say(f"under the wall: {wall_steps} steps, wall at {wall:.4f} m (packing top {top:.4f}), C/N = {cn:.2f}")
+    say(f"under the wall: {wall_steps} steps, wall at {wall:.4f} m (packing top {top:.4f}), C/N = {cn:.2f}, "
+        f"fastest sphere {fastest(dem):.3g} m/s")
     sb = dem.to_spherebin()
     dem.close()
     sb.contactmodel = 2
-    sb.gamma_n, sb.gamma_t = float(gamma_n), 100.0
+    sb.gamma_n, sb.gamma_t = float(gamma_n), float(gamma_t)
     sb.mu_s = sb.mu_d = float(mu)
     sb.gamma_wn = sb.gamma_wt = 0.0
     sb.mu_ws = sb.mu_wd = 0.0

@@ -144,7 +144,7 @@ struct DemDev {
     float* ldisp;       // mode 5: displacement since the build, per list index (FP32)
     uint32_t* linv;     // mode 5: list index -> sorted position of this step
     uint32_t* slidx;    // mode 5: list index per sorted position of this step
-    uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed
+    uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed, [3] builds so far
     float list_skin;    // mode 5: list reach beyond r_a + r_b (m)
     float list_lim;     // mode 5: displacement since the build that requests a rebuild (m)
     uint32_t list_shell;  // mode 5: list builds also scan the cells two away

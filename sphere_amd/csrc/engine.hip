@@ -71,6 +71,14 @@ sph::OwnSet own_set(const sph::PState& p) {
 // the buffer set holding the current private records
 const sph::PState& own(const sph_dem* h) { return h->own_cur ? h->d.a : h->d.b; }
 
+// A range of particles: original indices [base, base + n), which are also
+// the storage positions they occupy -- all of a handle's particles, or one
+// ensemble member's (the sort keeps each member inside its own range).
+struct Range {
+    size_t base, n;
+};
+Range whole(const sph_dem* h) { return {0, h->hs.np}; }
+
 // Margin of the search's FP32 distance prefilter: every coordinate is below
 // M = max |origo| + 2 L in magnitude, so converting, subtracting and shifting
 // by a periodic image errs by at most 3.5 ulp(M) per axis and 6.1 ulp(M) in
@@ -487,22 +495,32 @@ int upload_all(sph_dem* h) {
     return SPH_OK;
 }
 
-// Copies a storage-order double4 array into original order (w kept).
-int fetch4(sph_dem* h, const double4* src, const std::vector<uint32_t>& id, std::vector<double>& out) {
-    const size_t n = h->hs.np;
+// Copies a storage-order double4 array into original order (w kept), for the
+// particles of range r (ids relative to r.base, as fetch_ids returns them).
+int fetch4(sph_dem* h, const double4* src, const std::vector<uint32_t>& id, std::vector<double>& out, Range r) {
+    const size_t n = r.n;
     std::vector<double> t(4 * n);
-    HIP_TRY(hipMemcpy(t.data(), src, n * 32, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(t.data(), src + r.base, n * 32, hipMemcpyDeviceToHost));
     out.resize(4 * n);
     for (size_t q = 0; q < n; ++q)
         for (int a = 0; a < 4; ++a) out[4 * (size_t)id[q] + a] = t[4 * q + a];
     return SPH_OK;
 }
 
-int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) {
-    id.resize(h->hs.np);
-    HIP_TRY(hipMemcpy(id.data(), own(h).id, h->hs.np * 4, hipMemcpyDeviceToHost));
+int fetch4(sph_dem* h, const double4* src, const std::vector<uint32_t>& id, std::vector<double>& out) {
+    return fetch4(h, src, id, out, whole(h));
+}
+
+// Original indices of the records at the storage positions of range r,
+// relative to r.base.
+int fetch_ids(sph_dem* h, std::vector<uint32_t>& id, Range r) {
+    id.resize(r.n);
+    HIP_TRY(hipMemcpy(id.data(), own(h).id + r.base, r.n * 4, hipMemcpyDeviceToHost));
+    for (auto& v : id) v -= (uint32_t)r.base;
     return SPH_OK;
 }
+
+int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) { return fetch_ids(h, id, whole(h)); }
 
 int alloc_pstate(sph_dem* h, sph::PState& p, size_t n) {
     ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.pr0, n); ALLOC(p.pr1, n);
@@ -909,6 +927,7 @@ std::vector<Region> state_regions(sph_dem* h) {
     r.push_back({d.wvel0, 6 * ns * 8});
     r.push_back({d.t_cur, ns * 8});
     r.push_back({d.flags, 8});
+    if (d.nstate) r.push_back({d.nstate, 16});  // list-build requests and count: builds on a copy do not count
     r.push_back({d.force, n * 32});
     r.push_back({d.torque, n * 32});
     r.push_back({d.pres, n * 8});
@@ -1064,16 +1083,17 @@ std::vector<Comp> packed_comps(const sph::PState& p, int f) {
     }
 }
 
-// storage-order arrays -> original-order values, comps.size() per particle
-int get_comps(sph_dem* h, const std::vector<Comp>& comps, double* out) {
-    const size_t n = h->hs.np, nc = comps.size();
+// storage-order arrays -> original-order values, comps.size() per particle,
+// for the particles of range r
+int get_comps(sph_dem* h, const std::vector<Comp>& comps, double* out, Range r) {
+    const size_t n = r.n, nc = comps.size();
     std::vector<uint32_t> id;
     int rc;
-    if ((rc = fetch_ids(h, id))) return rc;
+    if ((rc = fetch_ids(h, id, r))) return rc;
     std::vector<double> t(4 * n);
     for (size_t a = 0; a < nc; ++a) {
         if (a == 0 || comps[a].arr != comps[a - 1].arr)
-            HIP_TRY(hipMemcpy(t.data(), comps[a].arr, n * 32, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(t.data(), comps[a].arr + r.base, n * 32, hipMemcpyDeviceToHost));
         for (size_t q = 0; q < n; ++q) out[nc * id[q] + a] = t[4 * q + comps[a].c];
     }
     return SPH_OK;
@@ -1094,6 +1114,90 @@ int set_comps(sph_dem* h, const std::vector<Comp>& comps, const double* in) {
         if (last) HIP_TRY(hipMemcpy(comps[a].arr, t.data(), n * 32, hipMemcpyHostToDevice));
     }
     return SPH_OK;
+}
+
+
+// Per-particle fields of the particles of range r, in their original order
+// relative to r.base (contact partners too; an empty slot reads r.n).
+// Returns 1 for a field that is not per particle.
+int get_particle_field(sph_dem* h, int f, void* dst, Range r) {
+    const DemDev& d = h->d;
+    const size_t n = r.n, nc = sph::NC;
+    double* dd = static_cast<double*>(dst);
+    uint32_t* du = static_cast<uint32_t*>(dst);
+    int rc;
+    std::vector<uint32_t> id;
+    std::vector<double> t;
+    // per-particle double4 record in storage order -> comps [c0, c0+nc_) in original order
+    auto gather = [&](const double4* src, int c0, int ncomp) -> int {
+        int q;
+        if ((q = fetch_ids(h, id, r)) || (q = fetch4(h, src, id, t, r))) return q;
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < ncomp; ++a) dd[(size_t)ncomp * i + a] = t[4 * i + c0 + a];
+        return SPH_OK;
+    };
+    switch (f) {
+        case SPH_X: return gather(own(h).x, 0, 4);
+        case SPH_VEL: return gather(own(h).vel, 0, 4);
+        case SPH_ANGVEL: return gather(own(h).angvel, 0, 3);
+        case SPH_FORCE: return gather(d.force, 0, 3);
+        case SPH_TORQUE: return gather(d.torque, 0, 3);
+        case SPH_ANGPOS: case SPH_VEL0: case SPH_ANGVEL0: case SPH_XYSUM: case SPH_ES: case SPH_EV:
+            return get_comps(h, packed_comps(own(h), f), dd, r);
+        case SPH_ES_DOT: return gather(d.force, 3, 1);
+        case SPH_EV_DOT: return gather(d.torque, 3, 1);
+        case SPH_P: {
+            std::vector<double> pv(n);
+            if ((rc = fetch_ids(h, id, r))) return rc;
+            HIP_TRY(hipMemcpy(pv.data(), d.pres + r.base, n * 8, hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < n; ++q) dd[id[q]] = pv[q];
+            return SPH_OK;
+        }
+        case SPH_CONTACTS: case SPH_DISTMOD: case SPH_DELTA_T: {
+            std::vector<uint32_t> smask(n), mask(n), slot(n * nc);
+            if ((rc = fetch_ids(h, id, r))) return rc;
+            HIP_TRY(hipMemcpy(smask.data(), own(h).cmask + r.base, n * 4, hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < n; ++q) mask[id[q]] = smask[q];
+            HIP_TRY(hipMemcpy(slot.data(), d.cslot + r.base * nc, n * nc * 4, hipMemcpyDeviceToHost));
+            auto occ = [&](size_t i, size_t k) { return ((mask[i] >> k) & 1u) != 0; };
+            if (f == SPH_CONTACTS) {
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t k = 0; k < nc; ++k)
+                        du[i * nc + k] = occ(i, k) ? (slot[i * nc + k] & sph::SLOT_IDX_MASK) - (uint32_t)r.base
+                                                   : (uint32_t)n;
+                return SPH_OK;
+            }
+            if (f == SPH_DISTMOD) {
+                std::vector<double> x;
+                if ((rc = fetch4(h, own(h).x, id, x, r))) return rc;
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t k = 0; k < nc; ++k) {
+                        double* o4 = dd + 4 * (i * nc + k);
+                        if (!occ(i, k)) { o4[0] = o4[1] = o4[2] = o4[3] = 0.0; continue; }
+                        const uint32_t w = slot[i * nc + k], code = w >> 28;
+                        const size_t b = (w & sph::SLOT_IDX_MASK) - r.base;
+                        const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
+                        o4[0] = cx == 1u ? h->hs.L[0] : (cx == 2u ? -h->hs.L[0] : 0.0);
+                        o4[1] = cy == 1u ? h->hs.L[1] : (cy == 2u ? -h->hs.L[1] : 0.0);
+                        o4[2] = 0.0;
+                        o4[3] = x[4 * b + 3];
+                    }
+                return SPH_OK;
+            }
+            if (h->precision == SPH_PREC_MIXED && d.cdt32) {
+                std::vector<float> tf(n * nc * 3);
+                HIP_TRY(hipMemcpy(tf.data(), d.cdt32 + r.base * nc * 3, tf.size() * 4, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < n * nc; ++i)
+                    for (int a = 0; a < 3; ++a) dd[3 * i + a] = occ(i / nc, i % nc) ? (double)tf[3 * i + a] : 0.0;
+            } else {
+                HIP_TRY(hipMemcpy(dd, d.cdt + r.base * nc * 3, n * nc * 24, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < n * nc; ++i)
+                    if (!occ(i / nc, i % nc)) dd[3 * i] = dd[3 * i + 1] = dd[3 * i + 2] = 0.0;
+            }
+            return SPH_OK;
+        }
+        default: return 1;
+    }
 }
 
 }  // namespace
@@ -1266,28 +1370,17 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
     }
     if (need == 0) return SPH_OK;
     HIP_TRY(hipStreamSynchronize(h->stream));
+    int rc = get_particle_field(h, f, dst, whole(h));
+    if (rc <= 0) return rc;
     const DemDev& d = h->d;
-    const size_t n = h->hs.np, nc = sph::NC, K = d.K;
+    const size_t n = h->hs.np, K = d.K;
     double* dd = static_cast<double*>(dst);
     uint32_t* du = static_cast<uint32_t*>(dst);
     auto copy = [&](const void* src) -> int {
         HIP_TRY(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
         return SPH_OK;
     };
-    int rc;
-    std::vector<uint32_t> id;
-    std::vector<double> t;
-    // per-particle double4 record in storage order -> comps [c0, c0+nc_) in original order
-    auto gather = [&](const double4* src, int c0, int ncomp) -> int {
-        int r;
-        if ((r = fetch_ids(h, id)) || (r = fetch4(h, src, id, t))) return r;
-        for (size_t i = 0; i < n; ++i)
-            for (int a = 0; a < ncomp; ++a) dd[(size_t)ncomp * i + a] = t[4 * i + c0 + a];
-        return SPH_OK;
-    };
     switch (f) {
-        case SPH_X: return gather(own(h).x, 0, 4);
-        case SPH_VEL: return gather(own(h).vel, 0, 4);
         case SPH_X_SORTED: {
             // the last step's input positions (the other set) in its sorted
             // order, wrapped as the step wrapped them
@@ -1310,20 +1403,6 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
                 }
                 std::memcpy(dd + 4 * q, v, 32);
             }
-            return SPH_OK;
-        }
-        case SPH_ANGVEL: return gather(own(h).angvel, 0, 3);
-        case SPH_FORCE: return gather(d.force, 0, 3);
-        case SPH_TORQUE: return gather(d.torque, 0, 3);
-        case SPH_ANGPOS: case SPH_VEL0: case SPH_ANGVEL0: case SPH_XYSUM: case SPH_ES: case SPH_EV:
-            return get_comps(h, packed_comps(own(h), f), dd);
-        case SPH_ES_DOT: return gather(d.force, 3, 1);
-        case SPH_EV_DOT: return gather(d.torque, 3, 1);
-        case SPH_P: {
-            std::vector<double> pv(n);
-            if ((rc = fetch_ids(h, id))) return rc;
-            HIP_TRY(hipMemcpy(pv.data(), d.pres, n * 8, hipMemcpyDeviceToHost));
-            for (size_t q = 0; q < n; ++q) dd[id[q]] = pv[q];
             return SPH_OK;
         }
         case SPH_SORT_IDX: return copy(d.sid);
@@ -1353,47 +1432,6 @@ int sph_get(sph_t* h, int f, void* dst, size_t bytes) {
         case SPH_TIME: {
             const double tm[4] = {h->hs.dt, h->hs.t_cur, h->hs.t_total, h->hs.file_dt};
             std::memcpy(dst, tm, 32);
-            return SPH_OK;
-        }
-        case SPH_CONTACTS: case SPH_DISTMOD: case SPH_DELTA_T: {
-            std::vector<uint32_t> smask(n), mask(n), slot(n * nc);
-            if ((rc = fetch_ids(h, id))) return rc;
-            HIP_TRY(hipMemcpy(smask.data(), own(h).cmask, n * 4, hipMemcpyDeviceToHost));
-            for (size_t q = 0; q < n; ++q) mask[id[q]] = smask[q];
-            HIP_TRY(hipMemcpy(slot.data(), d.cslot, n * nc * 4, hipMemcpyDeviceToHost));
-            auto occ = [&](size_t i, size_t k) { return ((mask[i] >> k) & 1u) != 0; };
-            if (f == SPH_CONTACTS) {
-                for (size_t i = 0; i < n; ++i)
-                    for (size_t k = 0; k < nc; ++k)
-                        du[i * nc + k] = occ(i, k) ? (slot[i * nc + k] & sph::SLOT_IDX_MASK) : (uint32_t)n;
-                return SPH_OK;
-            }
-            if (f == SPH_DISTMOD) {
-                std::vector<double> x;
-                if ((rc = fetch4(h, own(h).x, id, x))) return rc;
-                for (size_t i = 0; i < n; ++i)
-                    for (size_t k = 0; k < nc; ++k) {
-                        double* o4 = dd + 4 * (i * nc + k);
-                        if (!occ(i, k)) { o4[0] = o4[1] = o4[2] = o4[3] = 0.0; continue; }
-                        const uint32_t w = slot[i * nc + k], code = w >> 28, b = w & sph::SLOT_IDX_MASK;
-                        const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
-                        o4[0] = cx == 1u ? h->hs.L[0] : (cx == 2u ? -h->hs.L[0] : 0.0);
-                        o4[1] = cy == 1u ? h->hs.L[1] : (cy == 2u ? -h->hs.L[1] : 0.0);
-                        o4[2] = 0.0;
-                        o4[3] = x[4 * (size_t)b + 3];
-                    }
-                return SPH_OK;
-            }
-            if (h->precision == SPH_PREC_MIXED && d.cdt32) {
-                std::vector<float> tf(n * nc * 3);
-                HIP_TRY(hipMemcpy(tf.data(), d.cdt32, tf.size() * 4, hipMemcpyDeviceToHost));
-                for (size_t i = 0; i < n * nc; ++i)
-                    for (int a = 0; a < 3; ++a) dd[3 * i + a] = occ(i / nc, i % nc) ? (double)tf[3 * i + a] : 0.0;
-            } else {
-                HIP_TRY(hipMemcpy(dd, d.cdt, n * nc * 24, hipMemcpyDeviceToHost));
-                for (size_t i = 0; i < n * nc; ++i)
-                    if (!occ(i / nc, i % nc)) dd[3 * i] = dd[3 * i + 1] = dd[3 * i + 2] = 0.0;
-            }
             return SPH_OK;
         }
         default: break;
@@ -1574,6 +1612,12 @@ int sph_set_profiling(sph_t* h, int on) {
 int sph_get_timing(const sph_t* h, sph_timing_t* out) {
     if (!h || !out) return SPH_E_INVALID;
     *out = h->timing;
+    out->list_builds = 0;
+    if (h->d.nstate) {
+        if (hipMemcpyAsync(&out->list_builds, h->d.nstate + 3, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            return SPH_E_HIP;
+    }
     return SPH_OK;
 }
 
@@ -1915,7 +1959,7 @@ int sph_ens_get(sph_ens_t* e, uint32_t k, int f, void* dst, size_t bytes) {
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
     const sph::Sys& S = h->sys_host[k];
-    const size_t n = S.np, pb = S.pbase, N = h->d.np, nc = sph::NC;
+    const size_t n = S.np, pb = S.pbase, N = h->d.np;
     char* out = static_cast<char*>(dst);
     switch (f) {
         case SPH_WALLS_NX: HIP_TRY(hipMemcpy(dst, h->d.wnx + S.wbase, need, hipMemcpyDeviceToHost)); return SPH_OK;
@@ -1935,17 +1979,15 @@ int sph_ens_get(sph_ens_t* e, uint32_t k, int f, void* dst, size_t bytes) {
         }
         default: break;
     }
-    // per-particle fields: the whole ensemble's array, then the member's rows
-    const size_t all = sph_field_bytes(h, f), row = all / N;
-    std::vector<char> tmp(all);
-    int rc = sph_get(h, f, tmp.data(), all);
-    if (rc) return rc;
-    std::memcpy(out, tmp.data() + pb * row, n * row);
-    if (f == SPH_CONTACTS) {  // partners in the member's numbering, empty = its np
-        uint32_t* c = reinterpret_cast<uint32_t*>(out);
-        for (size_t i = 0; i < n * nc; ++i) c[i] = c[i] == (uint32_t)N ? (uint32_t)n : c[i] - (uint32_t)pb;
+    // per-particle fields: the member's own storage range (its particles'
+    // original indices and sorted positions are [pbase, pbase + np)), its
+    // contact partners in its own numbering
+    const int rc = get_particle_field(h, f, out, Range{pb, n});
+    if (rc > 0) {
+        h->err = "sph_ens_get: field " + std::to_string(f) + " is not available per member";
+        return SPH_E_UNSUPPORTED;
     }
-    return SPH_OK;
+    return rc;
 }
 
 int sph_ens_member_info(const sph_ens_t* e, uint32_t k, sph_info_t* out) {

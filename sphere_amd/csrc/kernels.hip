@@ -210,7 +210,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
             atomicOr(d.flags, f);
             *d.flags_next = 0u;
         }
-        d.nstate[0] = d.nstate[1] | d.nstate[2];
+        const uint32_t build = d.nstate[1] | d.nstate[2];
+        d.nstate[0] = build;
+        if (build && d.search_mode == 5) d.nstate[3] += 1u;  // list builds so far
         d.nstate[1] = 0u;
         d.nstate[2] = 0u;
     }

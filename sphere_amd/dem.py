@@ -107,7 +107,8 @@ class SphTiming(C.Structure):
                 ("reorder_ms", C.c_double), ("contact_ms", C.c_double), ("walls_ms", C.c_double),
                 ("total_ms", C.c_double), ("steps", C.c_uint64), ("search_ms", C.c_double),
                 ("force_ms", C.c_double), ("tile_x", C.c_uint32), ("tile_y", C.c_uint32),
-                ("search_mode", C.c_int32), ("search_auto", C.c_int32), ("force_variant", C.c_int32)]
+                ("search_mode", C.c_int32), ("search_auto", C.c_int32), ("force_variant", C.c_int32),
+                ("list_builds", C.c_uint32)]
 
 
 class SphEnsInfo(C.Structure):

@@ -40,10 +40,10 @@ def test_shear_weights_select_driven_layer():
     assert np.all(w[~driven] == 0.0) and np.all(w[:, 1:] == 0.0)
 
 
-def _fd(sb, w, nsteps, precision="fp64", rel=1e-6):
+def _fd(sb, w, nsteps, precision="fp64", rel=1e-6, theta0=THETA0):
     from sphere_amd.adjoint import objective_value
 
-    th0 = np.array(THETA0)
+    th0 = np.array(theta0)
     fd = np.zeros(4)
     for k in range(4):
         h = rel * abs(th0[k])
@@ -184,6 +184,61 @@ def test_adjoint_at_full_size_matches_central_differences():
     g = dem.adjoint_backward()
     dem.close()
     fd = _fd(sb, w, 10)
+    err = _relerr(g, fd)
+    _say(f"J={J:.6e} adjoint={g} fd={fd} relative error={err}")
+    assert np.all(err <= 1e-4), f"relative error {err}"
+
+
+# The differentiable shear test as specified: the 65,536-sphere polydisperse
+# cell (config-1 lognormal radii) compacted under its 10 kPa wall, then
+# sheared at 0.1 m/s; checkpoints every 100 steps.
+C3_THETA = (1e6, 1e6, 0.5, 0.5)
+C3_SEG = 100
+C3_FD_STEPS = 2 * C3_SEG  # two full checkpoint segments
+
+
+@pytest.fixture(scope="module")
+def c3_cell():
+    _say("consolidating the 65,536-sphere polydisperse shear cell")
+    return configs.consolidated_shear_cell(gamma_n=C3_THETA[2], mu=C3_THETA[3], log=_say)
+
+
+@pytest.mark.gpu
+def test_consolidated_shear_cell_is_dense(c3_cell):
+    """The recipe's packing: every sphere inside the box, the driven and the
+    held layers in place, about four or more contacts per sphere."""
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    sb = c3_cell
+    assert sb.np == 65536 and sb.contactmodel == 2
+    assert float(np.ptp(sb.radius)) > 0.5 * float(np.median(sb.radius))  # polydisperse
+    w = shear_stress_weights(sb)
+    assert int((w[:, 0] != 0.0).sum()) > 100
+    dem = DEM(sb, verbosity=0)
+    ij, _ = dem.findOverlaps()
+    dem.close()
+    cn = 2.0 * len(ij) / sb.np
+    _say(f"C3 packing: C/N = {cn:.2f}, driven {int((w[:, 0] != 0).sum())}")
+    assert cn >= 4.0
+
+
+@pytest.mark.gpu
+def test_adjoint_on_the_consolidated_shear_cell(c3_cell):
+    """FP64 adjoint of the mean shear stress on the compacted polydisperse
+    cell with checkpoints every 100 steps, over two full segments, against
+    central differences with relative step 1e-6 (bar 1e-4 per component)."""
+    from sphere_amd.adjoint import shear_stress_weights
+    from sphere_amd.dem import DEM
+
+    sb = c3_cell
+    w = shear_stress_weights(sb)
+    _say(f"fp64 adjoint vs central differences, consolidated cell, {C3_FD_STEPS} steps, segments of {C3_SEG}")
+    dem = DEM(sb, verbosity=0)
+    J = dem.adjoint_forward(C3_FD_STEPS, w, seg_len=C3_SEG)
+    g = dem.adjoint_backward()
+    dem.close()
+    fd = _fd(sb, w, C3_FD_STEPS, theta0=C3_THETA)
     err = _relerr(g, fd)
     _say(f"J={J:.6e} adjoint={g} fd={fd} relative error={err}")
     assert np.all(err <= 1e-4), f"relative error {err}"

@@ -245,6 +245,7 @@ def test_calibration_ensemble_members_match_single_handles():
 
 
 @pytest.mark.gpu
+@pytest.mark.timeout(900)
 def test_calibration_ensemble_full_size(tmp_path):
     """The calibration ensemble at its stated size: 512 members (uneven
     counts in [1,024, 32,768], about 8.5M spheres), autotuned grid, 1,000
@@ -268,10 +269,14 @@ def test_calibration_ensemble_full_size(tmp_path):
     for k, m in enumerate(ms):
         d = DEM(regridded(m.copy(), factor), verbosity=0)
         d.set_precision("mixed")
+        # every kernel variant gives the same bits: the ensemble's choice,
+        # fixed, spares each single handle its timing trials
+        d.set_search(5)
+        d.set_force_kernel(1)
         d.step(steps)
         errs += _compare(ens, k, d, _fields(m))
         d.close()
-        if k % 128 == 127:
+        if k % 16 == 15:
             _say(f"  {k + 1} members compared")
     ens.close()
     assert not errs, errs[:8]

@@ -408,6 +408,81 @@ def test_full_size_steps_match_oracle(tmp_path, workload):
     assert not errs, errs
 
 
+def _builds(dem):
+    return int(dem.timing()["list_builds"])
+
+
+@pytest.mark.parametrize("workload", ["c1-compression", "dense-crystal"])
+def test_list_search_across_a_rebuild_full_size(tmp_path, workload):
+    """Neighbour lists (search mode 5) forced on the 1,048,576-sphere inputs,
+    bit-identical to the oracle's full search across a list rebuild.  The
+    random compression packing falls under gravity until a sphere has moved
+    far enough to request a rebuild: a first handle finds the step of the
+    second build, a second one runs to three steps before it, hands its state
+    to the oracle, and both go on through the rebuild.  The crystal barely
+    moves, so there the rebuild is requested midway (sph_set_search(5))."""
+    from oracle_lib import Oracle
+    from parity import all_fields, compare_all, hand_over
+    from sphere_amd.dem import DEM
+
+    _say(f"list search across a rebuild, {workload}")
+    if workload == "dense-crystal":
+        sb = configs.dense_lattice()
+        dem = DEM(sb, verbosity=0)
+        dem.set_search(5)
+        dem.set_force_kernel(1)
+        orc = Oracle.from_spherebin(sb, tmp_path)
+        for k, rebuild in ((2, False), (1, True), (2, False)):
+            if rebuild:
+                dem.set_search(5)
+            dem.step(k)
+            assert orc.step(k) == 0
+            errs = compare_all(dem, orc, all_fields(orc))
+            assert not errs, errs
+        assert _builds(dem) == 2
+        dem.close()
+        orc.close()
+        return
+
+    def lists_only(d):  # lists and one force kernel: no timing trials
+        d.set_search(5)
+        d.set_force_kernel(1)
+
+    sb = configs.random_compression()
+    probe = DEM(sb, verbosity=0)
+    lists_only(probe)
+    done = 0
+    while _builds(probe) < 2:
+        probe.step(200)
+        done += 200
+        assert done <= 20000, "no list rebuild within 20,000 steps"
+    probe.close()
+    probe = DEM(sb, verbosity=0)
+    lists_only(probe)
+    probe.step(done - 200)
+    at = done - 200
+    while _builds(probe) < 2:
+        probe.step(1)
+        at += 1
+    probe.close()
+    _say(f"second list build during step {at}")
+    assert at > 3
+    dem = DEM(sb, verbosity=0)
+    lists_only(dem)
+    dem.step(at - 3)
+    assert _builds(dem) == 1
+    orc = Oracle.from_spherebin(sb, tmp_path)
+    hand_over(dem, orc)
+    for k in (2, 2, 2):
+        dem.step(k)
+        assert orc.step(k) == 0
+        errs = compare_all(dem, orc, all_fields(orc))
+        assert not errs, errs
+    assert _builds(dem) == 2
+    dem.close()
+    orc.close()
+
+
 def test_mixed_precision_stress_error_at_full_size():
     """Mixed precision (FP32 pair forces) on the 1,048,576-sphere dense
     crystal, 300 steps: the normal stress on the top wall (its per-particle
