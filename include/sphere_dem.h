@@ -264,6 +264,19 @@ int sph_pack_rsa(uint64_t n, const double* radii, const double* box, int periodi
 #define SPH_POROSITY_EXACT 1
 int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, double* porosity);
 
+/* No counterpart in the reference: sph_porosity for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * sph_porosity, bit for bit, but the call allocates nothing: its device work
+ * space is one static array per GPU of SPH_POROSITY_NOALLOC_MAX doubles
+ * (8 MiB, held from the library's load until the process ends), which must
+ * fit 1793 + slices * (ceil(np / 2048) + 2) doubles -- e.g. up to 2,036
+ * slices at 1,048,576 spheres; larger requests return SPH_E_INVALID.  Every
+ * call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (from several threads or handles) race. */
+#define SPH_POROSITY_NOALLOC_MAX 1048576
+int sph_porosity_noalloc(sph_t* h, int slices, int rule, double* z_pos, double* porosity);
+
 /* DEM::findOverlaps (src/sphere.cpp:553-604, used by the `forcechains`
  * tool): every pair of particles i < j (original indices) whose centres are
  * closer than r_i + r_j, periodic images not considered, ordered by i then

@@ -20,6 +20,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -89,18 +90,20 @@ __global__ __launch_bounds__(ABLOCK) void k_bounds(const double4* __restrict__ x
     if (threadIdx.x < 7) part[7 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// bounds[7] and the last particle's radius, on the host
-int bounds(sph_dem* h, const sph::PState& st, uint32_t n, Scratch& s, double out[7], double* r_last) {
-    const uint32_t nblk = std::max<uint32_t>(1, std::min<uint32_t>(1024, (n + ABLOCK - 1) / ABLOCK));
-    double *part, *rl;
-    A_TRY(s.get(&part, 7 * (size_t)nblk));
-    A_TRY(s.get(&rl, 1));
+// bounds[7] and the last particle's radius, on the host, with the device
+// work space given: part holds 7 * BOUNDS_BLOCKS doubles, rl one.
+constexpr uint32_t BOUNDS_BLOCKS = 256;
+constexpr size_t BOUNDS_WS = 7 * (size_t)BOUNDS_BLOCKS + 1;  // doubles
+
+int bounds_in(sph_dem* h, const sph::PState& st, uint32_t n, double* part, double* rl, double out[7],
+              double* r_last) {
+    const uint32_t nblk = std::max<uint32_t>(1, std::min<uint32_t>(BOUNDS_BLOCKS, (n + ABLOCK - 1) / ABLOCK));
     const double zero = 0.0;
     A_TRY(hipMemcpyAsync(rl, &zero, 8, hipMemcpyHostToDevice, h->stream));
     k_bounds<<<nblk, ABLOCK, 0, h->stream>>>(st.x, st.id, n, part, rl);
     A_TRY(hipGetLastError());
-    std::vector<double> hp(7 * (size_t)nblk);
-    A_TRY(hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    double hp[7 * BOUNDS_BLOCKS];
+    A_TRY(hipMemcpyAsync(hp, part, 7 * (size_t)nblk * 8, hipMemcpyDeviceToHost, h->stream));
     A_TRY(hipMemcpyAsync(r_last, rl, 8, hipMemcpyDeviceToHost, h->stream));
     A_TRY(hipStreamSynchronize(h->stream));
     for (int c = 0; c < 7; ++c) out[c] = hp[c];
@@ -110,6 +113,12 @@ int bounds(sph_dem* h, const sph::PState& st, uint32_t n, Scratch& s, double out
         out[6] = (std::isnan(out[6]) || std::isnan(hp[7 * b + 6])) ? NAN : std::max(out[6], hp[7 * b + 6]);
     }
     return SPH_OK;
+}
+
+int bounds(sph_dem* h, const sph::PState& st, uint32_t n, Scratch& s, double out[7], double* r_last) {
+    double* ws;
+    A_TRY(s.get(&ws, BOUNDS_WS));
+    return bounds_in(h, st, n, ws, ws + 7 * (size_t)BOUNDS_BLOCKS, out, r_last);
 }
 
 // ---------------------------------------------------------------- porosity
@@ -716,19 +725,46 @@ int check_handle(sph_dem* h, const char* fn) {
 
 }  // namespace
 
-extern "C" int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, double* porosity) {
+namespace {
+
+// DEM::porosity with its device work space: `ws` (ws_cap doubles) or, when
+// ws is null, a buffer allocated for the call.  Layout: the bounds partials
+// (BOUNDS_WS), the per-slice block sums (slices x particle blocks), z_pos and
+// porosity (slices each).
+int porosity_run(sph_dem* h, int slices, int rule, double* z_pos, double* porosity, double* ws, size_t ws_cap,
+                 const char* fn) {
     int rc;
-    if ((rc = check_handle(h, "sph_porosity"))) return rc;
+    if ((rc = check_handle(h, fn))) return rc;
     if (slices < 1 || !z_pos || !porosity || (rule != SPH_POROSITY_REFERENCE && rule != SPH_POROSITY_EXACT)) {
-        h->err = "sph_porosity: need slices >= 1, output arrays and rule 0 or 1";
+        h->err = std::string(fn) + ": need slices >= 1, output arrays and rule 0 or 1";
         return SPH_E_INVALID;
     }
     const sph::HostState& hs = h->hs;
     const sph::PState& st = cur(h);
     const uint32_t n = h->d.np;
+    const uint32_t nblk = (n + ABLOCK * PPT - 1) / (ABLOCK * PPT);
+    const size_t parts = (size_t)slices * std::max<uint32_t>(nblk, 1);
+    if (parts > ((size_t)1 << 28)) {
+        h->err = std::string(fn) + ": slices x particle blocks above 2^28";
+        return SPH_E_INVALID;
+    }
+    const size_t need = BOUNDS_WS + parts + 2 * (size_t)slices;
     Scratch s;
+    if (!ws) {
+        A_TRY(s.get(&ws, need));
+    } else if (need > ws_cap) {
+        h->err = std::string(fn) + ": " + std::to_string(slices) + " slices of " + std::to_string(n) +
+                 " spheres need " + std::to_string(need) + " doubles of work space, above its maximum of " +
+                 std::to_string(ws_cap);
+        return SPH_E_INVALID;
+    }
+    double* bpart = ws;
+    double* rl = bpart + 7 * (size_t)BOUNDS_BLOCKS;
+    double* part = ws + BOUNDS_WS;
+    double* dz = part + parts;
+    double* dp = dz + slices;
     double bnd[7], r_last = 0.0;
-    if (n && (rc = bounds(h, st, n, s, bnd, &r_last))) return rc;
+    if (n && (rc = bounds_in(h, st, n, bpart, rl, bnd, &r_last))) return rc;
     double top = hs.L[2];
     if (hs.nw > 0) {  // wall 0's current position (src/sphere.cpp:367-371)
         double4 w0;
@@ -742,20 +778,10 @@ extern "C" int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, doubl
         return SPH_E_INVALID;
     }
     if (!(h_slice > 0.0) || !std::isfinite(h_slice)) {
-        h->err = "sph_porosity: the slices have no positive thickness";
+        h->err = std::string(fn) + ": the slices have no positive thickness";
         return SPH_E_INVALID;
     }
     const double V_slice = h_slice * hs.L[0] * hs.L[1];
-    const uint32_t nblk = (n + ABLOCK * PPT - 1) / (ABLOCK * PPT);
-    const size_t parts = (size_t)slices * std::max<uint32_t>(nblk, 1);
-    if (parts > ((size_t)1 << 28)) {
-        h->err = "sph_porosity: slices x particle blocks above 2^28";
-        return SPH_E_INVALID;
-    }
-    double *part, *dz, *dp;
-    A_TRY(s.get(&part, parts));
-    A_TRY(s.get(&dz, (size_t)slices));
-    A_TRY(s.get(&dp, (size_t)slices));
     k_zero_f64<<<fill_blocks(parts), ABLOCK, 0, h->stream>>>(part, parts);
     if (nblk) k_porosity<<<nblk, ABLOCK, 0, h->stream>>>(st.x, n, slices, h_slice, rule, part);
     k_porosity_final<<<(uint32_t)slices, 64, 0, h->stream>>>(part, nblk, slices, h_slice, V_slice, dz, dp);
@@ -764,6 +790,57 @@ extern "C" int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, doubl
     A_TRY(hipMemcpyAsync(porosity, dp, (size_t)slices * 8, hipMemcpyDeviceToHost, h->stream));
     A_TRY(hipStreamSynchronize(h->stream));
     return SPH_OK;
+}
+
+// Work space of sph_porosity_noalloc: one array per device, held from the
+// library's load until the process ends.  Host code never passes `scratch`
+// itself to a kernel or a copy (on the host the name stands for a host-side
+// shadow, not device memory): it uses the device address that
+// hipGetSymbolAddress gives for the current device, looked up once per
+// device at the entry point's first call there.
+constexpr size_t SCRATCH_DOUBLES = SPH_POROSITY_NOALLOC_MAX;  // 8 MiB
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[MAX_DEVICES] = {};
+
+int scratch_of(sph_dem* h, double** out) {
+    const int dev = h->device;
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        h->err = "sph_porosity_noalloc: device index out of range";
+        return SPH_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        int cur_dev = -1;
+        A_TRY(hipGetDevice(&cur_dev));
+        if (cur_dev != dev) A_TRY(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur_dev != dev) A_TRY(hipSetDevice(cur_dev));
+        if (e != hipSuccess || !p) {
+            h->err = std::string("sph_porosity_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return SPH_E_HIP;
+        }
+        g_scratch_addr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_addr[dev];
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" int sph_porosity(sph_t* h, int slices, int rule, double* z_pos, double* porosity) {
+    return porosity_run(h, slices, rule, z_pos, porosity, nullptr, 0, "sph_porosity");
+}
+
+extern "C" int sph_porosity_noalloc(sph_t* h, int slices, int rule, double* z_pos, double* porosity) {
+    if (!h) return SPH_E_INVALID;
+    double* ws = nullptr;
+    int rc;
+    if ((rc = scratch_of(h, &ws))) return rc;
+    return porosity_run(h, slices, rule, z_pos, porosity, ws, SCRATCH_DOUBLES, "sph_porosity_noalloc");
 }
 
 extern "C" int sph_find_overlaps(sph_t* h, uint32_t* ij, double* delta_n, uint64_t capacity, uint64_t* count) {

@@ -149,6 +149,7 @@ EXPORTS = {
     "sph_adjoint_forward": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
     "sph_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "sph_porosity": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "sph_porosity_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "sph_find_overlaps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sph_render": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_int,
                              C.c_void_p]),
