@@ -172,14 +172,18 @@ def mixed_error(ref, mix, area, r_min):
 
 
 def adjoint_run(tmp, nsteps, seg=100):
-    """Mean-shear-stress gradient on the 65,536-sphere dense shear cell:
-    plain forward time against the adjoint's forward (with checkpoints every
-    `seg` steps) and backward (recompute with tape + reverse sweep)."""
+    """Mean-shear-stress gradient on the differentiable shear test: 65,536
+    polydisperse spheres (config-1 radii) compacted under the 10 kPa wall,
+    then sheared at 0.1 m/s (configs.consolidated_shear_cell): plain forward
+    time against the adjoint's forward (with checkpoints every `seg` steps)
+    and backward (recompute with tape + reverse sweep)."""
     from sphere_amd import configs
     from sphere_amd.adjoint import shear_stress_weights
     from sphere_amd.dem import DEM
 
-    sb = configs.fcc_shear(gamma_n=0.5)
+    t0 = time.perf_counter()
+    sb = configs.consolidated_shear_cell()
+    t_recipe = time.perf_counter() - t0
     path = os.path.join(tmp, "shear.bin")
     sb.writebin(path)
     w = shear_stress_weights(sb)
@@ -198,10 +202,11 @@ def adjoint_run(tmp, nsteps, seg=100):
     g = dem.adjoint_backward()
     t2 = time.perf_counter()
     dem.close()
-    log(f"adjoint: plain {t_plain * 1e3 / nsteps:.4f} ms/step, forward+checkpoints {(t1 - t0) * 1e3 / nsteps:.4f}, "
-        f"backward {(t2 - t1) * 1e3 / nsteps:.4f}; J={J:.6e} grad={g}")
-    return {"workload": f"{sb.np:,}-sphere dense FCC shear cell, top layer at 0.1 m/s, 10 kPa wall, {nsteps} steps, "
-                        f"checkpoints every {seg}", "objective": "mean shear stress (Pa)", "J": J,
+    log(f"adjoint: packing {t_recipe:.1f} s; plain {t_plain * 1e3 / nsteps:.4f} ms/step, forward+checkpoints "
+        f"{(t1 - t0) * 1e3 / nsteps:.4f}, backward {(t2 - t1) * 1e3 / nsteps:.4f}; J={J:.6e} grad={g}")
+    return {"workload": f"{sb.np:,}-sphere polydisperse shear cell (lognormal radii), compacted under 10 kPa, "
+                        f"top layer at 0.1 m/s, {nsteps} steps, checkpoints every {seg}",
+            "driven_spheres": int((w[:, 0] != 0.0).sum()), "objective": "mean shear stress (Pa)", "J": J,
             "grad": dict(zip(("k_n", "k_t", "gamma_n", "mu"), [float(v) for v in g])),
             "plain_ms_per_step": t_plain * 1e3 / nsteps, "forward_ms_per_step": (t1 - t0) * 1e3 / nsteps,
             "backward_ms_per_step": (t2 - t1) * 1e3 / nsteps,
@@ -390,7 +395,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=13)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-dense", action="store_true")
@@ -505,9 +510,9 @@ def main():
         dsk = SEARCH_PROFILE.get(d64["search_mode"])
         dense["fp64"]["roofline"] = {
             "force": roofline_entry(FORCE_PROFILE[d64["force_variant"]], kd["force"], d64["stage_ms"]["force"],
-                                    measured_traffic(FORCE_PROFILE[d64["force_variant"]], "dense_1M_pmc_traffic.md")),
+                                    measured_traffic(FORCE_PROFILE[d64["force_variant"]], "dense_1M_fp64_pmc_traffic.md")),
             "search": roofline_entry(dsk, kd["search"], d64["stage_ms"]["search"],
-                                     measured_traffic(dsk, "dense_1M_pmc_traffic.md")),
+                                     measured_traffic(dsk, "dense_1M_fp64_pmc_traffic.md")),
         }
         log(f"dense fp64: {d64['ms']:.4f} ms/step, C/N={d64['contacts'] / dn:.2f}")
         if not args.no_mixed:

@@ -256,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(DemDev d) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scatter(DemDev d) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= d.np) return;
     const uint32_t q = d.off[d.key[i]] + d.rank[i];
     if (q < d.np) d.perm[q] = i;  // (counts and keys agree by construction; never write past perm)
@@ -286,7 +286,7 @@ __device__ __forceinline__ void sort_small(const DemDev& d, uint32_t* p, uint32_
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cellsort(DemDev d) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= d.K) return;
     const uint32_t st = d.off[c];
     const uint32_t n = d.off[c + 1] - st;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(BLOCK) void k_reorder(DemDev d) {
 // bond_f/bond_t; the contact kernel adds them per particle in ascending bond
 // index (the reference adds them with racing += at :220-226).
 __global__ __launch_bounds__(BLOCK) void k_bonds(DemDev d) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = xcd_block() * blockDim.x + threadIdx.x;
     if (k >= d.nb0) return;
     const Sys& S = d.sys;
     const uint2 bond = d.bonds[k];
@@ -1220,7 +1220,7 @@ __device__ __forceinline__ void apply_hits(const DemDev& d, uint32_t s, uint64_t
 
 __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
-    const uint32_t s = blockIdx.x * APPLY_BLOCK + threadIdx.x;
+    const uint32_t s = xcd_block() * APPLY_BLOCK + threadIdx.x;
     if (s >= d.np) return;
     const uint32_t cnt = d.hcnt[s];
     const uint32_t mask0 = d.ro.cmask[d.perm[s]];
@@ -1291,7 +1291,7 @@ template <bool B>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
     if (d.nstate[0] != 0u) return;  // k_topo_rows<true> rebuilds the lists this step
-    const uint32_t s = blockIdx.x * APPLY_BLOCK + threadIdx.x;
+    const uint32_t s = xcd_block() * APPLY_BLOCK + threadIdx.x;
     if (s >= d.np) return;
     const Sys& S = sys_at<B>(d, s);
     uint32_t* sw = sw_all + threadIdx.x;
