@@ -268,8 +268,8 @@ def shear_cell(n=65536, box=None, stress=10e3, shear_velocity=0.1, gravity=-9.81
     return sb
 
 
-def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.5, gamma_t=10.0, mu=0.5,
-                            steps=2000, seed=BASE_SEED + 3, sid="c3-shear", fall_steps=40000, settle_steps=100000,
+def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=2.0, gamma_t=10.0, mu=0.5,
+                            steps=2000, seed=BASE_SEED + 3, sid="c3-shear", fall_steps=40000, settle_steps=160000,
                             settle_gamma=10.0, wall_steps=20000, wall_mass_scale=1e-3, chunk=5000,
                             log=None) -> Spherebin:
     """The differentiable shear test's input: shear_cell's seeded lognormal
@@ -281,9 +281,11 @@ def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.
        feeds energy in on approach) and no friction (a frictionless packing
        jams with about six contacts per sphere), the top wall parked at the
        top of the grid: `fall_steps` steps of free fall onto the floor, then
-       every `chunk` steps up to `settle_steps` every sphere is stopped
-       (velocities and half-step velocities zeroed), so that the small
-       spheres the collapse throws up fall back and come to rest.  The
+       every `chunk` steps up to `settle_steps` every sphere keeps only its
+       downward velocity (horizontal and upward components of the velocity
+       and half-step velocity zeroed, spins stopped), so that the small
+       spheres the collapse throws up fall back and the bed comes to rest.
+       The
        dashpot is `settle_gamma` = 10 N s/m: the explicit update of a dashpot
        is stable only while dt * (sum of the dashpots on a sphere) / m stays
        below 2, and with the smallest spheres (m = 1.4e-3 kg) touching a
@@ -293,10 +295,11 @@ def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.
        for `wall_steps` steps, its mass the reference's default times
        `wall_mass_scale` (0.001, as the reference's own shear example scales
        it, python/shear-test.py:77), so that it reaches the packing early.
-    3. Contact model 2 with friction `mu` and a small normal dashpot
-       `gamma_n` (the adjoint's parameters), a tangential dashpot `gamma_t`
-       within the same stability limit (10 N s/m: the tangential force also
-       spins the sphere up, 3.5 times its effect on the sliding speed alone),
+    3. Contact model 2 with friction `mu`, a normal dashpot `gamma_n`
+       (2 N s/m: model 2's normal dashpot feeds energy in on approach, so it
+       stays small) and a tangential dashpot `gamma_t` within the same
+       stability limit (10 N s/m: the tangential force also spins the sphere
+       up, 3.5 times its effect on the sliding speed alone),
        frictionless undamped walls, and the shear set-up of shear_cell: the
        top layer driven at `shear_velocity` in x and the bottom layer held
        (fixvel, python/sphere.py:1297-1356).
@@ -319,12 +322,16 @@ def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.
     def fastest(d):
         return float(np.max(np.linalg.norm(d.get("vel")[:, :3], axis=1)))
 
-    def stop_all(d):
+    def calm(d):  # keep only downward motion: spheres thrown up fall back, the rest rests
         v = d.get("vel")
-        v[:, :3] = 0.0
+        v[:, :2] = 0.0
+        v[:, 2] = np.minimum(v[:, 2], 0.0)
         d.set("vel", v)
+        v0 = d.get("vel0")
+        v0[:, :2] = 0.0
+        v0[:, 2] = np.minimum(v0[:, 2], 0.0)
+        d.set("vel0", v0)
         z3 = np.zeros((sb.np, 3))
-        d.set("vel0", z3)
         d.set("angvel", z3)
         d.set("angvel0", z3)
 
@@ -349,10 +356,12 @@ def consolidated_shear_cell(n=65536, stress=10e3, shear_velocity=0.1, gamma_n=0.
             say(f"settling: {done} steps, fastest sphere {fastest(dem):.3g} m/s, "
                 f"highest top {float(np.max(x[:, 2] + x[:, 3])):.4f} m")
         if done >= fall_steps:
-            stop_all(dem)
+            calm(dem)
     sb = dem.to_spherebin()
     dem.close()
-    top = float(np.max(sb.x[:, 2] + sb.radius))
+    tops = sb.x[:, 2] + sb.radius
+    top = float(np.max(tops))
+    say(f"settled: highest top {top:.4f} m, 99.9th percentile {float(np.percentile(tops, 99.9)):.4f} m")
     sb.wmode = np.array([1], dtype=np.int32)
     sb.w_x[0] = top + 0.1 * float(np.median(r))
     sb.w_vel[0] = 0.0

@@ -192,7 +192,7 @@ def test_adjoint_at_full_size_matches_central_differences():
 # The differentiable shear test as specified: the 65,536-sphere polydisperse
 # cell (config-1 lognormal radii) compacted under its 10 kPa wall, then
 # sheared at 0.1 m/s; checkpoints every 100 steps.
-C3_THETA = (1e6, 1e6, 0.5, 0.5)
+C3_THETA = (1e6, 1e6, 2.0, 0.5)
 C3_SEG = 100
 C3_FD_STEPS = 2 * C3_SEG  # two full checkpoint segments
 
