@@ -210,9 +210,9 @@ extern "C" orc_t* orc_open(const char* path, int nc, char* err, size_t errlen) {
     std::fclose(f);
     if (!r.ok) { delete s; seterr(err, errlen, "truncated"); return nullptr; }
     if (s->nu > 0.0) { delete s; seterr(err, errlen, "fluid (nu>0) unsupported"); return nullptr; }
-    if (s->contactmodel != 1 && s->contactmodel != 2) {
+    if (s->contactmodel < 1 || s->contactmodel > 3) {
         delete s;
-        seterr(err, errlen, "only contactmodels 1 and 2");
+        seterr(err, errlen, "only contactmodels 1, 2 and 3");
         return nullptr;
     }
     // src/device.cu:413,423: vel0 <- vel, angvel0 <- angvel; walls vel0 <- mvfd.y
@@ -487,6 +487,68 @@ void contact_linear(orc_t* s, Acc& acc, const D4& vel_a, V3 angvel_a, const D4& 
     delta_t_io = delta_t;
 }
 
+// src/contactmodels.cuh:499-666 contactHertz, contact model 3, restated for a
+// positive overlap xi = -delta.  The reference passes the (negative) overlap
+// delta to powf(delta, 3/2), powf(delta, 1/4) and powf(delta, 1/2)
+// (src/contactsearch.cuh:517-529), which gives NaN forces; here each power
+// is taken of xi with the sign the linear law's terms carry: the elastic
+// normal term -k_n delta becomes k_n xi^(3/2), and the viscous and tangential
+// coefficients gamma_n xi^(1/4), k_t xi^(1/2), gamma_t xi^(1/4).  The powers
+// are evaluated as xi sqrt(xi), sqrt(sqrt(xi)) and sqrt(xi) (correctly
+// rounded on both sides).  The rolling resistance (:642-653) opposes the
+// relative spin with the smaller of gamma_r r_a |w_ab| and mu_r r_a |f_n|.
+void contact_hertz(orc_t* s, Acc& acc, const D4& vel_a, V3 angvel_a, const D4& vel_b4, V3 angvel_b,
+                   double radius_a, double radius_b, V3 x, double x_length, double delta, V3& delta_t_io) {
+    const double xi = -delta;
+    const double sq = std::sqrt(xi);
+    const double xi32 = ORC_MUTANT == 10 ? xi : xi * sq;
+    const double xi14 = std::sqrt(sq), xi12 = ORC_MUTANT == 14 ? 1.0 : sq;
+    const V3 delta_t0_uncor = delta_t_io;
+    const V3 n = divs(x, x_length);
+    const V3 vel_linear = v3(vel_a.x - vel_b4.x, vel_a.y - vel_b4.y, vel_a.z - vel_b4.z);
+    const V3 vel = add3(add3(vel_linear, smul(radius_a + delta / 2.0, cross3(n, angvel_a))),
+                        smul(radius_b + delta / 2.0, cross3(n, angvel_b)));
+    const V3 angvel_ab = sub3(angvel_a, angvel_b);
+    const double angvel_ab_length = len3(angvel_ab);
+    const double vel_n = dot3(vel_linear, n);
+    const V3 vel_t = sub3(vel, muls(n, dot3(vel, n)));
+    const double vel_t_length = len3(vel_t);
+    const V3 delta_t0 = sub3(delta_t0_uncor, muls(n, dot3(delta_t0_uncor, n)));
+    const double delta_t0_length = len3(delta_t0);
+    V3 delta_t = delta_t0;  // the reference leaves it unset when neither branch runs
+    V3 f_n = smul(s->k_n * xi32 - s->gamma_n * (ORC_MUTANT == 13 ? 1.0 : xi14) * vel_n, n);
+    acc.ev_dot += s->gamma_n * vel_n * vel_n;
+    if (dot3(f_n, n) < 0.0) f_n = v3(0.0, 0.0, 0.0);
+    const double f_n_length = len3(f_n);
+    const V3 f_c = smul(-s->kappa * std::sqrt(radius_a * radius_b), n);
+    V3 f_t = v3(0.0, 0.0, 0.0), t_res = v3(0.0, 0.0, 0.0);
+    if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
+        const V3 f_t_elast = smul(-s->k_t * xi12, delta_t0);
+        const V3 f_t_visc = smul(-s->gamma_t * xi14, vel_t);
+        const double f_t_limit = vel_t_length > (double)0.001f ? s->mu_d * len3(sub3(f_n, f_c))
+                                                               : s->mu_s * len3(sub3(f_n, f_c));
+        f_t = add3(f_t_elast, f_t_visc);
+        const double f_t_length = len3(f_t);
+        if (f_t_length > f_t_limit) {
+            f_t = muls(f_t, f_t_limit / f_t_length);
+            delta_t = divs(add3(f_t, smul(s->gamma_t * xi14, vel_t)), -s->k_t * xi12);
+            acc.es_dot += len3(sub3(delta_t0, delta_t)) * s->k_t / s->dt;
+        } else {
+            delta_t = add3(delta_t0, muls(vel_t, s->dt));
+        }
+    }
+    if (angvel_ab_length > 0.0) {
+        const double cap = ORC_MUTANT == 12 ? s->gamma_r * radius_a * angvel_ab_length
+                                            : std::fmin(s->gamma_r * radius_a * angvel_ab_length,
+                                                        s->mu_r * radius_a * f_n_length);
+        t_res = divs(smul((ORC_MUTANT == 11 ? 1.0 : -1.0) * cap, angvel_ab), angvel_ab_length);
+    }
+    acc.F = add3(acc.F, add3(add3(f_n, f_t), f_c));
+    acc.T = add3(acc.T, add3(smul(-(radius_a + delta / 2.0), cross3(n, f_t)), t_res));
+    acc.p += f_n_length / (4.0 * PI * radius_a * radius_a);
+    delta_t_io = delta_t;
+}
+
 // src/contactmodels.cuh:10-109 contactLinear_wall.
 double contact_linear_wall(orc_t* s, Acc& acc, const D4& linvel4, V3 angvel, double radius_a,
                            V3 n, double delta, double wvel) {
@@ -635,7 +697,7 @@ static void stage_interact(orc_t* s) {
         const D4 vel_a = s->vel_sorted[a];
         const V3 angvel_a = s->angvel_sorted[a];
         if (s->contactmodel == 1) interact_viscous(s, a, acc);
-        for (uint32_t i = 0; i < NC && s->contactmodel == 2; ++i) {
+        for (uint32_t i = 0; i < NC && s->contactmodel != 1; ++i) {
             const size_t mp = (size_t)ao * NC + i;
             const uint32_t bo = s->contacts[mp];
             if (bo != n) {
@@ -645,7 +707,10 @@ static void stage_interact(orc_t* s) {
                 const V3 xab = v3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
                 const double len = len3(xab);
                 const double delta_n = len - (ra + rb);
-                if (delta_n < 0.0) {
+                if (delta_n < 0.0 && s->contactmodel == 3) {
+                    contact_hertz(s, acc, vel_a, angvel_a, s->vel[bo], s->angvel[bo], ra, rb, xab, len,
+                                  delta_n, s->delta_t[mp]);
+                } else if (delta_n < 0.0) {
                     contact_linear(s, acc, vel_a, angvel_a, s->vel[bo], s->angvel[bo], ra, rb, xab,
                                    len, delta_n, s->delta_t[mp]);
                 } else {

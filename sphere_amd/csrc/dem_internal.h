@@ -90,8 +90,10 @@ struct Sys {
     double g[3];
     double k_n, k_t, gamma_n, gamma_t, mu_s, mu_d;
     double gamma_wn, gamma_wt, mu_ws, mu_wd, rho, kappa;
+    double gamma_r, mu_r;   // rolling resistance (contact model 3)
     double devs_A, devs_f;
-    uint32_t contactmodel;  // 1: visco-frictional, found and processed at once; 2: linear with slots
+    uint32_t contactmodel;  // 1: visco-frictional, found and processed at once; 2: linear with slots;
+                            // 3: Hertz-Mindlin with rolling resistance, with slots
     double db, V_b;         // capillary bond reach and liquid volume (contact model 1)
 };
 

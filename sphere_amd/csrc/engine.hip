@@ -159,6 +159,7 @@ sph::Sys make_sys(const HostState& s) {
     S.k_n = s.k_n; S.k_t = s.k_t; S.gamma_n = s.gamma_n; S.gamma_t = s.gamma_t;
     S.mu_s = s.mu_s; S.mu_d = s.mu_d; S.gamma_wn = s.gamma_wn; S.gamma_wt = s.gamma_wt;
     S.mu_ws = s.mu_ws; S.mu_wd = s.mu_wd; S.rho = s.rho; S.kappa = s.kappa;
+    S.gamma_r = s.gamma_r; S.mu_r = s.mu_r;
     S.devs_A = s.devs_A; S.devs_f = s.devs_f;
     S.contactmodel = s.contactmodel;
     S.db = s.db; S.V_b = s.V_b;
@@ -255,14 +256,7 @@ void choose_tile(sph_dem* h, uint32_t want_x = 0, uint32_t want_y = 0) {
 int check_bond_pairs(sph_dem* h, const uint32_t* bonds, size_t nb, uint32_t np);
 
 int validate_system(sph_dem* h, const HostState& s) {
-    if (s.contactmodel == 3) {
-        // src/contactmodels.cuh:565-566,596-597 raise the (negative) overlap
-        // to the powers 3/2, 1/4 and 1/2: every contact force is NaN
-        h->err = "contactmodel 3: the reference's Hertz law raises the negative overlap to fractional powers "
-                 "(NaN forces); not provided";
-        return SPH_E_UNSUPPORTED;
-    }
-    if (s.contactmodel != 1 && s.contactmodel != 2) {
+    if (s.contactmodel < 1 || s.contactmodel > 3) {
         h->err = "contactmodel " + std::to_string(s.contactmodel) + " is not a model of the reference";
         return SPH_E_INVALID;
     }
@@ -1577,6 +1571,10 @@ int sph_set_params(sph_t* h, const sph_params_t* in) {
 int sph_set_precision(sph_t* h, int precision) {
     if (!h || (precision != SPH_PREC_FP64 && precision != SPH_PREC_MIXED)) return SPH_E_INVALID;
     if (precision == h->precision) return SPH_OK;
+    if (precision == SPH_PREC_MIXED && h->hs.contactmodel == 3) {
+        h->err = "sph_set_precision: contact model 3 runs in FP64 only";
+        return SPH_E_UNSUPPORTED;
+    }
     int rc;
     const size_t cnt = (size_t)h->hs.np * sph::NC * 3;
     HIP_TRY(hipStreamSynchronize(h->stream));

@@ -440,6 +440,63 @@ __device__ __forceinline__ void contact_linear(const Sys& S, Acc& acc, V3<T> vel
     delta_t_io = delta_t;
 }
 
+// src/contactmodels.cuh:499-666 contactHertz (contact model 3), FP64, for a
+// positive overlap xi = -delta.  The reference raises the negative overlap
+// itself to the powers 3/2, 1/4 and 1/2 (powf of a negative number: NaN
+// forces); each power is taken here of xi, with the sign the linear law's
+// terms carry (elastic normal term k_n xi^(3/2) repulsive, positive viscous
+// and tangential coefficients), as xi sqrt(xi), sqrt(sqrt(xi)), sqrt(xi).
+// The rolling resistance (:642-653) opposes the relative spin with the
+// smaller of gamma_r r_a |w_ab| and mu_r r_a |f_n|.
+__device__ __forceinline__ void contact_hertz(const Sys& S, Acc& acc, V3<double> vel_a, V3<double> angvel_a,
+                                              V3<double> vel_b, V3<double> angvel_b, double radius_a,
+                                              double radius_b, V3<double> x, double x_length, double delta,
+                                              V3<double>& delta_t_io) {
+    const double xi = -delta;
+    const double sq = sqrt(xi);
+    const double xi32 = xi * sq, xi14 = sqrt(sq), xi12 = sq;
+    const V3<double> delta_t0_uncor = delta_t_io;
+    const V3<double> n = x / x_length;
+    const V3<double> vel_linear = vel_a - vel_b;
+    const V3<double> vel = vel_linear + (radius_a + delta / 2.0) * cross(n, angvel_a)
+                           + (radius_b + delta / 2.0) * cross(n, angvel_b);
+    const V3<double> angvel_ab = angvel_a - angvel_b;
+    const double angvel_ab_length = vlen(angvel_ab);
+    const double vel_n = dot(vel_linear, n);
+    const V3<double> vel_t = vel - n * dot(vel, n);
+    const double vel_t_length = vlen(vel_t);
+    const V3<double> delta_t0 = delta_t0_uncor - n * dot(delta_t0_uncor, n);
+    const double delta_t0_length = vlen(delta_t0);
+    V3<double> delta_t = delta_t0;
+    V3<double> f_n = (S.k_n * xi32 - S.gamma_n * xi14 * vel_n) * n;
+    acc.ev_dot += S.gamma_n * vel_n * vel_n;
+    if (dot(f_n, n) < 0.0) f_n = mk(0.0, 0.0, 0.0);
+    const double f_n_length = vlen(f_n);
+    const V3<double> f_c = (-S.kappa * sqrt(radius_a * radius_b)) * n;
+    V3<double> f_t = mk(0.0, 0.0, 0.0), t_res = mk(0.0, 0.0, 0.0);
+    if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
+        const V3<double> f_t_elast = (-S.k_t * xi12) * delta_t0;
+        const V3<double> f_t_visc = (-S.gamma_t * xi14) * vel_t;
+        const double f_t_limit = vel_t_length > (double)0.001f ? S.mu_d * vlen(f_n - f_c) : S.mu_s * vlen(f_n - f_c);
+        f_t = f_t_elast + f_t_visc;
+        const double f_t_length = vlen(f_t);
+        if (f_t_length > f_t_limit) {
+            f_t = f_t * (f_t_limit / f_t_length);
+            delta_t = (f_t + (S.gamma_t * xi14) * vel_t) / (-S.k_t * xi12);
+            acc.es_dot += vlen(delta_t0 - delta_t) * S.k_t / S.dt;
+        } else {
+            delta_t = delta_t0 + vel_t * S.dt;
+        }
+    }
+    if (angvel_ab_length > 0.0)
+        t_res = ((-1.0 * fmin(S.gamma_r * radius_a * angvel_ab_length, S.mu_r * radius_a * f_n_length)) * angvel_ab) /
+                angvel_ab_length;
+    acc.F = acc.F + ((f_n + f_t) + f_c);
+    acc.T = acc.T + ((-(radius_a + delta / 2.0)) * cross(n, f_t) + t_res);
+    acc.p += f_n_length / (4.0 * PI_REF * radius_a * radius_a);
+    delta_t_io = delta_t;
+}
+
 // src/contactmodels.cuh:10-109 contactLinear_wall (FP64 in both modes).
 __device__ __forceinline__ double contact_wall(const Sys& S, Acc& acc, V3<double> linvel, V3<double> angvel,
                                                double radius_a, V3<double> n, double delta, double wvel) {
@@ -1597,9 +1654,11 @@ constexpr int FORCE_BATCH = 4;
 
 // VW: the partners' velocity records are gathered with their positions, ahead
 // of the slots' processing (more registers, fewer waves: faster with many
-// contacts per sphere, slower with few; the host picks by timing).
-template <class T, bool B, bool VW>
+// contacts per sphere, slower with few; the host picks by timing).  M3:
+// contact model 3's pair force (FP64, one system) instead of model 2's.
+template <class T, bool B, bool VW, bool M3 = false>
 __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
+    static_assert(!M3 || (sizeof(T) == 8 && !B), "contact model 3: FP64, one system");
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
     const Sys& S = sys_at<B>(d, s);
@@ -1667,8 +1726,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
                 V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
                 const typename PV::type vb = VW ? vbs[VW ? k : 0] : PV::vel(d, pb[k]);
                 const typename PV::type wb = VW ? wbs[VW ? k : 0] : PV::angvel(d, pb[k]);
-                contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), PV::v3(vb), PV::v3(wb), (T)ra,
-                                  (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
+                if constexpr (M3)
+                    contact_hertz(S, acc, va, wa, PV::v3(vb), PV::v3(wb), ra, xb.w, xab, len, delta_n, dt0);
+                else
+                    contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), PV::v3(vb), PV::v3(wb), (T)ra,
+                                      (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
                 SlotStore<T>::store(d, mp, dt0);
             } else {
                 mask &= ~(1u << i);  // contact ended: the slot is empty again
@@ -2013,6 +2075,7 @@ void launch_topology(const DemDev& d, hipStream_t s) {
 template <bool B, bool VW>
 void forces(const DemDev& d, bool mixed, hipStream_t s) {
     if (mixed) k_force<float, B, VW><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
+    else if (!B && d.sys.contactmodel == 3) k_force<double, false, VW, true><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
     else k_force<double, B, VW><<<nblk(d.np, d.tpb_force), d.tpb_force, 0, s>>>(d);
 }
 void launch_forces(const DemDev& d, int precision, hipStream_t s) {

@@ -385,15 +385,46 @@ def test_capillary_cohesion_parity(tmp_path):
     assert not errs, errs
 
 
-def test_contact_model_3_is_refused():
-    """The reference's Hertz law raises the negative overlap to fractional
-    powers (NaN forces); the handle refuses it with a message."""
+def _hertz(sb, gamma_r=2e-3, mu_r=0.05):
+    """Contact model 3 with its rolling resistance on; Hertz stiffness k_n =
+    k_t = 1e7 N/m^(3/2) (as stiff as the linear 1e6 N/m at 1e-2 m of overlap)."""
+    sb.contactmodel = 3
+    sb.k_n = sb.k_t = 1e7
+    sb.gamma_r, sb.mu_r = gamma_r, mu_r
+    sb.time_dt *= 0.5  # the walls keep the linear law, now with k_n = 1e7 N/m
+    return sb
+
+
+@pytest.mark.parametrize("periodic", [0, 1, 2])
+def test_contact_model_3_parity(tmp_path, periodic):
+    """Contact model 3 (contactHertz for a positive overlap, with rolling
+    resistance) on a settling lattice: every array bit for bit against the
+    oracle."""
+    _say(f"contact model 3 periodic={periodic}")
+    sb = _hertz(configs.lattice_settle(nside=(6, 6, 5), spacing=2.25e-2, jitter=5e-4, periodic=periodic,
+                                       gamma_n=5.0, gamma_t=10.0, sid=f"m3-{periodic}"))
+    errs = run_pair(sb, tmp_path, [1, 50, 300, 1200])
+    assert not errs, errs
+
+
+@pytest.mark.parametrize("search", [2, 4, 5])
+def test_contact_model_3_sheared_layer_parity(tmp_path, search):
+    """A sheared FCC block (top layer driven, spheres rolling against each
+    other) under contact model 3 with each search kernel."""
+    _say(f"contact model 3 shear search={search}")
+    sb = _hertz(configs.fcc_shear(cells=(4, 4, 8), gamma_n=20.0, sid=f"m3-shear-{search}"), gamma_r=5e-3)
+    errs = run_pair(sb, tmp_path, [1, 20, 40], setup=lambda d: d.set_search(search))
+    assert not errs, errs
+
+
+def test_contact_model_3_is_fp64_only():
+    """The mixed mode has no FP32 form of the Hertz law: refused."""
     from sphere_amd.dem import DEM, Unsupported
 
-    sb = configs.two_spheres(sid="hertz")
-    sb.contactmodel = 3
-    with pytest.raises(Unsupported, match="Hertz"):
-        DEM(sb, verbosity=0)
+    dem = DEM(_hertz(configs.two_spheres(sid="hertz")), verbosity=0)
+    with pytest.raises(Unsupported, match="FP64"):
+        dem.set_precision("mixed")
+    dem.close()
 
 
 @pytest.mark.parametrize("workload", ["c1-compression", "dense-crystal"])

@@ -1,9 +1,11 @@
 """The closed-form pins of the oracle's wall friction (contactLinear_wall,
-src/contactmodels.cuh:10-109) and integrate's fixvel branch
-(src/integration.cuh:65-77) catch plausible mistakes.  Each case builds the
+src/contactmodels.cuh:10-109), integrate's fixvel branch
+(src/integration.cuh:65-77) and contact model 3 (contactHertz,
+src/contactmodels.cuh:499-666) catch plausible mistakes.  Each case builds the
 oracle with one planted mistake (-DORC_MUTANT=k: a flipped sign, a swapped
-threshold, a dropped term), runs tests/test_oracle_physics.py against that
-build in a child process and requires the named pin to fail.  The mistakes
+threshold, a dropped term), runs the pins (tests/test_oracle_physics.py or
+tests/test_oracle_hertz.py) against that build in a child process and
+requires the named pin to fail.  The mistakes
 live only in those builds; the oracle's default build is the restatement."""
 
 import os
@@ -35,13 +37,27 @@ MUTANTS = {
 }
 
 
-@pytest.mark.parametrize("k", sorted(MUTANTS), ids=[MUTANTS[k][0] for k in sorted(MUTANTS)])
+# contact model 3 (contactHertz, src/contactmodels.cuh:499-666), pinned by
+# tests/test_oracle_hertz.py
+HERTZ_MUTANTS = {
+    10: ("Hertz normal force linear in the overlap", ["test_static_overlap_under_a_load",
+                                                      "test_elastic_collision_time_and_peak_overlap"]),
+    11: ("rolling resistance along the relative spin", ["test_rolling_resistance[0.0001-viscous]"]),
+    12: ("rolling resistance without its mu_r cap", ["test_rolling_resistance[10.0-capped]"]),
+    13: ("normal dashpot without the xi^(1/4) factor", ["test_damped_collision_against_the_ode"]),
+    14: ("tangential spring without the xi^(1/2) factor", ["test_tangential_stick_and_slip"]),
+}
+
+
+@pytest.mark.parametrize("k", sorted(MUTANTS) + sorted(HERTZ_MUTANTS),
+                         ids=[MUTANTS[k][0] for k in sorted(MUTANTS)] + [HERTZ_MUTANTS[k][0] for k in sorted(HERTZ_MUTANTS)])
 def test_pins_catch_planted_mistake(k):
-    what, pins = MUTANTS[k]
+    what, pins = MUTANTS[k] if k in MUTANTS else HERTZ_MUTANTS[k]
+    suite = "test_oracle_physics.py" if k in MUTANTS else "test_oracle_hertz.py"
     subprocess.run(["make", "-s", "-C", ORACLE, "mutant", f"M={k}"], check=True)
     env = dict(os.environ, SPH_ORACLE_MUTANT_LIB=os.path.join(ORACLE, "build", f"liboracle_dem_mut{k}.so"))
     res = subprocess.run([sys.executable, "-m", "pytest", "-q", "-rf", "-p", "no:cacheprovider", "-p", "no:xdist",
-                          os.path.join(HERE, "test_oracle_physics.py")],
+                          os.path.join(HERE, suite)],
                          capture_output=True, text=True, env=env, cwd=ROOT, timeout=600)
     failed = set(re.findall(r"FAILED \S+::(\S+)", res.stdout))
     assert res.returncode != 0, f"mutant {k} ({what}) passes every pin"
