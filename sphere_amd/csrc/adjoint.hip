@@ -578,18 +578,21 @@ __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
         t.tfresh[q] = d.tfresh[q];
         // the occupied slots only: the adjoint step reads no other (its own
         // and its partners' occupied slots of this step)
+        // (the input set keeps slot k's spring at 3 * (k * np + p), p the
+        // storage position; the tape keys it by original index and slot)
         const size_t o = (size_t)d.sid[q] * NC;
         for (uint32_t m = mask; m; m &= m - 1) {
-            const size_t i = o + (size_t)(__ffs(m) - 1);
+            const int k = __ffs(m) - 1;
+            const size_t i = o + (size_t)k, src = 3 * ((size_t)k * d.np + p);
             t.cslot[i] = d.cslot[i];
             if (t.cdt32) {
-                t.cdt32[3 * i] = d.cdt32[3 * i];
-                t.cdt32[3 * i + 1] = d.cdt32[3 * i + 1];
-                t.cdt32[3 * i + 2] = d.cdt32[3 * i + 2];
+                t.cdt32[3 * i] = d.ro.cdt32[src];
+                t.cdt32[3 * i + 1] = d.ro.cdt32[src + 1];
+                t.cdt32[3 * i + 2] = d.ro.cdt32[src + 2];
             } else {
-                t.cdt[3 * i] = d.cdt[3 * i];
-                t.cdt[3 * i + 1] = d.cdt[3 * i + 1];
-                t.cdt[3 * i + 2] = d.cdt[3 * i + 2];
+                t.cdt[3 * i] = d.ro.cdt[src];
+                t.cdt[3 * i + 1] = d.ro.cdt[src + 1];
+                t.cdt[3 * i + 2] = d.ro.cdt[src + 2];
             }
         }
     }

@@ -62,6 +62,13 @@ struct PState {
     uint32_t *cmask;  // occupied contact slots
     uint32_t *lidx;   // search mode 5: the particle's neighbour list (its sorted position at the build)
     float4 *vf, *wf;  // mixed mode: FP32 copies of vel and angvel (xyz) for the pair force's partner gathers
+    // tangential springs of the occupied contact slots, slot-major: slot k of
+    // the particle stored at q is (x, y, z) at 3 * (k * np + q).  A step reads
+    // them at the particle's storage position in its input set and writes
+    // them at its sorted position in the output set, so the lanes of a
+    // wavefront, working on consecutive particles, touch consecutive records.
+    double* cdt;   // FP64 mode
+    float* cdt32;  // mixed mode
 };
 
 // The records of one buffer set, as a step reads (ro) or writes (wo) them.
@@ -69,6 +76,8 @@ struct OwnSet {
     double4 *x, *vel, *angvel, *pr0, *pr1, *pr2;
     uint32_t *id, *cmask, *lidx;
     float4 *vf, *wf;
+    double* cdt;
+    float* cdt32;
 };
 
 // Box, cell grid, parameters and walls of one system.  A handle steps either
@@ -120,10 +129,8 @@ struct DemDev {
     double4 *torque;    // tx, ty, tz, ev_dot
     double* pres;       // p
 
-    // contact history, keyed by original index
+    // contact slots, keyed by original index (their springs live in the buffer sets)
     uint32_t* cslot;    // np * NC
-    double* cdt;        // np * NC * 3 (FP64 mode)
-    float* cdt32;       // np * NC * 3 (mixed mode)
 
     // cell list
     uint32_t *key, *rank, *perm, *count, *off, *scan_bsum;  // off: K+1 prefix

@@ -65,7 +65,7 @@ void destroy_graph(sph_dem* h) {
 }
 
 sph::OwnSet own_set(const sph::PState& p) {
-    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx, p.vf, p.wf};
+    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx, p.vf, p.wf, p.cdt, p.cdt32};
 }
 
 // the buffer set holding the current private records
@@ -412,7 +412,10 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemcpy(d.torque, t4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.pres, s.p.data(), n * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.cslot, 0, n * sph::NC * 4));
-    HIP_TRY(hipMemset(d.cdt, 0, n * sph::NC * 3 * 8));
+    for (const sph::PState* p : {&d.a, &d.b}) {
+        HIP_TRY(hipMemset(p->cdt, 0, n * sph::NC * 3 * 8));
+        if (p->cdt32) HIP_TRY(hipMemset(p->cdt32, 0, n * sph::NC * 3 * 4));
+    }
     HIP_TRY(hipMemset(d.flags, 0, 8));
     HIP_TRY(hipMemset(d.key, 0, n * 4));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
@@ -518,7 +521,8 @@ int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) { return fetch_ids(h, id, w
 
 int alloc_pstate(sph_dem* h, sph::PState& p, size_t n) {
     ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.pr0, n); ALLOC(p.pr1, n);
-    ALLOC(p.pr2, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n);
+    ALLOC(p.pr2, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n); ALLOC(p.cdt, n * sph::NC * 3);
+    p.cdt32 = nullptr;
     return SPH_OK;
 }
 
@@ -529,8 +533,7 @@ int allocate(sph_dem* h) {
     if ((rc = alloc_pstate(h, d.a, n)) || (rc = alloc_pstate(h, d.b, n))) return rc;
     ALLOC(d.pos_of, n);
     ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.pres, n);
-    ALLOC(d.cslot, n * sph::NC); ALLOC(d.cdt, n * sph::NC * 3);
-    d.cdt32 = nullptr;
+    ALLOC(d.cslot, n * sph::NC);
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
     ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
     ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
@@ -660,10 +663,11 @@ int check_flags(sph_dem* h) {
 }
 
 int ensure_mixed_buffers(sph_dem* h) {
-    if (h->d.cdt32) return SPH_OK;
+    if (h->d.a.cdt32) return SPH_OK;
     const size_t n = h->hs.np;
-    ALLOC(h->d.cdt32, n * sph::NC * 3);
-    HIP_TRY(hipMemset(h->d.cdt32, 0, n * sph::NC * 3 * 4));
+    ALLOC(h->d.a.cdt32, n * sph::NC * 3); ALLOC(h->d.b.cdt32, n * sph::NC * 3);
+    HIP_TRY(hipMemset(h->d.a.cdt32, 0, n * sph::NC * 3 * 4));
+    HIP_TRY(hipMemset(h->d.b.cdt32, 0, n * sph::NC * 3 * 4));
     ALLOC(h->d.a.vf, n); ALLOC(h->d.a.wf, n); ALLOC(h->d.b.vf, n); ALLOC(h->d.b.wf, n);
     return SPH_OK;
 }
@@ -894,8 +898,11 @@ struct Region {
     size_t bytes;
 };
 
-std::vector<Region> state_regions(sph_dem* h) {
+// `own_cur`: which buffer set holds the state (the springs are kept for that
+// set only: the other set's are written by the next step before any read).
+std::vector<Region> state_regions(sph_dem* h, int own_cur) {
     const DemDev& d = h->d;
+    const sph::PState& cur = own_cur ? d.a : d.b;
     const size_t n = h->hs.np, nc = sph::NC;
     const size_t ns = h->members.empty() ? 1 : h->members.size();
     std::vector<Region> r;
@@ -914,8 +921,8 @@ std::vector<Region> state_regions(sph_dem* h) {
     r.push_back({d.count, (size_t)d.K * 4});
     r.push_back({d.rank, n * 4});
     r.push_back({d.cslot, n * nc * 4});
-    if (d.cdt) r.push_back({d.cdt, n * nc * 3 * 8});
-    if (d.cdt32) r.push_back({d.cdt32, n * nc * 3 * 4});
+    r.push_back({cur.cdt, n * nc * 3 * 8});
+    if (cur.cdt32) r.push_back({cur.cdt32, n * nc * 3 * 4});
     r.push_back({d.wnx, 6 * ns * 32});
     r.push_back({d.wmvfd, 6 * ns * 32});
     r.push_back({d.wvel0, 6 * ns * 8});
@@ -942,7 +949,7 @@ sph::Checkpoint::~Checkpoint() {
 }
 
 int sph::save_state(sph_dem* h, Checkpoint& c) {
-    const auto regs = state_regions(h);
+    const auto regs = state_regions(h, h->own_cur);
     if (c.bufs.size() != regs.size()) {
         for (void* p : c.bufs) (void)hipFree(p);
         c.bufs.clear();
@@ -964,7 +971,7 @@ int sph::save_state(sph_dem* h, Checkpoint& c) {
 }
 
 int sph::restore_state(sph_dem* h, const Checkpoint& c) {
-    const auto regs = state_regions(h);
+    const auto regs = state_regions(h, c.own_cur);
     if (c.bufs.size() != regs.size()) {
         h->err = "state copy: the checkpoint does not match the handle";
         return SPH_E_STATE;
@@ -1178,15 +1185,23 @@ int get_particle_field(sph_dem* h, int f, void* dst, Range r) {
                     }
                 return SPH_OK;
             }
-            if (h->precision == SPH_PREC_MIXED && d.cdt32) {
+            // springs: slot k of storage position q at 3 * (k * np + q) in the current set
+            const size_t np = d.np;
+            std::vector<double> t(n * nc * 3);
+            if (h->precision == SPH_PREC_MIXED && own(h).cdt32) {
                 std::vector<float> tf(n * nc * 3);
-                HIP_TRY(hipMemcpy(tf.data(), d.cdt32 + r.base * nc * 3, tf.size() * 4, hipMemcpyDeviceToHost));
-                for (size_t i = 0; i < n * nc; ++i)
-                    for (int a = 0; a < 3; ++a) dd[3 * i + a] = occ(i / nc, i % nc) ? (double)tf[3 * i + a] : 0.0;
+                HIP_TRY(hipMemcpy2D(tf.data(), n * 12, own(h).cdt32 + 3 * r.base, np * 12, n * 12, nc,
+                                    hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < tf.size(); ++i) t[i] = (double)tf[i];
             } else {
-                HIP_TRY(hipMemcpy(dd, d.cdt + r.base * nc * 3, n * nc * 24, hipMemcpyDeviceToHost));
-                for (size_t i = 0; i < n * nc; ++i)
-                    if (!occ(i / nc, i % nc)) dd[3 * i] = dd[3 * i + 1] = dd[3 * i + 2] = 0.0;
+                HIP_TRY(hipMemcpy2D(t.data(), n * 24, own(h).cdt + 3 * r.base, np * 24, n * 24, nc,
+                                    hipMemcpyDeviceToHost));
+            }
+            for (size_t q = 0; q < n; ++q) {
+                const size_t i = id[q];
+                for (size_t k = 0; k < nc; ++k)
+                    for (int a = 0; a < 3; ++a)
+                        dd[3 * (i * nc + k) + a] = occ(i, k) ? t[3 * (k * n + q) + a] : 0.0;
             }
             return SPH_OK;
         }
@@ -1585,15 +1600,15 @@ int sph_set_precision(sph_t* h, int precision) {
         if ((rc = ensure_mixed_buffers(h))) return rc;
         std::vector<double> a(cnt);
         std::vector<float> b(cnt);
-        HIP_TRY(hipMemcpy(a.data(), h->d.cdt, cnt * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(a.data(), own(h).cdt, cnt * 8, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < cnt; ++i) b[i] = (float)a[i];
-        HIP_TRY(hipMemcpy(h->d.cdt32, b.data(), cnt * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(own(h).cdt32, b.data(), cnt * 4, hipMemcpyHostToDevice));
     } else {
         std::vector<double> a(cnt);
         std::vector<float> b(cnt);
-        HIP_TRY(hipMemcpy(b.data(), h->d.cdt32, cnt * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(b.data(), own(h).cdt32, cnt * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < cnt; ++i) a[i] = (double)b[i];
-        HIP_TRY(hipMemcpy(h->d.cdt, a.data(), cnt * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(own(h).cdt, a.data(), cnt * 8, hipMemcpyHostToDevice));
     }
     h->precision = precision;
     refresh_f32(h);

@@ -616,24 +616,26 @@ template <> struct PartnerVel<float> {
     static __device__ __forceinline__ V3<float> v3(float4 a) { return mk<float>(a.x, a.y, a.z); }
 };
 
+// The spring of contact slot k: read at the particle's storage position in
+// the step's input set, written at its sorted position in the output set.
 template <class T> struct SlotStore;
 template <> struct SlotStore<double> {
-    static __device__ __forceinline__ V3<double> load(const DemDev& d, size_t i) {
-        const double* p = d.cdt + 3 * i;
+    static __device__ __forceinline__ V3<double> load(const DemDev& d, int k, uint32_t q) {
+        const double* p = d.ro.cdt + 3 * ((size_t)k * d.np + q);
         return mk(p[0], p[1], p[2]);
     }
-    static __device__ __forceinline__ void store(const DemDev& d, size_t i, V3<double> v) {
-        double* p = d.cdt + 3 * i;
+    static __device__ __forceinline__ void store(const DemDev& d, int k, uint32_t q, V3<double> v) {
+        double* p = d.wo.cdt + 3 * ((size_t)k * d.np + q);
         p[0] = v.x; p[1] = v.y; p[2] = v.z;
     }
 };
 template <> struct SlotStore<float> {
-    static __device__ __forceinline__ V3<float> load(const DemDev& d, size_t i) {
-        const float* p = d.cdt32 + 3 * i;
+    static __device__ __forceinline__ V3<float> load(const DemDev& d, int k, uint32_t q) {
+        const float* p = d.ro.cdt32 + 3 * ((size_t)k * d.np + q);
         return mk(p[0], p[1], p[2]);
     }
-    static __device__ __forceinline__ void store(const DemDev& d, size_t i, V3<float> v) {
-        float* p = d.cdt32 + 3 * i;
+    static __device__ __forceinline__ void store(const DemDev& d, int k, uint32_t q, V3<float> v) {
+        float* p = d.wo.cdt32 + 3 * ((size_t)k * d.np + q);
         p[0] = v.x; p[1] = v.y; p[2] = v.z;
     }
 };
@@ -1722,8 +1724,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
             const double len = vlen(xab);
             const double delta_n = len - (ra + xb.w);
             if (delta_n < 0.0) {
-                const size_t mp = (size_t)o * NC + i;
-                V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, mp);
+                V3<T> dt0 = (fresh >> i) & 1u ? mk<T>(0, 0, 0) : SlotStore<T>::load(d, i, prev);
                 const typename PV::type vb = VW ? vbs[VW ? k : 0] : PV::vel(d, pb[k]);
                 const typename PV::type wb = VW ? wbs[VW ? k : 0] : PV::angvel(d, pb[k]);
                 if constexpr (M3)
@@ -1731,7 +1732,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
                 else
                     contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), PV::v3(vb), PV::v3(wb), (T)ra,
                                       (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
-                SlotStore<T>::store(d, mp, dt0);
+                SlotStore<T>::store(d, i, s, dt0);
             } else {
                 mask &= ~(1u << i);  // contact ended: the slot is empty again
             }
