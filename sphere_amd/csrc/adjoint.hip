@@ -578,13 +578,14 @@ __global__ __launch_bounds__(256) void k_adj_tape(DemDev d, TapeOut t) {
         t.tfresh[q] = d.tfresh[q];
         // the occupied slots only: the adjoint step reads no other (its own
         // and its partners' occupied slots of this step)
-        // (the input set keeps slot k's spring at 3 * (k * np + p), p the
-        // storage position; the tape keys it by original index and slot)
+        // (the input set keeps slot k's word at k * np + p and its spring at
+        // 3 * (k * np + p), p the storage position; the tape keys them by
+        // original index and slot)
         const size_t o = (size_t)d.sid[q] * NC;
         for (uint32_t m = mask; m; m &= m - 1) {
             const int k = __ffs(m) - 1;
             const size_t i = o + (size_t)k, src = 3 * ((size_t)k * d.np + p);
-            t.cslot[i] = d.cslot[i];
+            t.cslot[i] = d.ro.cslot[(size_t)k * d.np + p];
             if (t.cdt32) {
                 t.cdt32[3 * i] = d.ro.cdt32[src];
                 t.cdt32[3 * i + 1] = d.ro.cdt32[src + 1];

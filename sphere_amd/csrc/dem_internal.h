@@ -15,12 +15,12 @@
 // and the sorted ids (`sid`).  Records are 32-byte double4 so each access
 // fills whole sectors.
 //
-// Contact history stays keyed by original index: a 32-bit occupancy mask
-// (moved with the particle) plus SPH_NC slots.  A slot word packs the
-// partner's original index (low 28 bits) and the periodic image code (high 4
-// bits: 2 bits per periodic axis, 0 = none, 1 = +L, 2 = -L); the tangential
-// spring delta_t of a slot is 3 doubles (FP64) or 3 floats (mixed).  Only
-// occupied slots are read or written.
+// Contact history moves with the particle: a 32-bit occupancy mask and, per
+// slot (SPH_NC of them), a slot word and a tangential spring (3 doubles in
+// FP64, 3 floats in mixed mode), slot-major in each buffer set.  A slot word
+// packs the partner's original index (low 28 bits) and the periodic image
+// code (high 4 bits: 2 bits per periodic axis, 0 = none, 1 = +L, 2 = -L).
+// Only occupied slots are read or written.
 #ifndef SPHERE_DEM_INTERNAL_H_
 #define SPHERE_DEM_INTERNAL_H_
 
@@ -49,6 +49,32 @@ enum : uint32_t {
     FLAG_NAN = 4u,
 };
 
+// Unsigned division by an invariant divisor d >= 1 as a multiply-high, a
+// subtract, an add and two shifts (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, figure 4.1): with
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1 and t = umulhi(m, n),
+// n / d = (t + ((n - t) >> s1)) >> s2 for every 32-bit n, s1 = min(l, 1),
+// s2 = max(l - 1, 0).  The contact search takes cell coordinates apart per
+// candidate pair, where a 32-bit integer division (no single instruction on
+// the GPU) costs tens of instructions.
+struct FastDiv {
+    uint32_t d, m, s1, s2;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (uint64_t(1) << l) < d) ++l;
+    FastDiv f;
+    f.d = d;
+    f.m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 0 ? l - 1 : 0;
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    const uint32_t t = __umulhi(f.m, n);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 // One buffer set of per-particle state in storage order.
 struct PState {
     double4 *x;       // x, y, z, r
@@ -62,11 +88,14 @@ struct PState {
     uint32_t *cmask;  // occupied contact slots
     uint32_t *lidx;   // search mode 5: the particle's neighbour list (its sorted position at the build)
     float4 *vf, *wf;  // mixed mode: FP32 copies of vel and angvel (xyz) for the pair force's partner gathers
-    // tangential springs of the occupied contact slots, slot-major: slot k of
-    // the particle stored at q is (x, y, z) at 3 * (k * np + q).  A step reads
-    // them at the particle's storage position in its input set and writes
-    // them at its sorted position in the output set, so the lanes of a
-    // wavefront, working on consecutive particles, touch consecutive records.
+    // contact slots, slot-major: the slot word of slot k of the particle
+    // stored at q is cslot[k * np + q], its tangential spring (x, y, z) at
+    // 3 * (k * np + q).  A step's searches update the particle's words in its
+    // input set; the force kernel reads words and springs there and writes
+    // those of the contacts that last at the particle's sorted position in
+    // the output set, so the lanes of a wavefront, working on consecutive
+    // particles, touch consecutive records.
+    uint32_t* cslot;
     double* cdt;   // FP64 mode
     float* cdt32;  // mixed mode
 };
@@ -76,6 +105,7 @@ struct OwnSet {
     double4 *x, *vel, *angvel, *pr0, *pr1, *pr2;
     uint32_t *id, *cmask, *lidx;
     float4 *vf, *wf;
+    uint32_t* cslot;
     double* cdt;
     float* cdt32;
 };
@@ -89,6 +119,7 @@ struct OwnSet {
 struct Sys {
     double origo[3], L[3], cell_len[3];
     uint32_t num[3];
+    FastDiv div_x, div_xy;  // division by num[0] and by num[0] * num[1] (cell coordinates of a cell id)
     int periodic;
     float Lf[2];        // L[0], L[1] in FP32 (periodic images in the prefilter)
     uint32_t cbase, K;  // first cell id, cells
@@ -129,8 +160,6 @@ struct DemDev {
     double4 *torque;    // tx, ty, tz, ev_dot
     double* pres;       // p
 
-    // contact slots, keyed by original index (their springs live in the buffer sets)
-    uint32_t* cslot;    // np * NC
 
     // cell list
     uint32_t *key, *rank, *perm, *count, *off, *scan_bsum;  // off: K+1 prefix
@@ -150,6 +179,8 @@ struct DemDev {
     uint32_t* ncnt;     // mode 5: list length per list index
     float4* xref;       // mode 5: position at the last list build, per list index
     float* nlistg;      // mode 5: np * NLIST, each entry's gap at the build (FP32), transposed like nlist
+    uint8_t* nhint;     // mode 5: np * NLIST, the slot where the entry's partner was last found (a hint,
+                        // checked against the slot word before use), transposed like nlist
     float* ldisp;       // mode 5: displacement since the build, per list index (FP32)
     uint32_t* linv;     // mode 5: list index -> sorted position of this step
     uint32_t* slidx;    // mode 5: list index per sorted position of this step

@@ -65,7 +65,7 @@ void destroy_graph(sph_dem* h) {
 }
 
 sph::OwnSet own_set(const sph::PState& p) {
-    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx, p.vf, p.wf, p.cdt, p.cdt32};
+    return {p.x, p.vel, p.angvel, p.pr0, p.pr1, p.pr2, p.id, p.cmask, p.lidx, p.vf, p.wf, p.cslot, p.cdt, p.cdt32};
 }
 
 // the buffer set holding the current private records
@@ -148,6 +148,8 @@ sph::Sys make_sys(const HostState& s) {
         S.cell_len[a] = s.L[a] / (double)s.num[a];
         S.g[a] = s.g[a];
     }
+    S.div_x = sph::make_fastdiv(s.num[0]);
+    S.div_xy = sph::make_fastdiv(s.num[0] * s.num[1]);
     S.periodic = s.periodic;
     S.Lf[0] = (float)s.L[0];
     S.Lf[1] = (float)s.L[1];
@@ -411,8 +413,8 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemcpy(d.force, f4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.torque, t4.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.pres, s.p.data(), n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(d.cslot, 0, n * sph::NC * 4));
     for (const sph::PState* p : {&d.a, &d.b}) {
+        HIP_TRY(hipMemset(p->cslot, 0, n * sph::NC * 4));
         HIP_TRY(hipMemset(p->cdt, 0, n * sph::NC * 3 * 8));
         if (p->cdt32) HIP_TRY(hipMemset(p->cdt32, 0, n * sph::NC * 3 * 4));
     }
@@ -423,6 +425,7 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.count, 0, (size_t)d.K * 4));
     HIP_TRY(hipMemset(d.hcnt, 0, n * 4));
     HIP_TRY(hipMemset(d.ncnt, 0, n * 4));
+    HIP_TRY(hipMemset(d.nhint, 0, n * sph::NLIST));
     HIP_TRY(hipMemset(d.a.lidx, 0, n * 4));
     HIP_TRY(hipMemset(d.b.lidx, 0, n * 4));
     {
@@ -521,7 +524,7 @@ int fetch_ids(sph_dem* h, std::vector<uint32_t>& id) { return fetch_ids(h, id, w
 
 int alloc_pstate(sph_dem* h, sph::PState& p, size_t n) {
     ALLOC(p.x, n); ALLOC(p.vel, n); ALLOC(p.angvel, n); ALLOC(p.pr0, n); ALLOC(p.pr1, n);
-    ALLOC(p.pr2, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n); ALLOC(p.cdt, n * sph::NC * 3);
+    ALLOC(p.pr2, n); ALLOC(p.id, n); ALLOC(p.cmask, n); ALLOC(p.lidx, n); ALLOC(p.cslot, n * sph::NC); ALLOC(p.cdt, n * sph::NC * 3);
     p.cdt32 = nullptr;
     return SPH_OK;
 }
@@ -533,11 +536,10 @@ int allocate(sph_dem* h) {
     if ((rc = alloc_pstate(h, d.a, n)) || (rc = alloc_pstate(h, d.b, n))) return rc;
     ALLOC(d.pos_of, n);
     ALLOC(d.force, n); ALLOC(d.torque, n); ALLOC(d.pres, n);
-    ALLOC(d.cslot, n * sph::NC);
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
     ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
     ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
-    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.nlistg, n * sph::NLIST); ALLOC(d.ldisp, n); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
+    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.nlistg, n * sph::NLIST); ALLOC(d.nhint, n * sph::NLIST); ALLOC(d.ldisp, n); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     const size_t ns = h->members.empty() ? 1 : h->members.size();
     ALLOC(d.wnx, 6 * ns); ALLOC(d.wmvfd, 6 * ns); ALLOC(d.wvel0, 6 * ns); ALLOC(d.wmode, 6 * ns);
@@ -898,8 +900,9 @@ struct Region {
     size_t bytes;
 };
 
-// `own_cur`: which buffer set holds the state (the springs are kept for that
-// set only: the other set's are written by the next step before any read).
+// `own_cur`: which buffer set holds the state (the slot words and springs
+// are kept for that set only: the other set's are written by the next step
+// before any read).
 std::vector<Region> state_regions(sph_dem* h, int own_cur) {
     const DemDev& d = h->d;
     const sph::PState& cur = own_cur ? d.a : d.b;
@@ -920,7 +923,7 @@ std::vector<Region> state_regions(sph_dem* h, int own_cur) {
     r.push_back({d.key, n * 4});  // the next step's keys with their counts and ranks
     r.push_back({d.count, (size_t)d.K * 4});
     r.push_back({d.rank, n * 4});
-    r.push_back({d.cslot, n * nc * 4});
+    r.push_back({cur.cslot, n * nc * 4});
     r.push_back({cur.cdt, n * nc * 3 * 8});
     if (cur.cdt32) r.push_back({cur.cdt32, n * nc * 3 * 4});
     r.push_back({d.wnx, 6 * ns * 32});
@@ -1159,7 +1162,13 @@ int get_particle_field(sph_dem* h, int f, void* dst, Range r) {
             if ((rc = fetch_ids(h, id, r))) return rc;
             HIP_TRY(hipMemcpy(smask.data(), own(h).cmask + r.base, n * 4, hipMemcpyDeviceToHost));
             for (size_t q = 0; q < n; ++q) mask[id[q]] = smask[q];
-            HIP_TRY(hipMemcpy(slot.data(), d.cslot + r.base * nc, n * nc * 4, hipMemcpyDeviceToHost));
+            {  // slot k of storage position q at k * np + q in the current set
+                std::vector<uint32_t> t(n * nc);
+                HIP_TRY(hipMemcpy2D(t.data(), n * 4, own(h).cslot + r.base, (size_t)d.np * 4, n * 4, nc,
+                                    hipMemcpyDeviceToHost));
+                for (size_t q = 0; q < n; ++q)
+                    for (size_t k = 0; k < nc; ++k) slot[id[q] * nc + k] = t[k * n + q];
+            }
             auto occ = [&](size_t i, size_t k) { return ((mask[i] >> k) & 1u) != 0; };
             if (f == SPH_CONTACTS) {
                 for (size_t i = 0; i < n; ++i)
@@ -1185,7 +1194,7 @@ int get_particle_field(sph_dem* h, int f, void* dst, Range r) {
                     }
                 return SPH_OK;
             }
-            // springs: slot k of storage position q at 3 * (k * np + q) in the current set
+            // springs: slot k of storage position q at 3 * (k * np + q), like the slot words
             const size_t np = d.np;
             std::vector<double> t(n * nc * 3);
             if (h->precision == SPH_PREC_MIXED && own(h).cdt32) {

@@ -656,6 +656,18 @@ struct TopoShared {
     uint32_t scan[BLOCK / 64];
 };
 
+// The slot words of one particle in the step's input set: word k of the
+// particle stored at q is cslot[k * np + q] (slot-major, like the springs).
+// A search updates the row in place (each particle's row is its own); the
+// force kernel carries the words of the contacts that last into the output
+// set at the particle's sorted position.
+struct SlotRow {
+    uint32_t* p;
+    size_t ld;
+    __device__ __forceinline__ uint32_t& operator[](int k) const { return p[(size_t)k * ld]; }
+};
+__device__ __forceinline__ SlotRow slot_row(const DemDev& d, uint32_t s) { return {d.ro.cslot + d.perm[s], d.np}; }
+
 // Slot bookkeeping of one confirmed overlap (src/contactsearch.cuh:252-284):
 // the slot already holding b (lowest), else the highest empty slot.
 struct SlotState {
@@ -663,7 +675,7 @@ struct SlotState {
     bool stop;
 };
 
-__device__ __noinline__ void slot_insert(uint32_t* slot, SlotState& st, uint32_t bo, uint32_t code) {
+__device__ __noinline__ void slot_insert(SlotRow slot, SlotState& st, uint32_t bo, uint32_t code) {
     int cpos = -1;
     for (uint32_t m = st.mask; m; m &= m - 1) {
         const int i = __ffs(m) - 1;
@@ -715,8 +727,7 @@ __device__ __forceinline__ void topo_search(const DemDev& d, const TopoShared& s
         const uint32_t s = sh.cglob[eh] + (li - sh.cstart[eh]);
         const double4 xa = staged_pos<FROM_LDS>(d, sh, li, s);
         const double ra = xa.w;
-        const uint32_t o = d.sid[s];
-        uint32_t* slot = d.cslot + (size_t)o * NC;
+        const SlotRow slot = slot_row(d, s);
         SlotState st;
         st.mask = d.ro.cmask[d.perm[s]];
         st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false;
@@ -844,7 +855,7 @@ struct RowSlots {
 // (Takes plain pointers rather than the DemDev, which a non-inlined call
 // would have to copy to scratch memory.)
 __device__ __noinline__ void rows_try(PosView pv, const uint32_t* __restrict__ sid, uint32_t* sw,
-                                      const uint32_t* gslot, RowSlots& st, double4 xa, uint32_t sb, uint32_t code) {
+                                      SlotRow gslot, RowSlots& st, double4 xa, uint32_t sb, uint32_t code) {
     const double4 xb = pv_pos(pv, sb);
     const uint32_t cx = code & 3u, cy = (code >> 2) & 3u;
     const double dmx = cx == 1u ? pv.Lx : (cx == 2u ? -pv.Lx : 0.0);
@@ -891,7 +902,7 @@ constexpr int ROW_BATCH = 8;
 template <bool BUILD>
 __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t lo, uint32_t hi, float dmx, float dmy,
                                          uint32_t code, uint32_t s, float4 xaf, float ra_eps, float ra_skin,
-                                         double4 xa, uint32_t* sw, const uint32_t* gslot, RowSlots& st,
+                                         double4 xa, uint32_t* sw, SlotRow gslot, RowSlots& st,
                                          uint32_t* list, uint32_t& lcnt) {
     for (uint32_t sb0 = lo; sb0 < hi; sb0 += ROW_BATCH) {
         float4 c[ROW_BATCH];
@@ -1024,8 +1035,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     const float4 xaf = d.xf[s];
     const float ra_eps = xaf.w + d.search_eps;
     const float ra_skin = ra_eps + d.list_skin;
-    const uint32_t o = d.sid[s];
-    uint32_t* gslot = d.cslot + (size_t)o * NC;
+    const SlotRow gslot = slot_row(d, s);
     uint32_t* list = BUILD ? d.nlist + s : nullptr;  // entry k at list[k * np]
     uint32_t lcnt = 0;
     // rows of cells farther than any hit (or list entry) can reach are skipped
@@ -1225,8 +1235,7 @@ constexpr int APPLY_BLOCK = 128;
 // APPLY_BLOCK).
 __device__ __forceinline__ void apply_hits(const DemDev& d, uint32_t s, uint64_t* k, uint32_t n, uint32_t mask0,
                                            uint32_t* sw, uint32_t err) {
-    const uint32_t o = d.sid[s];
-    uint32_t* gslot = d.cslot + (size_t)o * NC;
+    const SlotRow gslot = slot_row(d, s);
     for (uint32_t m = mask0; m; m &= m - 1) {
         const int i = __ffs(m) - 1;
         sw[i * APPLY_BLOCK] = gslot[i];
@@ -1320,18 +1329,26 @@ __device__ __forceinline__ bool cell_offset(int g, int b, int n, bool wrap, int&
 
 constexpr int LIST_BATCH = 4;
 
+// Cell coordinates of a system-local cell id (x fastest, then y, then z).
+__device__ __forceinline__ void cell_xyz(const Sys& S, uint32_t key, int& gx, int& gy, int& gz) {
+    const uint32_t z = fdiv(key, S.div_xy), r = key - z * S.div_xy.d, y = fdiv(r, S.div_x);
+    gx = (int)(r - y * S.div_x.d);
+    gy = (int)y;
+    gz = (int)z;
+}
+
 // Full FP64 treatment of one list partner that passed the prefilter: its cell
 // offset and image as the reference's search would reach it and the exact
 // overlap test; returns the hit key, or 0 for no overlap (a real key is
 // never 0: its sorted position differs from the particle's own).
 __device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_t kb, double4 x, double4 xa,
                                               int gx, int gy, int gz) {
-    const uint32_t nx = S.num[0], ny = S.num[1];
-    int dx, dy, dz;
+    int bx, by, bz, dx, dy, dz;
     uint32_t cx, cy, cz;
-    if (!cell_offset(gz, (int)(kb / (nx * ny)), (int)S.num[2], false, dz, cz)) return 0ull;
-    if (!cell_offset(gy, (int)((kb / nx) % ny), (int)ny, S.periodic == 1, dy, cy)) return 0ull;
-    if (!cell_offset(gx, (int)(kb % nx), (int)nx, S.periodic != 0, dx, cx)) return 0ull;
+    cell_xyz(S, kb, bx, by, bz);
+    if (!cell_offset(gz, bz, (int)S.num[2], false, dz, cz)) return 0ull;
+    if (!cell_offset(gy, by, (int)S.num[1], S.periodic == 1, dy, cy)) return 0ull;
+    if (!cell_offset(gx, bx, (int)S.num[0], S.periodic != 0, dx, cx)) return 0ull;
     const uint32_t code = cx | (cy << 2);
     (void)wrap_xy(S, x.x, x.y);
     const double dmx = cx == 1u ? S.L[0] : (cx == 2u ? -S.L[0] : 0.0);
@@ -1345,7 +1362,9 @@ __device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_
 
 // An overlap whose partner already holds a slot is applied on the spot (its
 // slot does not depend on the order, a partner occurs once); the others are
-// collected and given empty slots in key order at the end.
+// collected and given empty slots in key order at the end.  Each list entry
+// keeps the slot its partner was last found in: one slot-word compare
+// confirms it, and only a stale or missing hint scans the occupied slots.
 template <bool B>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
@@ -1359,6 +1378,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     const float ra_eps = xaf.w + d.search_eps;
     const uint32_t li = d.slidx[s];
     const uint32_t* list = d.nlist + li;  // entry k at list[k * np]
+    uint8_t* hint = d.nhint + li;
     const uint32_t nl = d.ncnt[li];
     const uint32_t mask0 = d.ro.cmask[p];
     const bool px = S.periodic != 0, py = S.periodic == 1;
@@ -1367,7 +1387,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     bool have_a = false;
     double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
     int gx = 0, gy = 0, gz = 0;
-    uint32_t* gslot = nullptr;
+    SlotRow gslot{nullptr, 0};
     // A pair whose gap at the build exceeds the two spheres' displacements
     // since then cannot overlap now: skip it without touching the partner's
     // record (margins of three FP32 prefilter widths for the rounding of the
@@ -1375,13 +1395,14 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     const float da = d.ldisp[li] + 3.0f * d.search_eps;
     const float* gap = d.nlistg + li;
     for (uint32_t i0 = 0; i0 < nl; i0 += LIST_BATCH) {
-        uint32_t lb[LIST_BATCH], sb[LIST_BATCH];
+        uint32_t lb[LIST_BATCH], sb[LIST_BATCH], hb[LIST_BATCH];
         float g[LIST_BATCH];
         float4 c[LIST_BATCH];
 #pragma unroll
         for (int k = 0; k < LIST_BATCH; ++k) {
             lb[k] = i0 + k < nl ? list[(size_t)(i0 + k) * d.np] : li;
             g[k] = i0 + k < nl ? gap[(size_t)(i0 + k) * d.np] : 1e30f;
+            hb[k] = i0 + k < nl ? hint[(size_t)(i0 + k) * d.np] & 31u : 0u;
         }
         uint32_t near = 0;
 #pragma unroll
@@ -1407,11 +1428,8 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
         if (pass == 0u) continue;
         if (!have_a) {
             xa = spos(d, S, s);
-            const uint32_t key = d.skey[s] - S.cbase;
-            gx = (int)(key % S.num[0]);
-            gy = (int)((key / S.num[0]) % S.num[1]);
-            gz = (int)(key / (S.num[0] * S.num[1]));
-            gslot = d.cslot + (size_t)d.sid[s] * NC;
+            cell_xyz(S, d.skey[s] - S.cbase, gx, gy, gz);
+            gslot = slot_row(d, s);
             for (uint32_t m = mask0; m; m &= m - 1) {
                 const int i = __ffs(m) - 1;
                 sw[i * APPLY_BLOCK] = gslot[i];
@@ -1434,9 +1452,14 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
             if (v == 0ull) continue;
             ++n;
             int cpos = -1;
-            for (uint32_t m = mask0; m; m &= m - 1) {
-                const int i = __ffs(m) - 1;
-                if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) { cpos = i; break; }
+            if (((mask0 >> hb[k]) & 1u) && (sw[hb[k] * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) {
+                cpos = (int)hb[k];
+            } else {
+                for (uint32_t m = mask0; m; m &= m - 1) {
+                    const int i = __ffs(m) - 1;
+                    if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) { cpos = i; break; }
+                }
+                if (cpos >= 0) hint[(size_t)(i0 + k) * d.np] = (uint8_t)cpos;
             }
             if (cpos >= 0) {  // kept; written back only if its image changed
                 const uint32_t w = bo[k] | ((uint32_t)(v >> 32) & 15u) << 28;
@@ -1536,7 +1559,7 @@ struct MemberState {
 
 // Overlaps of the member at sorted position m with one chunk of candidates.
 __device__ __forceinline__ void member_chunk(const Cand& cb, uint32_t lane, uint32_t m, double4 xa,
-                                             const uint32_t* slot, uint32_t& w, bool& dirty, MemberState& st) {
+                                             SlotRow slot, uint32_t& w, bool& dirty, MemberState& st) {
     const double ex = (xa.x - cb.x) + cb.dmx, ey = (xa.y - cb.y) + cb.dmy, ez = xa.z - cb.z;
     const double d2 = ex * ex + ey * ey + ez * ez;
     const double rsum = xa.w + cb.r;
@@ -1550,7 +1573,7 @@ __device__ __forceinline__ void member_chunk(const Cand& cb, uint32_t lane, uint
         const uint32_t bo = __builtin_amdgcn_readlane(cb.bid, l);
         const uint32_t code = __builtin_amdgcn_readlane(cb.code, l);
         if (!st.loaded) {
-            w = lane < (uint32_t)NC ? slot[lane] : 0u;
+            w = lane < (uint32_t)NC ? slot[(int)lane] : 0u;
             st.loaded = true;
         }
         const uint64_t match = __ballot(lane < (uint32_t)NC && ((st.mask >> lane) & 1u) && (w & SLOT_IDX_MASK) == bo);
@@ -1626,8 +1649,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
 
     for (uint32_t m = m0; m < m1; ++m) {
         const double4 xa = spos(d, S, m);
-        const uint32_t o = d.sid[m];
-        const uint32_t* slot = d.cslot + (size_t)o * NC;
+        const SlotRow slot = slot_row(d, m);
         MemberState st;
         st.mask = d.ro.cmask[d.perm[m]];
         st.fresh = 0; st.nc = 0; st.err = 0; st.stop = false; st.loaded = false;
@@ -1640,7 +1662,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_wave(DemDev d) {
             const Cand cb = load_cand(d, base + lane, T, runs, r_lo, r_pre, r_len, r_code);
             member_chunk(cb, lane, m, xa, slot, w, dirty, st);
         }
-        if (dirty) d.cslot[(size_t)o * NC + lane] = w;
+        if (dirty) slot[(int)lane] = w;
         if (lane == 0) {
             d.tmask[m] = st.mask;
             d.tfresh[m] = st.fresh;
@@ -1674,7 +1696,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     const bool viscous = S.contactmodel == 1;
     uint32_t mask = viscous ? 0u : d.tmask[s];
     const uint32_t fresh = viscous ? 0u : d.tfresh[s];  // slots taken from empty this step: delta_t starts at 0
-    const uint32_t* slot = d.cslot + (size_t)o * NC;
+    const SlotRow slot{d.ro.cslot + prev, d.np};
 
     // ---- forces from the occupied slots in slot order (src/contactsearch.cuh:479-543)
     const double4 va4 = d.ro.vel[prev], wa4 = d.ro.angvel[prev];
@@ -1733,6 +1755,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
                     contact_linear<T>(S, acc, cvt<T>(va), cvt<T>(wa), PV::v3(vb), PV::v3(wb), (T)ra,
                                       (T)xb.w, cvt<T>(xab), (T)len, (T)delta_n, dt0);
                 SlotStore<T>::store(d, i, s, dt0);
+                d.wo.cslot[(size_t)i * d.np + s] = sw[k];  // the contact lasts: its slot word moves with it
             } else {
                 mask &= ~(1u << i);  // contact ended: the slot is empty again
             }
