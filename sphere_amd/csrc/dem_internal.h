@@ -179,12 +179,14 @@ struct DemDev {
     uint32_t* ncnt;     // mode 5: list length per list index
     float4* xref;       // mode 5: position at the last list build, per list index
     float* nlistg;      // mode 5: np * NLIST, each entry's gap at the build (FP32), transposed like nlist
+    uint32_t* nlo;      // mode 5: np * NLIST, each entry's partner by original index, transposed like nlist
     uint8_t* nhint;     // mode 5: np * NLIST, the slot where the entry's partner was last found (a hint,
                         // checked against the slot word before use), transposed like nlist
     float* ldisp;       // mode 5: displacement since the build, per list index (FP32)
     uint32_t* linv;     // mode 5: list index -> sorted position of this step
     uint32_t* slidx;    // mode 5: list index per sorted position of this step
-    uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed, [3] builds so far
+    uint32_t* nstate;   // mode 5: [0] rebuild this step, [1] rebuild requested, [2] build failed, [3] builds so far,
+                        // [4] a sphere crossed a periodic face in the last step, [5] ... in this step
     float list_skin;    // mode 5: list reach beyond r_a + r_b (m)
     float list_lim;     // mode 5: displacement since the build that requests a rebuild (m)
     uint32_t list_shell;  // mode 5: list builds also scan the cells two away

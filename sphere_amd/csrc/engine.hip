@@ -429,8 +429,8 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.a.lidx, 0, n * 4));
     HIP_TRY(hipMemset(d.b.lidx, 0, n * 4));
     {
-        const uint32_t ns[4] = {0u, 1u, 0u, 0u};  // neighbour lists: rebuild at the first step
-        HIP_TRY(hipMemcpy(d.nstate, ns, 16, hipMemcpyHostToDevice));
+        const uint32_t ns[6] = {0u, 1u, 0u, 0u, 0u, 0u};  // neighbour lists: rebuild at the first step
+        HIP_TRY(hipMemcpy(d.nstate, ns, 24, hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemcpy(d.a.x, s.x.data(), n * 32, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d.perm, 0, n * 4));
@@ -539,7 +539,7 @@ int allocate(sph_dem* h) {
     ALLOC(d.key, n); ALLOC(d.rank, n); ALLOC(d.perm, n);
     ALLOC(d.tmask, n); ALLOC(d.tfresh, n); ALLOC(d.skey, n); ALLOC(d.xf, n);
     ALLOC(d.hcnt, n); ALLOC(d.hits, n * sph::NC); ALLOC(d.sid, n);
-    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.nlistg, n * sph::NLIST); ALLOC(d.nhint, n * sph::NLIST); ALLOC(d.ldisp, n); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 4); ALLOC(d.linv, n); ALLOC(d.slidx, n);
+    ALLOC(d.nlist, n * sph::NLIST); ALLOC(d.nlistg, n * sph::NLIST); ALLOC(d.nhint, n * sph::NLIST); ALLOC(d.nlo, n * sph::NLIST); ALLOC(d.ldisp, n); ALLOC(d.ncnt, n); ALLOC(d.xref, n); ALLOC(d.nstate, 6); ALLOC(d.linv, n); ALLOC(d.slidx, n);
     ALLOC(d.count, d.K); ALLOC(d.off, (size_t)d.K + 1); ALLOC(d.scan_bsum, d.scan_blocks);
     const size_t ns = h->members.empty() ? 1 : h->members.size();
     ALLOC(d.wnx, 6 * ns); ALLOC(d.wmvfd, 6 * ns); ALLOC(d.wvel0, 6 * ns); ALLOC(d.wmode, 6 * ns);
@@ -931,7 +931,7 @@ std::vector<Region> state_regions(sph_dem* h, int own_cur) {
     r.push_back({d.wvel0, 6 * ns * 8});
     r.push_back({d.t_cur, ns * 8});
     r.push_back({d.flags, 8});
-    if (d.nstate) r.push_back({d.nstate, 16});  // list-build requests and count: builds on a copy do not count
+    if (d.nstate) r.push_back({d.nstate, 24});  // list-build requests and count: builds on a copy do not count
     r.push_back({d.force, n * 32});
     r.push_back({d.torque, n * 32});
     r.push_back({d.pres, n * 8});

@@ -215,6 +215,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(DemDev d) {
         if (build && d.search_mode == 5) d.nstate[3] += 1u;  // list builds so far
         d.nstate[1] = 0u;
         d.nstate[2] = 0u;
+        d.nstate[4] = d.nstate[5];  // a sphere crossed a periodic face in the last step
+        d.nstate[5] = 0u;
     }
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     uint32_t sum = 0;
@@ -928,6 +930,7 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
                 if (sb == s) continue;
                 if (lcnt < (uint32_t)NLIST) {
                     list[(size_t)lcnt * d.np] = sb;  // the partner's list index
+                    d.nlo[(size_t)lcnt * d.np + s] = d.sid[sb];  // and its original index
                     const float ex = (xaf.x - c[k].x) + dmx, ey = (xaf.y - c[k].y) + dmy, ez = xaf.z - c[k].z;
                     d.nlistg[(size_t)lcnt * d.np + s] = sqrtf(ex * ex + ey * ey + ez * ez) - (xaf.w + c[k].w);
                 }
@@ -970,6 +973,7 @@ struct CellBox {
 };
 
 __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const float4* __restrict__ xf,
+                                        const uint32_t* __restrict__ sid, uint32_t* lo,
                                         uint32_t* list, float* gap, uint32_t np, uint32_t& lcnt, int gx, int gy, int gz, int nx,
                                         int ny, int nz, int periodic, float Lx, float Ly, uint32_t cbase,
                                         float4 xaf, float ra_skin, CellBox cb3) {
@@ -1008,6 +1012,7 @@ __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const 
                     if (d2 <= rl * rl) {
                         if (lcnt < (uint32_t)NLIST) {
                             list[(size_t)lcnt * np] = sb;
+                            lo[(size_t)lcnt * np] = sid[sb];
                             gap[(size_t)lcnt * np] = sqrtf(d2) - (xaf.w + cb.w);
                         }
                         ++lcnt;
@@ -1101,7 +1106,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     if (st.err) atomicOr(d.flags, st.err);
     if constexpr (BUILD) {  // list index = this sorted position
         if (d.list_shell && !st.stop)
-            list_shell(d.off, d.xf, list, d.nlistg + s, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0],
+            list_shell(d.off, d.xf, d.sid, d.nlo + s, list, d.nlistg + s, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0],
                        S.Lf[1], S.cbase, xaf, ra_skin, cb3);
         d.ncnt[s] = lcnt < (uint32_t)NLIST ? lcnt : (uint32_t)NLIST;
         d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
@@ -1227,6 +1232,27 @@ __global__ __launch_bounds__(BLOCK) void k_topo_half(DemDev d) {
 // in place, the slot words cached in LDS.
 constexpr int APPLY_BLOCK = 128;
 
+// Copies the occupied slot words of a row into the particle's LDS column
+// (stride APPLY_BLOCK), eight independent loads at a time.
+__device__ __forceinline__ void stage_slots(SlotRow gslot, uint32_t mask, uint32_t* sw) {
+    constexpr int G = 8;
+    for (uint32_t m = mask; m;) {
+        int si[G];
+        uint32_t w[G];
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            si[k] = m ? __ffs(m) - 1 : 0;
+            if (m) { m &= m - 1; ++cnt; }
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) w[k] = k < cnt ? gslot[si[k]] : 0u;
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (k < cnt) sw[si[k] * APPLY_BLOCK] = w[k];
+    }
+}
+
 // Applies the n (<= NC) hit keys k of sorted particle s through the slot
 // rule as if in key order.  A partner occurs at most once among the hits, so
 // a hit whose partner already holds a slot lands in that slot whatever the
@@ -1236,10 +1262,7 @@ constexpr int APPLY_BLOCK = 128;
 __device__ __forceinline__ void apply_hits(const DemDev& d, uint32_t s, uint64_t* k, uint32_t n, uint32_t mask0,
                                            uint32_t* sw, uint32_t err) {
     const SlotRow gslot = slot_row(d, s);
-    for (uint32_t m = mask0; m; m &= m - 1) {
-        const int i = __ffs(m) - 1;
-        sw[i * APPLY_BLOCK] = gslot[i];
-    }
+    stage_slots(gslot, mask0, sw);
     uint32_t mask = mask0, fresh = 0, dirty = 0, nnew = 0;
     for (uint32_t h = 0; h < n; ++h) {
         const uint64_t v = k[h];
@@ -1329,6 +1352,7 @@ __device__ __forceinline__ bool cell_offset(int g, int b, int n, bool wrap, int&
 
 constexpr int LIST_BATCH = 4;
 
+
 // Cell coordinates of a system-local cell id (x fastest, then y, then z).
 __device__ __forceinline__ void cell_xyz(const Sys& S, uint32_t key, int& gx, int& gy, int& gz) {
     const uint32_t z = fdiv(key, S.div_xy), r = key - z * S.div_xy.d, y = fdiv(r, S.div_x);
@@ -1365,6 +1389,16 @@ __device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_
 // collected and given empty slots in key order at the end.  Each list entry
 // keeps the slot its partner was last found in: one slot-word compare
 // confirms it, and only a stale or missing hint scans the occupied slots.
+// Held contacts are skipped outright where the search could only rewrite
+// their slot word unchanged.  That is so when no sphere crossed a periodic
+// face in the last step: the image through which two overlapping spheres
+// meet changes only when one of them wraps, and the word was written (or
+// confirmed) by the last step's search, while the pair overlapped.  It is
+// also so, in any step, when the word carries no periodic image and the
+// particle's cell is off the periodic faces: any overlap the search finds
+// with that partner has no image either (and one it does not find leaves
+// the word as it is).  Skipped entries are not counted; a contact overflow is
+// still caught when a new contact finds no empty slot.
 template <bool B>
 __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
@@ -1379,6 +1413,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     const uint32_t li = d.slidx[s];
     const uint32_t* list = d.nlist + li;  // entry k at list[k * np]
     uint8_t* hint = d.nhint + li;
+    const uint32_t* lorig = d.nlo + li;
     const uint32_t nl = d.ncnt[li];
     const uint32_t mask0 = d.ro.cmask[p];
     const bool px = S.periodic != 0, py = S.periodic == 1;
@@ -1386,27 +1421,56 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     uint32_t n = 0, nnew = 0, err = 0, dirty = 0;
     bool have_a = false;
     double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
-    int gx = 0, gy = 0, gz = 0;
-    SlotRow gslot{nullptr, 0};
+    int gx, gy, gz;
+    cell_xyz(S, d.skey[s] - S.cbase, gx, gy, gz);
+    const bool interior = (!px || (gx > 0 && gx < (int)S.num[0] - 1)) && (!py || (gy > 0 && gy < (int)S.num[1] - 1));
+    const bool no_wraps = d.nstate[4] == 0u;
+    const bool skip_held = interior || no_wraps;
+    const uint32_t held_bits = no_wraps ? SLOT_IDX_MASK : 0xffffffffu;  // with wraps: the word must carry no image
+    const SlotRow gslot = slot_row(d, s);
+    stage_slots(gslot, mask0, sw);
     // A pair whose gap at the build exceeds the two spheres' displacements
     // since then cannot overlap now: skip it without touching the partner's
     // record (margins of three FP32 prefilter widths for the rounding of the
     // gap and of the displacements)
     const float da = d.ldisp[li] + 3.0f * d.search_eps;
+    // No sphere has moved more than list_lim since the build (a step after
+    // one that did rebuilds the lists instead): a gap beyond this cannot
+    // close, whatever the partner's displacement, which is then not read.
+    const float dfar = da + d.list_lim + 3.0f * d.search_eps;
     const float* gap = d.nlistg + li;
+    // the next batch's entries are loaded while the current one is processed
+    uint32_t qlb[LIST_BATCH], qhb[LIST_BATCH], qlw[LIST_BATCH];
+    float qg[LIST_BATCH];
+    auto fetch = [&](uint32_t i0) {
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k) {
+            const bool in = i0 + k < nl;
+            qlb[k] = in ? list[(size_t)(i0 + k) * d.np] : li;
+            qg[k] = in ? gap[(size_t)(i0 + k) * d.np] : 1e30f;
+            qhb[k] = in ? hint[(size_t)(i0 + k) * d.np] & 31u : 0u;
+            qlw[k] = in && skip_held ? lorig[(size_t)(i0 + k) * d.np] : 0xffffffffu;
+        }
+    };
+    fetch(0);
     for (uint32_t i0 = 0; i0 < nl; i0 += LIST_BATCH) {
-        uint32_t lb[LIST_BATCH], sb[LIST_BATCH], hb[LIST_BATCH];
+        uint32_t lb[LIST_BATCH], sb[LIST_BATCH], hb[LIST_BATCH], lw[LIST_BATCH];
         float g[LIST_BATCH];
         float4 c[LIST_BATCH];
 #pragma unroll
         for (int k = 0; k < LIST_BATCH; ++k) {
-            lb[k] = i0 + k < nl ? list[(size_t)(i0 + k) * d.np] : li;
-            g[k] = i0 + k < nl ? gap[(size_t)(i0 + k) * d.np] : 1e30f;
-            hb[k] = i0 + k < nl ? hint[(size_t)(i0 + k) * d.np] & 31u : 0u;
+            lb[k] = qlb[k];
+            g[k] = qg[k];
+            hb[k] = qhb[k];
+            lw[k] = qlw[k];
         }
+        if (i0 + LIST_BATCH < nl) fetch(i0 + LIST_BATCH);
         uint32_t near = 0;
 #pragma unroll
-        for (int k = 0; k < LIST_BATCH; ++k) near |= g[k] <= da + d.ldisp[lb[k]] ? 1u << k : 0u;
+        for (int k = 0; k < LIST_BATCH; ++k) {
+            const bool held = ((mask0 >> hb[k]) & 1u) && (sw[hb[k] * APPLY_BLOCK] & held_bits) == lw[k];
+            near |= !held && g[k] <= dfar && g[k] <= da + d.ldisp[lb[k]] ? 1u << k : 0u;
+        }
         if (near == 0u) continue;
 #pragma unroll
         for (int k = 0; k < LIST_BATCH; ++k) sb[k] = (near >> k) & 1u ? d.linv[lb[k]] : s;
@@ -1428,12 +1492,6 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
         if (pass == 0u) continue;
         if (!have_a) {
             xa = spos(d, S, s);
-            cell_xyz(S, d.skey[s] - S.cbase, gx, gy, gz);
-            gslot = slot_row(d, s);
-            for (uint32_t m = mask0; m; m &= m - 1) {
-                const int i = __ffs(m) - 1;
-                sw[i * APPLY_BLOCK] = gslot[i];
-            }
             have_a = true;
         }
         // the batch's keys, FP64 positions and original indices, independent loads
@@ -1912,6 +1970,14 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     d.key[s] = nkey;
     if (cerr) atomicOr(d.flags_next, cerr);
     count_key(d, s, nkey);
+    if (d.search_mode == 5 && S.periodic != 0) {
+        // a cell coordinate that jumps by more than one along a periodic axis:
+        // the sphere crossed the face (its contacts' image codes may change)
+        int ox, oy, oz, nx_, ny_, nz_;
+        cell_xyz(S, d.skey[s] - S.cbase, ox, oy, oz);
+        cell_xyz(S, nkey - S.cbase, nx_, ny_, nz_);
+        if (abs(nx_ - ox) > 1 || (S.periodic == 1 && abs(ny_ - oy) > 1)) atomicOr(&d.nstate[5], 1u);
+    }
 
     // neighbour lists (search mode 5): the list index travels with the
     // particle; request a rebuild once it moved list_lim since the build
