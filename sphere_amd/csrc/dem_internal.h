@@ -179,9 +179,10 @@ struct DemDev {
     uint32_t* ncnt;     // mode 5: list length per list index
     float4* xref;       // mode 5: position at the last list build, per list index
     float* nlistg;      // mode 5: np * NLIST, each entry's gap at the build (FP32), transposed like nlist
-    uint32_t* nlo;      // mode 5: np * NLIST, each entry's partner by original index, transposed like nlist
+    uint32_t* nlo;      // mode 5: np * NLIST, the entry's partner by original index, transposed like nlist
+                        // (valid with the hint: both are written when the partner is found in a slot)
     uint8_t* nhint;     // mode 5: np * NLIST, the slot where the entry's partner was last found (a hint,
-                        // checked against the slot word before use), transposed like nlist
+                        // checked against the slot word before use; >= 32 for none), transposed like nlist
     float* ldisp;       // mode 5: displacement since the build, per list index (FP32)
     uint32_t* linv;     // mode 5: list index -> sorted position of this step
     uint32_t* slidx;    // mode 5: list index per sorted position of this step

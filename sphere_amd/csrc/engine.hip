@@ -425,7 +425,7 @@ int upload_all(sph_dem* h) {
     HIP_TRY(hipMemset(d.count, 0, (size_t)d.K * 4));
     HIP_TRY(hipMemset(d.hcnt, 0, n * 4));
     HIP_TRY(hipMemset(d.ncnt, 0, n * 4));
-    HIP_TRY(hipMemset(d.nhint, 0, n * sph::NLIST));
+    HIP_TRY(hipMemset(d.nhint, 0xff, n * sph::NLIST));
     HIP_TRY(hipMemset(d.a.lidx, 0, n * 4));
     HIP_TRY(hipMemset(d.b.lidx, 0, n * 4));
     {

@@ -930,7 +930,7 @@ __device__ __forceinline__ void rows_run(const DemDev& d, const Sys& S, uint32_t
                 if (sb == s) continue;
                 if (lcnt < (uint32_t)NLIST) {
                     list[(size_t)lcnt * d.np] = sb;  // the partner's list index
-                    d.nlo[(size_t)lcnt * d.np + s] = d.sid[sb];  // and its original index
+                    d.nhint[(size_t)lcnt * d.np + s] = 0xffu;  // no slot known yet
                     const float ex = (xaf.x - c[k].x) + dmx, ey = (xaf.y - c[k].y) + dmy, ez = xaf.z - c[k].z;
                     d.nlistg[(size_t)lcnt * d.np + s] = sqrtf(ex * ex + ey * ey + ez * ez) - (xaf.w + c[k].w);
                 }
@@ -973,7 +973,7 @@ struct CellBox {
 };
 
 __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const float4* __restrict__ xf,
-                                        const uint32_t* __restrict__ sid, uint32_t* lo,
+                                        uint8_t* hint,
                                         uint32_t* list, float* gap, uint32_t np, uint32_t& lcnt, int gx, int gy, int gz, int nx,
                                         int ny, int nz, int periodic, float Lx, float Ly, uint32_t cbase,
                                         float4 xaf, float ra_skin, CellBox cb3) {
@@ -1012,7 +1012,7 @@ __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const 
                     if (d2 <= rl * rl) {
                         if (lcnt < (uint32_t)NLIST) {
                             list[(size_t)lcnt * np] = sb;
-                            lo[(size_t)lcnt * np] = sid[sb];
+                            hint[(size_t)lcnt * np] = 0xffu;
                             gap[(size_t)lcnt * np] = sqrtf(d2) - (xaf.w + cb.w);
                         }
                         ++lcnt;
@@ -1106,7 +1106,7 @@ __global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
     if (st.err) atomicOr(d.flags, st.err);
     if constexpr (BUILD) {  // list index = this sorted position
         if (d.list_shell && !st.stop)
-            list_shell(d.off, d.xf, d.sid, d.nlo + s, list, d.nlistg + s, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0],
+            list_shell(d.off, d.xf, d.nhint + s, list, d.nlistg + s, d.np, lcnt, gx, gy, gz, nx, ny, nz, S.periodic, S.Lf[0],
                        S.Lf[1], S.cbase, xaf, ra_skin, cb3);
         d.ncnt[s] = lcnt < (uint32_t)NLIST ? lcnt : (uint32_t)NLIST;
         d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
@@ -1387,8 +1387,10 @@ __device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_
 // An overlap whose partner already holds a slot is applied on the spot (its
 // slot does not depend on the order, a partner occurs once); the others are
 // collected and given empty slots in key order at the end.  Each list entry
-// keeps the slot its partner was last found in: one slot-word compare
-// confirms it, and only a stale or missing hint scans the occupied slots.
+// keeps the slot its partner was last found in, with the partner's original
+// index (both written when the search finds the partner in a slot; a build
+// marks them unknown): one slot-word compare confirms the slot, and only a
+// stale or unknown hint scans the occupied slots.
 // Held contacts are skipped outright where the search could only rewrite
 // their slot word unchanged.  That is so when no sphere crossed a periodic
 // face in the last step: the image through which two overlapping spheres
@@ -1413,7 +1415,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     const uint32_t li = d.slidx[s];
     const uint32_t* list = d.nlist + li;  // entry k at list[k * np]
     uint8_t* hint = d.nhint + li;
-    const uint32_t* lorig = d.nlo + li;
+    uint32_t* lorig = d.nlo + li;
     const uint32_t nl = d.ncnt[li];
     const uint32_t mask0 = d.ro.cmask[p];
     const bool px = S.periodic != 0, py = S.periodic == 1;
@@ -1421,14 +1423,21 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
     uint32_t n = 0, nnew = 0, err = 0, dirty = 0;
     bool have_a = false;
     double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
-    int gx, gy, gz;
-    cell_xyz(S, d.skey[s] - S.cbase, gx, gy, gz);
-    const bool interior = (!px || (gx > 0 && gx < (int)S.num[0] - 1)) && (!py || (gy > 0 && gy < (int)S.num[1] - 1));
+    int gx = 0, gy = 0, gz = 0;
+    bool have_cell = false;
+    bool skip_held = false;
     const bool no_wraps = d.nstate[4] == 0u;
-    const bool skip_held = interior || no_wraps;
     const uint32_t held_bits = no_wraps ? SLOT_IDX_MASK : 0xffffffffu;  // with wraps: the word must carry no image
     const SlotRow gslot = slot_row(d, s);
-    stage_slots(gslot, mask0, sw);
+    if (mask0) {
+        skip_held = no_wraps;
+        if (!no_wraps) {
+            cell_xyz(S, d.skey[s] - S.cbase, gx, gy, gz);
+            have_cell = true;
+            skip_held = (!px || (gx > 0 && gx < (int)S.num[0] - 1)) && (!py || (gy > 0 && gy < (int)S.num[1] - 1));
+        }
+        stage_slots(gslot, mask0, sw);
+    }
     // A pair whose gap at the build exceeds the two spheres' displacements
     // since then cannot overlap now: skip it without touching the partner's
     // record (margins of three FP32 prefilter widths for the rounding of the
@@ -1448,8 +1457,8 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
             const bool in = i0 + k < nl;
             qlb[k] = in ? list[(size_t)(i0 + k) * d.np] : li;
             qg[k] = in ? gap[(size_t)(i0 + k) * d.np] : 1e30f;
-            qhb[k] = in ? hint[(size_t)(i0 + k) * d.np] & 31u : 0u;
-            qlw[k] = in && skip_held ? lorig[(size_t)(i0 + k) * d.np] : 0xffffffffu;
+            qhb[k] = in && mask0 ? hint[(size_t)(i0 + k) * d.np] : 0xffu;
+            qlw[k] = in && mask0 && skip_held ? lorig[(size_t)(i0 + k) * d.np] : 0xffffffffu;
         }
     };
     fetch(0);
@@ -1465,12 +1474,16 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
             lw[k] = qlw[k];
         }
         if (i0 + LIST_BATCH < nl) fetch(i0 + LIST_BATCH);
-        uint32_t near = 0;
+        uint32_t held = 0, near = 0;
+        if (mask0) {
 #pragma unroll
-        for (int k = 0; k < LIST_BATCH; ++k) {
-            const bool held = ((mask0 >> hb[k]) & 1u) && (sw[hb[k] * APPLY_BLOCK] & held_bits) == lw[k];
-            near |= !held && g[k] <= dfar && g[k] <= da + d.ldisp[lb[k]] ? 1u << k : 0u;
+            for (int k = 0; k < LIST_BATCH; ++k)
+                held |= hb[k] < 32u && ((mask0 >> hb[k]) & 1u) && (sw[(hb[k] & 31u) * APPLY_BLOCK] & held_bits) == lw[k]
+                            ? 1u << k : 0u;
         }
+#pragma unroll
+        for (int k = 0; k < LIST_BATCH; ++k)
+            near |= !((held >> k) & 1u) && g[k] <= dfar && g[k] <= da + d.ldisp[lb[k]] ? 1u << k : 0u;
         if (near == 0u) continue;
 #pragma unroll
         for (int k = 0; k < LIST_BATCH; ++k) sb[k] = (near >> k) & 1u ? d.linv[lb[k]] : s;
@@ -1492,6 +1505,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
         if (pass == 0u) continue;
         if (!have_a) {
             xa = spos(d, S, s);
+            if (!have_cell) cell_xyz(S, d.skey[s] - S.cbase, gx, gy, gz);
             have_a = true;
         }
         // the batch's keys, FP64 positions and original indices, independent loads
@@ -1510,14 +1524,17 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
             if (v == 0ull) continue;
             ++n;
             int cpos = -1;
-            if (((mask0 >> hb[k]) & 1u) && (sw[hb[k] * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) {
+            if (hb[k] < 32u && ((mask0 >> hb[k]) & 1u) && (sw[(hb[k] & 31u) * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) {
                 cpos = (int)hb[k];
             } else {
                 for (uint32_t m = mask0; m; m &= m - 1) {
                     const int i = __ffs(m) - 1;
                     if ((sw[i * APPLY_BLOCK] & SLOT_IDX_MASK) == bo[k]) { cpos = i; break; }
                 }
-                if (cpos >= 0) hint[(size_t)(i0 + k) * d.np] = (uint8_t)cpos;
+                if (cpos >= 0) {  // the entry's hint and partner, written together
+                    hint[(size_t)(i0 + k) * d.np] = (uint8_t)cpos;
+                    lorig[(size_t)(i0 + k) * d.np] = bo[k];
+                }
             }
             if (cpos >= 0) {  // kept; written back only if its image changed
                 const uint32_t w = bo[k] | ((uint32_t)(v >> 32) & 15u) << 28;
@@ -1737,9 +1754,12 @@ constexpr int FORCE_BATCH = 4;
 // VW: the partners' velocity records are gathered with their positions, ahead
 // of the slots' processing (more registers, fewer waves: faster with many
 // contacts per sphere, slower with few; the host picks by timing).  M3:
-// contact model 3's pair force (FP64, one system) instead of model 2's.
+// contact model 3's pair force (FP64, one system) instead of model 2's.  The
+// one-system FP64 kernel without the gathers ahead is held to three waves per
+// SIMD (168 VGPRs, no spills): with two it loses 4-8 % on both a contact-free
+// packing and the dense crystal.
 template <class T, bool B, bool VW, bool M3 = false>
-__global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
+__global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 && !B && !VW && !M3) ? 3 : 1) void k_force(DemDev d) {
     static_assert(!M3 || (sizeof(T) == 8 && !B), "contact model 3: FP64, one system");
     const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= d.np) return;
@@ -1970,13 +1990,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(DemDev d) {
     d.key[s] = nkey;
     if (cerr) atomicOr(d.flags_next, cerr);
     count_key(d, s, nkey);
-    if (d.search_mode == 5 && S.periodic != 0) {
-        // a cell coordinate that jumps by more than one along a periodic axis:
-        // the sphere crossed the face (its contacts' image codes may change)
-        int ox, oy, oz, nx_, ny_, nz_;
-        cell_xyz(S, d.skey[s] - S.cbase, ox, oy, oz);
-        cell_xyz(S, nkey - S.cbase, nx_, ny_, nz_);
-        if (abs(nx_ - ox) > 1 || (S.periodic == 1 && abs(ny_ - oy) > 1)) atomicOr(&d.nstate[5], 1u);
+    if (d.search_mode == 5) {
+        // the next step wraps the new position: the sphere crossed a periodic
+        // face (the image through which its contacts meet may change)
+        double wx_ = px, wy_ = py;
+        if (wrap_xy(S, wx_, wy_)) atomicOr(&d.nstate[5], 1u);
     }
 
     // neighbour lists (search mode 5): the list index travels with the
