@@ -83,6 +83,19 @@ def test_lists_cover_pairs_beyond_the_build_scan(tmp_path, periodic, L):
     assert not errs, errs
 
 
+def test_lists_with_contacts_crossing_the_periodic_faces(tmp_path):
+    """A crystal (every nearest-neighbour pair in contact) drifting along x
+    and y through its periodic faces under the neighbour-list search: the
+    list steps skip held contacts, except after a step in which a sphere
+    crossed a face, when the image codes of its contacts change.  Contacts,
+    image codes and springs against the oracle, which re-tests every pair."""
+    _say("lists: drifting crystal through the periodic faces")
+    sb = configs.dense_lattice(cells=(6, 6, 4), sid="drift")
+    sb.vel[:, 0], sb.vel[:, 1] = 5.0, 3.0  # 0.09 and 0.05 mm per step; faces 7 mm from the nearest spheres
+    errs = run_pair(sb, tmp_path, [1, 49, 100, 150, 100], setup=lambda d: d.set_search(5))
+    assert not errs, errs
+
+
 @pytest.mark.parametrize("search", [2, 4, 5])
 def test_position_update_midway(tmp_path, search):
     """sph_set of positions between steps (cell ids and neighbour lists must
