@@ -1402,7 +1402,7 @@ __device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_
 // the word as it is).  Skipped entries are not counted; a contact overflow is
 // still caught when a new contact finds no empty slot.
 template <bool B>
-__global__ __launch_bounds__(APPLY_BLOCK) void k_list_search(DemDev d) {
+__global__ __launch_bounds__(APPLY_BLOCK, 4) void k_list_search(DemDev d) {  // four waves per SIMD: <= 128 VGPRs
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
     if (d.nstate[0] != 0u) return;  // k_topo_rows<true> rebuilds the lists this step
     const uint32_t s = xcd_block() * APPLY_BLOCK + threadIdx.x;
