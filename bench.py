@@ -410,6 +410,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--probe-only", action="store_true",
                     help="run only the copy-kernel bandwidth probe (counter calibration)")
+    ap.add_argument("--profile-leg", choices=["bonded", "ensemble"],
+                    help="run only this leg (for kernel profiles): the bonded solid for --bonded-steps, or the "
+                         "calibration ensemble for --ensemble-steps without the one-at-a-time comparison")
     ap.add_argument("--profile-dense", choices=["fp64", "mixed"],
                     help="run only the dense crystal in this precision (kernel profiles); prints its timings")
     args = ap.parse_args()
@@ -433,6 +436,14 @@ def main():
 
     prof_steps = min(args.steps, 50)
     tmp = tempfile.mkdtemp(prefix="sphere-bench-")
+    if args.profile_leg == "bonded":
+        print(json.dumps(bonded_run(tmp, args.bonded_steps, args.warmup, prof_steps)), flush=True)
+        return
+    if args.profile_leg == "ensemble":
+        members = configs.calibration_ensemble(members=args.ensemble_members, steps=args.ensemble_steps)
+        print(json.dumps(ensemble_run(members, args.ensemble_steps, max(args.warmup, 1), sample_every=len(members),
+                                      sample_steps=1)), flush=True)
+        return
     if args.profile_dense:
         sbd = configs.dense_lattice()
         dpath = os.path.join(tmp, "dense.bin")

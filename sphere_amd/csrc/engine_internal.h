@@ -1,5 +1,5 @@
 // engine_internal.h -- the handle behind the C-ABI, shared by the step
-// engine (engine.hip) and the adjoint driver (adjoint.hip).
+// engine (engine.hip) and the adjoint (adjoint.hip).
 #ifndef SPHERE_ENGINE_INTERNAL_H_
 #define SPHERE_ENGINE_INTERNAL_H_
 
