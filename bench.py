@@ -46,12 +46,12 @@ STAGE_KERNELS = {
     "bonds": "k_bonds", "cellid": "(ids, counts and ranks from k_force)", "sort": "k_scan_reduce/k_scan_down/k_scatter/k_cellsort",
     "reorder": "k_reorder", "search": "k_topo_rows", "force": "k_force", "walls": "k_wallsum/k_walls",
 }
-PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false>",
+PROFILE_KERNEL = {"reorder": "k_reorder<false>", "search": "k_topo_rows<false, false>",
                   "force": "k_force<double, false, false, false>"}
 FORCE_PROFILE = {1: "k_force<double, false, false, false>", 2: "k_force<double, false, true, false>"}
 SEARCH_KERNELS = {1: "k_topo", 2: "k_topo_rows", 3: "k_topo_wave", 4: "k_topo_half/k_topo_apply",
-                  5: "k_list_search (row search and list build on rebuild steps)"}
-SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false>", 3: "k_topo_wave", 4: "k_topo_half<false>",
+                  5: "k_list_search (k_topo_rows<true> on rebuild steps)"}
+SEARCH_PROFILE = {1: "k_topo", 2: "k_topo_rows<false, false>", 3: "k_topo_wave", 4: "k_topo_half<false>",
                   5: "k_list_search<false>"}
 
 

@@ -1023,12 +1023,16 @@ __device__ __noinline__ void list_shell(const uint32_t* __restrict__ off, const 
     }
 }
 
-// The row search of sorted particle s (sw: its LDS column of slot words,
-// stride blockDim.x).  BUILD (search mode 5, on steps whose neighbour lists
-// must be rebuilt): also rebuilds the particle's list.
+// BUILD (search mode 5): runs only on steps whose neighbour lists must be
+// rebuilt, and then also rebuilds the lists.
 template <bool BUILD, bool B>
-__device__ __forceinline__ void rows_one(const DemDev& d, uint32_t s, uint32_t* sw) {
+__global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
+    extern __shared__ uint32_t sw_all[];  // NC x blockDim: slot i of thread t at [i * blockDim + t]
+    if (BUILD && d.nstate[0] == 0u) return;
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= d.np) return;
     const Sys& S = sys_at<B>(d, s);
+    uint32_t* sw = sw_all + threadIdx.x;
     const int nx = (int)S.num[0], ny = (int)S.num[1], nz = (int)S.num[2];
     const uint32_t key = d.skey[s] - S.cbase;
     const int gx = (int)(key % S.num[0]), gy = (int)((key / S.num[0]) % S.num[1]), gz = (int)(key / (S.num[0] * S.num[1]));
@@ -1108,14 +1112,6 @@ __device__ __forceinline__ void rows_one(const DemDev& d, uint32_t s, uint32_t* 
         d.xref[s] = make_float4(xaf.x, xaf.y, xaf.z, 0.0f);
         if (lcnt > (uint32_t)NLIST || st.stop) atomicOr(&d.nstate[2], 1u);  // list unusable: rebuild next step
     }
-}
-
-template <bool B>
-__global__ __launch_bounds__(BLOCK) void k_topo_rows(DemDev d) {
-    extern __shared__ uint32_t sw_all[];  // NC x blockDim: slot i of thread t at [i * blockDim + t]
-    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= d.np) return;
-    rows_one<false, B>(d, s, sw_all + threadIdx.x);
 }
 
 // ---------------------------------------------------- half-shell search
@@ -1330,8 +1326,7 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_topo_apply(DemDev d) {
 }
 
 // ------------------------------------------------- neighbour-list search
-// Search mode 5.  On rebuild steps the list kernel runs the row search
-// (rows_one<true>) and rebuilds, per particle, the list of every
+// Search mode 5.  k_topo_rows<true> rebuilds, per particle, the list of every
 // sphere within r_a + r_b + skin (original indices) and keeps the position of
 // the build; the force kernel requests a rebuild once any sphere has moved
 // more than list_lim since.  list_lim is at most 0.45 skin, so no pair
@@ -1409,12 +1404,9 @@ __device__ __forceinline__ uint64_t list_pair(const Sys& S, uint32_t sb, uint32_
 template <bool B>
 __global__ __launch_bounds__(APPLY_BLOCK, 4) void k_list_search(DemDev d) {  // four waves per SIMD: <= 128 VGPRs
     __shared__ uint32_t sw_all[NC * APPLY_BLOCK];
+    if (d.nstate[0] != 0u) return;  // k_topo_rows<true> rebuilds the lists this step
     const uint32_t s = xcd_block() * APPLY_BLOCK + threadIdx.x;
     if (s >= d.np) return;
-    if (d.nstate[0] != 0u) {  // this step rebuilds the lists: the row search, writing them
-        rows_one<true, B>(d, s, sw_all + threadIdx.x);
-        return;
-    }
     const Sys& S = sys_at<B>(d, s);
     uint32_t* sw = sw_all + threadIdx.x;
     const uint32_t p = d.perm[s];
@@ -2173,12 +2165,13 @@ void launch_reorder(const DemDev& d, hipStream_t s) {
 template <bool B>
 void topology(const DemDev& d, hipStream_t s) {
     if (!B && d.sys.contactmodel == 1) return;  // contact model 1 keeps no slots (src/device.cu:809)
-    if (d.search_mode == 5) {  // the list search, or on rebuild steps (nstate[0]) the row search and build
+    if (d.search_mode == 5) {  // one of the two returns at once (nstate[0])
+        k_topo_rows<true, B><<<nblk(d.np, d.tpb_search), d.tpb_search, NC * 4 * d.tpb_search, s>>>(d);
         k_list_search<B><<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
     } else if (d.search_mode == 4) {  // hcnt starts at zero: k_topo_apply clears it
         k_topo_half<B><<<nblk(d.np, d.tpb_search), d.tpb_search, 0, s>>>(d);
         k_topo_apply<<<nblk(d.np, APPLY_BLOCK), APPLY_BLOCK, 0, s>>>(d);
-    } else if (d.search_mode == 2) k_topo_rows<B><<<nblk(d.np, d.tpb_search), d.tpb_search, NC * 4 * d.tpb_search, s>>>(d);
+    } else if (d.search_mode == 2) k_topo_rows<false, B><<<nblk(d.np, d.tpb_search), d.tpb_search, NC * 4 * d.tpb_search, s>>>(d);
     else if (B) return;  // (modes 1 and 3 walk one grid; the host refuses them for ensembles)
     else if (d.search_mode == 3) k_topo_wave<<<(d.K + BLOCK / 64 - 1) / (BLOCK / 64), BLOCK, 0, s>>>(d);
     else k_topo<<<d.tiles_x * d.tiles_y * d.sys.num[2], BLOCK, 0, s>>>(d);
